@@ -1,0 +1,160 @@
+/*
+ * zebsd.h — C ABI of the MI355X (gfx950) indexing hot path: libzebsd_hip.so.
+ *
+ * Stateless kernel entry points over caller-owned device memory. Every pointer here that is
+ * not a struct of parameters is a device pointer (hipMalloc'd, or a torch tensor's data_ptr);
+ * the structs themselves live in host memory and are read during the call. Every call enqueues
+ * work on `stream` (a hipStream_t, NULL = the null stream) and returns without waiting.
+ *
+ * Return value: 0 (or a size, for the *_workspace queries) on success; a negative code on
+ * failure, with the reason kept per thread and read by zb_last_error. This replaces the
+ * reference's engine-held error text (src/exports.zig:22-40, `ze_error`).
+ *
+ * The reference binds its path through an engine object (src/exports.zig, src/native_api.zig,
+ * python/zebsdsolve/_native.py:95-174). Its kernels are WGSL entry points recorded by
+ * src/spherical.zig and src/detector.zig; each entry point below names the recording function
+ * and the WGSL entry points it replaces. The engine-level surface (ze.Solver, ze.index) is
+ * mirrored above this ABI in Python (zebsdsolve_amd/__init__.py).
+ */
+#ifndef ZEBSD_H
+#define ZEBSD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define ZB_OK 0
+#define ZB_EINVAL -1   /* a bad argument: size, shape, null pointer */
+#define ZB_ESPACE -2   /* workspace too small */
+#define ZB_ELAUNCH -3  /* a kernel launch failed */
+
+/* Flags of zb_sh_coefficients (src/spherical.zig:96-98). */
+#define ZB_FLAG_CENTER 1 /* subtract the window mean */
+#define ZB_FLAG_NORM 2   /* scale to unit harmonic norm */
+#define ZB_FLAG_WINDOW 4 /* project the window itself (ones) instead of a pattern */
+
+/* The detector's attitude and circular mask in pixels of the patterns worked on
+ * (spherical.Frame, src/spherical.zig:80-90). */
+typedef struct zb_frame {
+    float c0[3]; /* detector row direction in the sample frame */
+    float c1[3]; /* column direction */
+    float c2[3]; /* normal */
+    int32_t pat_h, pat_w;
+    float mask[3]; /* centre row, centre column, squared radius (<= 0: no mask) */
+} zb_frame;
+
+/* What depends only on the bandwidth L (spherical.Tables, src/spherical.zig:106-190). Device
+ * pointers. n = fft_size(L): the sphere grid has n rings of 2n points, the orientation grid n
+ * points per turn. */
+typedef struct zb_band {
+    int32_t L;
+    int32_t n;
+    const float* ring;    /* [n][2]: cos(theta), quadrature weight */
+    const float* wplm;    /* [n/2][L(L+1)/2]: w_y P_l^m(cos theta_y) 2 pi / (2n) */
+    const float* dq;      /* [L][L][L]: d^l_{k,m}(pi/2) at (m L + l) L + k */
+    const float* twiddle; /* [1024][2]: exp(-2 pi i t / 1024) */
+} zb_band;
+
+/* What a phase adds at a bandwidth (spherical.PhaseGpu, src/spherical.zig:192-242). */
+typedef struct zb_phase {
+    int32_t fold;   /* 1, 2 or 4: the master's orders m are multiples of it */
+    int32_t nc;     /* n / fold: points of the c axis */
+    int32_t mu_max; /* (L - 1) / fold */
+    int32_t l_step; /* 2 when the master has no odd l, else 1 */
+    const float* w_f;  /* [2 mu_max + 1][L][L][2]: i^{-m} f_lm d^l_{km}(pi/2), k innermost */
+    const float* w_f2; /* the same for the master's square */
+} zb_phase;
+
+/* Gauss-Newton refinement options (detector.RefineOptions, src/detector.zig:57-70). */
+typedef struct zb_refine_opts {
+    int32_t iters;    /* Gauss-Newton iterations (0: score the seeds only) */
+    float max_step;   /* largest rotation of one iteration, radians */
+    float scale0;     /* curvature scale a refinement starts with */
+    float scale_min;
+    float scale_max;
+    int32_t nmi_bins; /* histogram levels of the normalized mutual information, 2..32 */
+    uint32_t stamp;   /* written into each record's last slot */
+} zb_refine_opts;
+
+/* Library version string and the last error of this thread (length written, without NUL). */
+const char* zb_version(void);
+int32_t zb_last_error(char* buf, int32_t cap);
+
+/* Patterns -> float32, zero mean and unit norm under the mask, zero outside it.
+ * raw: n patterns of h x w, pixel 0 u8, 1 u16, 2 f32; out: [n][h][w] f32.
+ * Replaces Prep.load + the `normalize` filter (src/prep.zig:143-204, 263-272; prep.wgsl:57-70,
+ * 399-443). */
+int32_t zb_normalize(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], float* out,
+                     void* stream);
+
+/* Bytes of workspace zb_sh_coefficients needs for n_items. */
+int64_t zb_sh_workspace(const zb_band* band, int32_t n_items);
+
+/* Harmonic coefficients of items: each pattern resampled onto the Gauss-Legendre sphere grid
+ * from its detector position, ring spectra, Legendre sums, norm.
+ * items: [n_items][4] f32 = (t in pixels, pattern index); pats: [*][h][w] f32 (unused with
+ * ZB_FLAG_WINDOW, may be NULL); alm: [n_items][L(L+1)/2][2] f32 out; stats: [n_items][4] out
+ * (window area, window mean, harmonic norm, a_00).
+ * Replaces Spherical.coefficients (src/spherical.zig:319-361: sphere.wgsl sphere_sums, rings,
+ * legendre, sh_norm). */
+int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame, const float* pats, const float* items,
+                           int32_t n_items, int32_t flags, void* work, int64_t work_bytes, float* alm, float* stats,
+                           void* stream);
+
+/* Bytes of workspace zb_so3_correlate / zb_so3_denominators need for n_items. */
+int64_t zb_so3_workspace(const zb_band* band, const zb_phase* phase, int32_t n_items);
+
+/* Best orientation of each item over SO(3): the Wigner-d contraction of its coefficients with
+ * the phase's weights, transforms along b, c and a, times the reciprocal denominator volume of
+ * item b / rden_div, the argmax (ties: lowest index) and its parabolic sub-cell peak.
+ * alm: [n_items][pairs][2]; rden: [*][n/2+1][nc][n] f32; peaks: [n_items][8] out =
+ * quaternion (w, x, y, z), score, a, b, c.
+ * Replaces Spherical.correlate (src/spherical.zig:401-432: so3.wgsl spectrum, a_peak_m,
+ * peak_reduce, peak_finish_m). */
+int32_t zb_so3_correlate(const zb_band* band, const zb_phase* phase, const float* alm, int32_t n_items,
+                         const float* rden, int32_t rden_div, void* work, int64_t work_bytes, float* peaks,
+                         void* stream);
+
+/* Reciprocal denominator volumes of the normalized correlation for n_windows window
+ * coefficient sets (zb_sh_coefficients with ZB_FLAG_WINDOW): 1 / sqrt(<w,f^2> - <w,f>^2 / area),
+ * 0 where that is <= 1e-6 area. w_alm: [n][pairs][2]; w_stats: [n][4]; rden: [n][n/2+1][nc][n] out.
+ * Replaces Spherical.denominators (src/spherical.zig:446-464: so3.wgsl a_volume, rden_build). */
+int32_t zb_so3_denominators(const zb_band* band, const zb_phase* phase, const float* w_alm, const float* w_stats,
+                            int32_t n_windows, void* work, int64_t work_bytes, float* rden, void* stream);
+
+/* Gauss-Newton refinement of each item's orientation on the detector, then its NMI, into one
+ * record per item: records[(out0 + b) * 8 ..] = best q (w, x, y, z), NCC, NMI, last improving
+ * step (radians), stamp. mgrad: [2][side][side][4] f32 (master gradient on the sphere, master
+ * value in .w); items: [n][4] (t, pattern index); seeds: orientation of item b at
+ * seeds[b * seed_stride .. + 4]; pats: normalized patterns [*][h][w].
+ * Replaces Detector.seed + Detector.refine (src/detector.zig:178-212: detector.wgsl seed,
+ * refine_init, sums_rot, refine_rot_step, nmi, collect), fused into one launch. */
+int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* items,
+                  const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* opts, float* records,
+                  int32_t out0, void* stream);
+
+/* Simulated patterns: the master (bilinear, [2][side][side] f32) in direction q . ray of every
+ * pixel. poses: [n][8] = (t, pattern index, q); out: [n][h][w].
+ * Replaces Detector.project (src/detector.zig:137-142: detector.wgsl project). */
+int32_t zb_project(const float* master, int32_t side, const zb_frame* frame, const float* poses, int32_t n,
+                   float* out, void* stream);
+
+/* One pass of Gaussian smoothing on the sphere of a [2][side][side] master: each texel the
+ * weighted mean over a 9 x 9 lattice of directions in its tangent plane, step 0.6 sigma_rad.
+ * Replaces one `sphere_blur` dispatch of Prep.sphereBlur (src/prep.zig:215-237;
+ * mastersph.wgsl:87-109). */
+int32_t zb_master_blur(const float* src, float* dst, int32_t side, float sigma_rad, void* stream);
+
+/* The master's gradient on the sphere at every texel by central differences one texel apart,
+ * with its value in .w: grad [2][side][side][4].
+ * Replaces Prep.gradient (src/prep.zig:253-260; mastersph.wgsl:124-138). */
+int32_t zb_master_gradient(const float* sheets, int32_t side, float* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

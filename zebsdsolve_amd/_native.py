@@ -1,0 +1,86 @@
+"""ctypes binding of libzebsd_hip.so (include/zebsd.h), the in-tree build of csrc/.
+
+The library is required: there is no CPU fallback. `load()` raises when it is missing, with the
+command that builds it.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzebsd_hip.so")
+
+f32p = ctypes.POINTER(ctypes.c_float)
+vp = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("c0", ctypes.c_float * 3), ("c1", ctypes.c_float * 3), ("c2", ctypes.c_float * 3),
+                ("pat_h", i32), ("pat_w", i32), ("mask", ctypes.c_float * 3)]
+
+
+class Band(ctypes.Structure):
+    _fields_ = [("L", i32), ("n", i32), ("ring", vp), ("wplm", vp), ("dq", vp), ("twiddle", vp)]
+
+
+class Phase(ctypes.Structure):
+    _fields_ = [("fold", i32), ("nc", i32), ("mu_max", i32), ("l_step", i32), ("w_f", vp), ("w_f2", vp)]
+
+
+class RefineOpts(ctypes.Structure):
+    _fields_ = [("iters", i32), ("max_step", ctypes.c_float), ("scale0", ctypes.c_float), ("scale_min", ctypes.c_float),
+                ("scale_max", ctypes.c_float), ("nmi_bins", i32), ("stamp", ctypes.c_uint32)]
+
+
+_SIGS = {
+    "zb_version": ([], ctypes.c_char_p),
+    "zb_last_error": ([ctypes.c_char_p, i32], i32),
+    "zb_normalize": ([vp, i32, i32, i32, i32, f32p, vp, vp], i32),
+    "zb_sh_workspace": ([ctypes.POINTER(Band), i32], i64),
+    "zb_sh_coefficients": ([ctypes.POINTER(Band), ctypes.POINTER(Frame), vp, vp, i32, i32, vp, i64, vp, vp, vp], i32),
+    "zb_so3_workspace": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), i32], i64),
+    "zb_so3_correlate": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, i32, vp, i32, vp, i64, vp, vp], i32),
+    "zb_so3_denominators": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, vp, i32, vp, i64, vp, vp], i32),
+    "zb_refine": ([vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
+    "zb_project": ([vp, i32, ctypes.POINTER(Frame), vp, i32, vp, vp], i32),
+    "zb_master_blur": ([vp, vp, i32, ctypes.c_float, vp], i32),
+    "zb_master_gradient": ([vp, i32, vp, vp], i32),
+}
+
+EXPORTS = tuple(_SIGS)
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    """A call into libzebsd_hip.so failed; the message is the library's."""
+
+
+def load(path=LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NativeError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "or `python -m zebsdsolve_amd._build` (there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def last_error():
+    lib = load()
+    buf = ctypes.create_string_buffer(1024)
+    n = lib.zb_last_error(buf, 1024)
+    return buf.raw[:min(n, 1023)].decode(errors="replace")
+
+
+def check(rc, what):
+    if rc < 0:
+        raise NativeError(f"{what}: {last_error() or rc}")
+    return rc

@@ -1,0 +1,209 @@
+// Shared device code for the indexing kernels (gfx950, wave64): float32 rotations, the
+// equal-area map, and an in-place LDS line FFT.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/zebsd.h"
+
+#define ZB_PI 3.14159265358979f
+#define ZB_TAU 6.28318530717959f
+// twiddle table length: tw[t] = exp(-2 pi i t / ZB_TW)
+#define ZB_TW 1024
+
+namespace zb {
+
+// ── error reporting (api.hip) ──
+void set_error(const char* fmt, ...);
+int check_launch(const char* what);
+
+// ── quaternions, (w, x, y, z) in (.x, .y, .z, .w), acting as q v q* ──
+__device__ __forceinline__ float4 qmul(float4 a, float4 b) {
+    return make_float4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w,
+                       a.x * b.y + a.y * b.x + a.z * b.w - a.w * b.z,
+                       a.x * b.z - a.y * b.w + a.z * b.x + a.w * b.y,
+                       a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
+}
+
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ __forceinline__ float3 qrot(float4 q, float3 v) {
+    float3 u = make_float3(q.y, q.z, q.w);
+    float3 t = cross3(u, v);
+    t = make_float3(2.f * t.x, 2.f * t.y, 2.f * t.z);
+    float3 c = cross3(u, t);
+    return make_float3(v.x + q.x * t.x + c.x, v.y + q.x * t.y + c.y, v.z + q.x * t.z + c.z);
+}
+
+__device__ __forceinline__ float4 qnormalize(float4 q) {
+    float n = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    if (n > 0.f) return make_float4(q.x / n, q.y / n, q.z / n, q.w / n);
+    return make_float4(1.f, 0.f, 0.f, 0.f);
+}
+
+// exp of a rotation vector (common.wgsl qexp)
+__device__ __forceinline__ float4 qexp(float3 p) {
+    float th2 = dot3(p, p);
+    float th = sqrtf(th2);
+    float k = 0.5f - th2 / 48.f;
+    if (th > 1e-3f) k = sinf(0.5f * th) / th;
+    return make_float4(cosf(0.5f * th), k * p.x, k * p.y, k * p.z);
+}
+
+__device__ __forceinline__ float3 normalize3(float3 v) {
+    float r = 1.f / sqrtf(dot3(v, v));
+    return make_float3(v.x * r, v.y * r, v.z * r);
+}
+
+// Unit vector -> equal-area square coordinates in [-1, 1]^2 of its own hemisphere. 1 - |z| is
+// taken as (x^2 + y^2) / (1 + |z|), which keeps its digits near the pole (common.wgsl:44-57).
+__device__ __forceinline__ float2 sphere_to_square(float3 v) {
+    float ax = fabsf(v.x), ay = fabsf(v.y);
+    float xn = sqrtf(fmaxf(ZB_PI * 0.5f * (ax * ax + ay * ay) / (1.f + fabsf(v.z)), 0.f));
+    float u = xn, w = xn;
+    if (ax >= ay) {
+        w = (4.f * xn / ZB_PI) * atan2f(ay, fmaxf(ax, 1e-30f));
+    } else {
+        u = (4.f * xn / ZB_PI) * atan2f(ax, fmaxf(ay, 1e-30f));
+    }
+    const float sc = 0.79788456080286536f;  // 1 / sqrt(pi / 2)
+    return make_float2(u * sc * (v.x < 0.f ? -1.f : 1.f), w * sc * (v.y < 0.f ? -1.f : 1.f));
+}
+
+// Square coordinates -> unit vector on the northern or southern hemisphere (f32).
+__device__ __forceinline__ float3 square_to_sphere(float2 p, bool south) {
+    const float sc = 1.2533141373155003f;  // sqrt(pi / 2)
+    float xa = fabsf(p.x) * sc, ya = fabsf(p.y) * sc;
+    bool swap = xa >= ya;
+    float xn = fmaxf(xa, ya), yn = fminf(xa, ya);
+    float3 v = make_float3(0.f, 0.f, 1.f);
+    if (xn > 0.f) {
+        float r = (2.f * xn / ZB_PI) * sqrtf(fmaxf(ZB_PI - xn * xn, 0.f));
+        float ang = ZB_PI * yn / (4.f * xn);
+        float hx = r * cosf(ang), hy = r * sinf(ang);
+        if (!swap) { float t = hx; hx = hy; hy = t; }
+        v = normalize3(make_float3(hx * (p.x < 0.f ? -1.f : 1.f), hy * (p.y < 0.f ? -1.f : 1.f), 1.f - 2.f * xn * xn / ZB_PI));
+    }
+    if (south) v.z = -v.z;
+    return v;
+}
+
+// Bilinear lookup of a [north | south] Lambert sheet pair of side x side texels of T (float or
+// float4) in direction v: the texel row/column clamps as the reference's master_at does.
+template <typename T>
+__device__ __forceinline__ T sheet_bilinear(const T* __restrict__ s, int side, float3 v);
+
+template <>
+__device__ __forceinline__ float sheet_bilinear<float>(const float* __restrict__ s, int side, float3 v) {
+    float2 uv = sphere_to_square(v);
+    float last = (float)(side - 1);
+    float fj = fminf(fmaxf((uv.x + 1.f) * 0.5f * last, 0.f), last);
+    float fi = fminf(fmaxf((uv.y + 1.f) * 0.5f * last, 0.f), last);
+    int c0 = min((int)fj, side - 2), r0 = min((int)fi, side - 2);
+    int o = r0 * side + c0 + (v.z < 0.f ? side * side : 0);
+    float tj = fj - (float)c0, ti = fi - (float)r0;
+    float a = s[o], b = s[o + 1], c = s[o + side], d = s[o + side + 1];
+    float top = a + (b - a) * tj, bot = c + (d - c) * tj;
+    return top + (bot - top) * ti;
+}
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+
+template <>
+__device__ __forceinline__ float4 sheet_bilinear<float4>(const float4* __restrict__ s, int side, float3 v) {
+    float2 uv = sphere_to_square(v);
+    float last = (float)(side - 1);
+    float fj = fminf(fmaxf((uv.x + 1.f) * 0.5f * last, 0.f), last);
+    float fi = fminf(fmaxf((uv.y + 1.f) * 0.5f * last, 0.f), last);
+    int c0 = min((int)fj, side - 2), r0 = min((int)fi, side - 2);
+    int o = r0 * side + c0 + (v.z < 0.f ? side * side : 0);
+    float tj = fj - (float)c0, ti = fi - (float)r0;
+    float4 a = s[o], b = s[o + 1], c = s[o + side], d = s[o + side + 1];
+    float4 top = make_float4(lerpf(a.x, b.x, tj), lerpf(a.y, b.y, tj), lerpf(a.z, b.z, tj), lerpf(a.w, b.w, tj));
+    float4 bot = make_float4(lerpf(c.x, d.x, tj), lerpf(c.y, d.y, tj), lerpf(c.z, d.z, tj), lerpf(c.w, d.w, tj));
+    return make_float4(lerpf(top.x, bot.x, ti), lerpf(top.y, bot.y, ti), lerpf(top.z, bot.z, ti), lerpf(top.w, bot.w, ti));
+}
+
+// ── complex helpers ──
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (-i)
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+// i^p x
+__device__ __forceinline__ float2 ipow(float2 x, int p) {
+    switch (p & 3) {
+        case 1: return make_float2(-x.y, x.x);
+        case 2: return make_float2(-x.x, -x.y);
+        case 3: return make_float2(x.y, -x.x);
+        default: return x;
+    }
+}
+
+__device__ __forceinline__ int bitrev(int x, int bits) { return (int)(__brev((unsigned)x) >> (32 - bits)); }
+
+// In-place forward (e^{-i...}) radix-2 decimation-in-frequency FFT of `nlines` lines of 2^log2n
+// points held in LDS: point p of line l at buf[l * lstride + p * pstride]. Natural order in,
+// bit-reversed order out. Stages run in pairs as radix-4 butterflies (a thread takes four points
+// through both stages in registers); an odd last stage runs alone. All NT threads of the block
+// call it; it ends with a barrier.
+template <int NT>
+__device__ void fft_dif(float2* buf, int log2n, int nlines, int lstride, int pstride, const float2* __restrict__ tw, int tid) {
+    int s = log2n;
+    const int nq = 1 << (log2n - 2 < 0 ? 0 : log2n - 2);
+    for (; s >= 2; s -= 2) {
+        const int lq = s - 2;
+        const int q = 1 << lq;
+        const int t4 = ZB_TW >> s;        // W_{4q}^j  = tw[j * t4]
+        const int t2 = ZB_TW >> (s - 1);  // W_{2q}^j  = tw[j * t2]
+        const int total = nlines * nq;
+        for (int e = tid; e < total; e += NT) {
+            int line = e >> (log2n - 2);
+            int r = e & (nq - 1);
+            int j = r & (q - 1);
+            int i0 = ((r >> lq) << s) + j;
+            float2* b = buf + line * lstride;
+            float2 x0 = b[i0 * pstride];
+            float2 x1 = b[(i0 + q) * pstride];
+            float2 x2 = b[(i0 + 2 * q) * pstride];
+            float2 x3 = b[(i0 + 3 * q) * pstride];
+            float2 w4 = tw[j * t4];
+            float2 w2 = tw[j * t2];
+            float2 a0 = cadd(x0, x2);
+            float2 a2 = cmul(csub(x0, x2), w4);
+            float2 a1 = cadd(x1, x3);
+            float2 a3 = cmul(cmul_mi(csub(x1, x3)), w4);
+            b[i0 * pstride] = cadd(a0, a1);
+            b[(i0 + q) * pstride] = cmul(csub(a0, a1), w2);
+            b[(i0 + 2 * q) * pstride] = cadd(a2, a3);
+            b[(i0 + 3 * q) * pstride] = cmul(csub(a2, a3), w2);
+        }
+        __syncthreads();
+    }
+    if (s == 1) {
+        const int half = 1 << (log2n - 1);
+        const int total = nlines * half;
+        for (int e = tid; e < total; e += NT) {
+            int line = e >> (log2n - 1);
+            int r = e & (half - 1);
+            float2* b = buf + line * lstride;
+            float2 x0 = b[(2 * r) * pstride];
+            float2 x1 = b[(2 * r + 1) * pstride];
+            b[(2 * r) * pstride] = cadd(x0, x1);
+            b[(2 * r + 1) * pstride] = csub(x0, x1);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ float warp_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace zb

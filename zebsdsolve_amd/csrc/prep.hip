@@ -1,0 +1,196 @@
+// Patterns in, masters baked: per-pattern normalization under the mask, simulated patterns,
+// and the master's smoothing and gradient on the sphere (setup, once per master and geometry).
+#include "common.h"
+
+namespace zb {
+
+template <int NT>
+__device__ float block_sum(float v, float* red) {
+    v = warp_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    float t = 0.f;
+    for (int i = 0; i < NT / 64; ++i) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ bool in_mask(float row, float col, const float* m) {
+    float dr = row - m[0], dc = col - m[1];
+    return m[2] <= 0.f || dr * dr + dc * dc <= m[2];
+}
+
+struct MaskArg { float m[3]; };
+
+// One block of 256 per pattern: mean and norm under the mask in two passes (prep.wgsl:405-443).
+template <typename T>
+__global__ void __launch_bounds__(256) k_normalize(const T* __restrict__ raw, int h, int w, MaskArg mask, float* __restrict__ out) {
+    __shared__ float red[4];
+    const int b = blockIdx.x;
+    const int hw = h * w;
+    const T* src = raw + (size_t)b * hw;
+    float* dst = out + (size_t)b * hw;
+    float s = 0.f, c = 0.f;
+    for (int i = threadIdx.x; i < hw; i += 256) {
+        if (in_mask((float)(i / w), (float)(i % w), mask.m)) {
+            s += (float)src[i];
+            c += 1.f;
+        }
+    }
+    s = block_sum<256>(s, red);
+    c = block_sum<256>(c, red);
+    const float mean = s / fmaxf(c, 1.f);
+    float ss = 0.f;
+    for (int i = threadIdx.x; i < hw; i += 256) {
+        float v = 0.f;
+        if (in_mask((float)(i / w), (float)(i % w), mask.m)) v = (float)src[i] - mean;
+        dst[i] = v;
+        ss += v * v;
+    }
+    ss = block_sum<256>(ss, red);
+    const float inv = 1.f / sqrtf(fmaxf(ss, 1e-30f));
+    for (int i = threadIdx.x; i < hw; i += 256) dst[i] *= inv;
+}
+
+struct FrameArg {
+    float c0[3], c1[3], c2[3];
+    int h, w;
+    float mask[3];
+};
+
+static FrameArg frame_arg(const zb_frame* f) {
+    FrameArg a;
+    for (int i = 0; i < 3; ++i) {
+        a.c0[i] = f->c0[i];
+        a.c1[i] = f->c1[i];
+        a.c2[i] = f->c2[i];
+        a.mask[i] = f->mask[i];
+    }
+    a.h = f->pat_h;
+    a.w = f->pat_w;
+    return a;
+}
+
+// Simulated pattern pixels: the master in direction q . normalize(ray) (detector.wgsl:254-263).
+__global__ void __launch_bounds__(256) k_project(const float* __restrict__ master, int side, FrameArg fr, const float* __restrict__ poses, int n, float* __restrict__ out) {
+    const int hw = fr.h * fr.w;
+    const long tid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (tid >= (long)n * hw) return;
+    const int b = (int)(tid / hw);
+    const int pix = (int)(tid - (long)b * hw);
+    const float row = (float)(pix / fr.w), col = (float)(pix % fr.w);
+    const float* p = poses + (size_t)b * 8;
+    float3 u = make_float3(fr.c0[0] * row + fr.c1[0] * col + p[0], fr.c0[1] * row + fr.c1[1] * col + p[1], fr.c0[2] * row + fr.c1[2] * col + p[2]);
+    float4 q = make_float4(p[4], p[5], p[6], p[7]);
+    out[tid] = sheet_bilinear<float>(master, side, qrot(q, normalize3(u)));
+}
+
+__device__ __forceinline__ float3 texel_dir(int tid, int side) {
+    const int plane = side * side;
+    const int pix = tid % plane;
+    const float last = (float)(side - 1);
+    float2 p = make_float2(2.f * (float)(pix % side) / last - 1.f, 2.f * (float)(pix / side) / last - 1.f);
+    return square_to_sphere(p, tid >= plane);
+}
+
+__device__ __forceinline__ void tangent_frame(float3 d, float3& e1, float3& e2) {
+    float3 ref = fabsf(d.x) > 0.8f ? make_float3(0.f, 1.f, 0.f) : make_float3(1.f, 0.f, 0.f);
+    e1 = normalize3(cross3(d, ref));
+    e2 = cross3(d, e1);
+}
+
+// Gaussian on the sphere, one pass: a 9 x 9 lattice in each texel's tangent plane
+// (mastersph.wgsl:87-109).
+__global__ void __launch_bounds__(256) k_blur(const float* __restrict__ src, float* __restrict__ dst, int side, float step, float k2) {
+    const int tid = blockIdx.x * 256 + threadIdx.x;
+    if (tid >= 2 * side * side) return;
+    float3 d = texel_dir(tid, side);
+    float3 e1, e2;
+    tangent_frame(d, e1, e2);
+    float acc = 0.f, wsum = 0.f;
+    for (int i = -4; i <= 4; ++i) {
+        for (int j = -4; j <= 4; ++j) {
+            float a = (float)i * step, b = (float)j * step;
+            float w = expf(-(a * a + b * b) * k2);
+            float3 v = normalize3(make_float3(d.x + a * e1.x + b * e2.x, d.y + a * e1.y + b * e2.y, d.z + a * e1.z + b * e2.z));
+            acc += w * sheet_bilinear<float>(src, side, v);
+            wsum += w;
+        }
+    }
+    dst[tid] = acc / wsum;
+}
+
+// (grad M, M) at every texel by central differences one texel apart (mastersph.wgsl:124-138).
+__global__ void __launch_bounds__(256) k_gradient(const float* __restrict__ src, int side, float h, float4* __restrict__ grad) {
+    const int tid = blockIdx.x * 256 + threadIdx.x;
+    if (tid >= 2 * side * side) return;
+    float3 c = texel_dir(tid, side);
+    float3 e1, e2;
+    tangent_frame(c, e1, e2);
+    float g1 = (sheet_bilinear<float>(src, side, normalize3(make_float3(c.x + h * e1.x, c.y + h * e1.y, c.z + h * e1.z))) -
+                sheet_bilinear<float>(src, side, normalize3(make_float3(c.x - h * e1.x, c.y - h * e1.y, c.z - h * e1.z)))) / (2.f * h);
+    float g2 = (sheet_bilinear<float>(src, side, normalize3(make_float3(c.x + h * e2.x, c.y + h * e2.y, c.z + h * e2.z))) -
+                sheet_bilinear<float>(src, side, normalize3(make_float3(c.x - h * e2.x, c.y - h * e2.y, c.z - h * e2.z)))) / (2.f * h);
+    grad[tid] = make_float4(g1 * e1.x + g2 * e2.x, g1 * e1.y + g2 * e2.y, g1 * e1.z + g2 * e2.z, src[tid]);
+}
+
+}  // namespace zb
+
+using namespace zb;
+
+extern "C" int32_t zb_normalize(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], float* out,
+                                void* stream) {
+    if (n < 0 || h < 2 || w < 2 || (n > 0 && (!raw || !out)) || !mask || pixel < 0 || pixel > 2) {
+        set_error("zb_normalize: bad arguments (n %d, h %d, w %d, pixel %d)", n, h, w, pixel);
+        return ZB_EINVAL;
+    }
+    if (n == 0) return ZB_OK;
+    MaskArg m;
+    m.m[0] = mask[0];
+    m.m[1] = mask[1];
+    m.m[2] = mask[2];
+    hipStream_t s = (hipStream_t)stream;
+    if (pixel == 0)
+        hipLaunchKernelGGL(k_normalize<uint8_t>, dim3(n), dim3(256), 0, s, (const uint8_t*)raw, h, w, m, out);
+    else if (pixel == 1)
+        hipLaunchKernelGGL(k_normalize<uint16_t>, dim3(n), dim3(256), 0, s, (const uint16_t*)raw, h, w, m, out);
+    else
+        hipLaunchKernelGGL(k_normalize<float>, dim3(n), dim3(256), 0, s, (const float*)raw, h, w, m, out);
+    return check_launch("zb_normalize");
+}
+
+extern "C" int32_t zb_project(const float* master, int32_t side, const zb_frame* frame, const float* poses, int32_t n, float* out,
+                              void* stream) {
+    if (!frame || side < 2 || n < 0 || frame->pat_h < 2 || frame->pat_w < 2 || (n > 0 && (!master || !poses || !out))) {
+        set_error("zb_project: bad arguments");
+        return ZB_EINVAL;
+    }
+    if (n == 0) return ZB_OK;
+    FrameArg fr = frame_arg(frame);
+    long total = (long)n * fr.h * fr.w;
+    hipLaunchKernelGGL(k_project, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, master, side, fr, poses, n, out);
+    return check_launch("zb_project");
+}
+
+extern "C" int32_t zb_master_blur(const float* src, float* dst, int32_t side, float sigma_rad, void* stream) {
+    if (!src || !dst || side < 2 || !(sigma_rad > 0.f)) {
+        set_error("zb_master_blur: bad arguments");
+        return ZB_EINVAL;
+    }
+    int total = 2 * side * side;
+    hipLaunchKernelGGL(k_blur, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, src, dst, side, 0.6f * sigma_rad,
+                       1.f / (2.f * sigma_rad * sigma_rad));
+    return check_launch("zb_master_blur");
+}
+
+extern "C" int32_t zb_master_gradient(const float* sheets, int32_t side, float* grad, void* stream) {
+    if (!sheets || !grad || side < 2) {
+        set_error("zb_master_gradient: bad arguments");
+        return ZB_EINVAL;
+    }
+    int total = 2 * side * side;
+    float h = (float)(2.5066282746310002 / (double)side);  // sqrt(2 pi) / side
+    hipLaunchKernelGGL(k_gradient, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, sheets, side, h, (float4*)grad);
+    return check_launch("zb_master_gradient");
+}

@@ -1,0 +1,267 @@
+// Patterns onto the sphere and into spherical harmonics (the reference's sphere.wgsl stage,
+// src/spherical.zig:319-361), for gfx950.
+//
+// An item is one (detector position t, pattern) pair. The sphere is sampled on a Gauss-Legendre
+// grid of n rings by 2n points in the sample frame: each grid direction's ray meets the detector
+// plane, and the bilinear pattern value there counts where it lands on the detector and inside
+// the mask (the window). Four kernels:
+//   k_sums     per item, the window's quadrature sums (for its mean)
+//   k_rings    ring y with its mirror n-1-y as one complex line (real: the ring, imaginary: the
+//              mirror), mean-subtracted and windowed, FFT along phi in LDS, split apart; of the
+//              orders m < L the sum and difference of the two rings' spectra are kept
+//   k_legendre a_lm = sum_y wplm[y, pair] (sum or difference by the parity of l + m), a tile of
+//              items a block so the Legendre table is read once per tile
+//   k_norm     unit harmonic norm and the item's stats
+// Workspace: per item the window sums (float2) and the ring spectra [L][n/2] float4 (m-major,
+// so that k_legendre reads them contiguously).
+#include "common.h"
+
+namespace zb {
+
+struct SphArg {
+    float c0[3], c1[3], c2[3];
+    float mask[3];
+    int h, w;
+    int L, n, log2nphi, pairs, flags;
+};
+
+// Grid point (ring y, point j) of an item: (pattern value, window) (sphere.wgsl:67-101).
+__device__ __forceinline__ float2 sample_point(const SphArg& a, const float* __restrict__ ring, const float2* __restrict__ tw,
+                                               const float* __restrict__ pats, float4 it, int y, int j) {
+    const float ct = ring[2 * y];
+    const float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+    const float2 e = tw[j << (10 - a.log2nphi)];  // exp(-i phi_j)
+    const float3 d = make_float3(st * e.x, -st * e.y, ct);
+    const float nd = a.c2[0] * d.x + a.c2[1] * d.y + a.c2[2] * d.z;
+    float val = 0.f, w = 0.f;
+    if (nd > 1e-4f) {
+        const float s = (a.c2[0] * it.x + a.c2[1] * it.y + a.c2[2] * it.z) / nd;
+        const float hx = s * d.x - it.x, hy = s * d.y - it.y, hz = s * d.z - it.z;
+        const float row = a.c0[0] * hx + a.c0[1] * hy + a.c0[2] * hz;
+        const float col = a.c1[0] * hx + a.c1[1] * hy + a.c1[2] * hz;
+        const float dr = row - a.mask[0], dc = col - a.mask[1];
+        if (row >= 0.f && row <= (float)(a.h - 1) && col >= 0.f && col <= (float)(a.w - 1) &&
+            (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2])) {
+            w = 1.f;
+            val = 1.f;
+            if ((a.flags & ZB_FLAG_WINDOW) == 0) {
+                const int r0 = min((int)row, a.h - 2), q0 = min((int)col, a.w - 2);
+                const float fr = row - (float)r0, fc = col - (float)q0;
+                const float* p = pats + (size_t)(int)it.w * a.h * a.w + r0 * a.w + q0;
+                const float top = lerpf(p[0], p[1], fc);
+                const float bot = lerpf(p[a.w], p[a.w + 1], fc);
+                val = lerpf(top, bot, fr);
+            }
+        }
+    }
+    return make_float2(val, w);
+}
+
+// Per item: sum of w wq value and of w wq over its whole grid (sphere.wgsl:114-151).
+__global__ void __launch_bounds__(256) k_sums(SphArg a, const float* __restrict__ ring, const float2* __restrict__ tw,
+                                              const float* __restrict__ pats, const float4* __restrict__ items, float2* __restrict__ sums) {
+    __shared__ float2 red[4];
+    const int b = blockIdx.x;
+    const float4 it = items[b];
+    const int nphi = 1 << a.log2nphi;
+    const int total = a.n * nphi;
+    float sv = 0.f, sw = 0.f;
+    for (int p = threadIdx.x; p < total; p += 256) {
+        const int y = p >> a.log2nphi;
+        const int j = p & (nphi - 1);
+        const float2 g = sample_point(a, ring, tw, pats, it, y, j);
+        const float wq = ring[2 * y + 1];
+        sv += g.x * g.y * wq;
+        sw += g.y * wq;
+    }
+    sv = warp_sum(sv);
+    sw = warp_sum(sw);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_float2(sv, sw);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float2 t = red[0];
+        for (int i = 1; i < 4; ++i) t = make_float2(t.x + red[i].x, t.y + red[i].y);
+        sums[b] = t;
+    }
+}
+
+__device__ __forceinline__ float window_mean(const SphArg& a, float2 tot) {
+    return (a.flags & ZB_FLAG_CENTER) ? tot.x / fmaxf(tot.y, 1e-20f) : 0.f;
+}
+
+// Ring pairs y0 .. y0 + G - 1 of one item: sample, FFT along phi, keep m < L (sphere.wgsl:157-207).
+__global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __restrict__ ring, const float2* __restrict__ tw,
+                                               const float* __restrict__ pats, const float4* __restrict__ items,
+                                               const float2* __restrict__ sums, float4* __restrict__ spectra) {
+    extern __shared__ float2 buf[];
+    const int nh = a.n >> 1;
+    const int groups = nh / G;
+    const int b = blockIdx.x / groups;
+    const int y0 = (blockIdx.x - b * groups) * G;
+    const int nphi = 1 << a.log2nphi;
+    const float4 it = items[b];
+    const float mean = window_mean(a, sums[b]);
+    for (int e = threadIdx.x; e < G * nphi; e += 256) {
+        const int i = e >> a.log2nphi;
+        const int j = e & (nphi - 1);
+        const int y = y0 + i;
+        const float2 gn = sample_point(a, ring, tw, pats, it, y, j);
+        const float2 gs = sample_point(a, ring, tw, pats, it, a.n - 1 - y, j);
+        buf[e] = make_float2((gn.x - mean) * gn.y, (gs.x - mean) * gs.y);
+    }
+    __syncthreads();
+    fft_dif<256>(buf, a.log2nphi, G, nphi, 1, tw, threadIdx.x);
+    // Z = N + i S for real N, S: N_m = (Z_m + conj Z_{-m}) / 2, S_m = (Z_m - conj Z_{-m}) / 2i
+    const int L = a.L;
+    for (int e = threadIdx.x; e < G * L; e += 256) {
+        const int m = e / G;
+        const int i = e - m * G;
+        const float2 z = buf[i * nphi + bitrev(m, a.log2nphi)];
+        const float2 zr = buf[i * nphi + bitrev((nphi - m) & (nphi - 1), a.log2nphi)];
+        const float2 rn = make_float2(0.5f * (z.x + zr.x), 0.5f * (z.y - zr.y));
+        const float2 rs = make_float2(0.5f * (z.y + zr.y), -0.5f * (z.x - zr.x));
+        spectra[((size_t)b * L + m) * nh + y0 + i] = make_float4(rn.x + rs.x, rn.y + rs.y, rn.x - rs.x, rn.y - rs.y);
+    }
+}
+
+// a_lm for a tile of TB items at one order m: thread l - m (sphere.wgsl:209-226).
+template <int TB>
+__global__ void __launch_bounds__(64) k_legendre(SphArg a, int n_items, const float* __restrict__ wplm, const float4* __restrict__ spectra,
+                                                 float2* __restrict__ alm) {
+    extern __shared__ float4 sp[];
+    const int nh = a.n >> 1;
+    const int L = a.L;
+    const int m = blockIdx.y;
+    const int b0 = blockIdx.x * TB;
+    for (int e = threadIdx.x; e < TB * nh; e += 64) {
+        const int t = e / nh;
+        const int y = e - t * nh;
+        const int b = b0 + t;
+        sp[e] = b < n_items ? spectra[((size_t)b * L + m) * nh + y] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    const int base = m * L - (m * (m - 1)) / 2 - m;  // pair(m, l) = base + l
+    for (int l = m + threadIdx.x; l < L; l += 64) {
+        const int p = base + l;
+        const bool odd = ((l + m) & 1) != 0;
+        float2 acc[TB];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) acc[t] = make_float2(0.f, 0.f);
+        for (int y = 0; y < nh; ++y) {
+            const float wv = wplm[(size_t)y * a.pairs + p];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                const float4 s = sp[t * nh + y];
+                acc[t].x += wv * (odd ? s.z : s.x);
+                acc[t].y += wv * (odd ? s.w : s.y);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TB; ++t)
+            if (b0 + t < n_items) alm[(size_t)(b0 + t) * a.pairs + p] = acc[t];
+    }
+}
+
+// Unit norm sqrt(sum_m k |a|^2), k = 1 for m = 0 else 2, and the stats (sphere.wgsl:228-262).
+__global__ void __launch_bounds__(256) k_norm(SphArg a, const float2* __restrict__ sums, float2* __restrict__ alm, float* __restrict__ stats) {
+    __shared__ float red[4];
+    const int b = blockIdx.x;
+    float2* c = alm + (size_t)b * a.pairs;
+    float e = 0.f;
+    for (int p = threadIdx.x; p < a.pairs; p += 256) {
+        const float2 v = c[p];
+        e += (p < a.L ? 1.f : 2.f) * (v.x * v.x + v.y * v.y);
+    }
+    e = warp_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    const float tot = red[0] + red[1] + red[2] + red[3];
+    const float nrm = sqrtf(fmaxf(tot, 1e-30f));
+    if (a.flags & ZB_FLAG_NORM) {
+        for (int p = threadIdx.x; p < a.pairs; p += 256) c[p] = make_float2(c[p].x / nrm, c[p].y / nrm);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float2 t = sums[b];
+        stats[4 * b] = t.y * ZB_TAU / (float)(1 << a.log2nphi);
+        stats[4 * b + 1] = window_mean(a, t);
+        stats[4 * b + 2] = nrm;
+        stats[4 * b + 3] = c[0].x;
+    }
+}
+
+}  // namespace zb
+
+using namespace zb;
+
+static int64_t sh_ws(int L, int n, int n_items) {
+    const int64_t nh = n / 2;
+    int64_t sums = ((int64_t)n_items * 8 + 255) / 256 * 256;
+    return sums + (int64_t)n_items * L * nh * 16;
+}
+
+extern "C" int64_t zb_sh_workspace(const zb_band* band, int32_t n_items) {
+    if (!band || band->L < 2 || n_items < 0) return ZB_EINVAL;
+    return sh_ws(band->L, band->n, n_items);
+}
+
+static int ilog2(int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return (1 << r) == v ? r : -1;
+}
+
+extern "C" int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame, const float* pats, const float* items,
+                                      int32_t n_items, int32_t flags, void* work, int64_t work_bytes, float* alm, float* stats,
+                                      void* stream) {
+    if (!band || !frame || n_items < 0 || band->L < 2 || ilog2(band->n) < 4 || band->n < 2 * band->L - 1 || 2 * band->n > ZB_TW ||
+        frame->pat_h < 2 || frame->pat_w < 2) {
+        set_error("zb_sh_coefficients: bad band or frame");
+        return ZB_EINVAL;
+    }
+    if (n_items == 0) return ZB_OK;
+    if (!items || !alm || !stats || !work || !band->ring || !band->wplm || !band->twiddle || (!(flags & ZB_FLAG_WINDOW) && !pats)) {
+        set_error("zb_sh_coefficients: null pointer");
+        return ZB_EINVAL;
+    }
+    const int L = band->L, n = band->n;
+    if (work_bytes < sh_ws(L, n, n_items)) {
+        set_error("zb_sh_coefficients: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)sh_ws(L, n, n_items));
+        return ZB_ESPACE;
+    }
+    SphArg a;
+    for (int i = 0; i < 3; ++i) {
+        a.c0[i] = frame->c0[i];
+        a.c1[i] = frame->c1[i];
+        a.c2[i] = frame->c2[i];
+        a.mask[i] = frame->mask[i];
+    }
+    a.h = frame->pat_h;
+    a.w = frame->pat_w;
+    a.L = L;
+    a.n = n;
+    a.log2nphi = ilog2(2 * n);
+    a.pairs = L * (L + 1) / 2;
+    a.flags = flags;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)work;
+    float2* sums = (float2*)ws;
+    float4* spectra = (float4*)(ws + ((int64_t)n_items * 8 + 255) / 256 * 256);
+    const float2* tw = (const float2*)band->twiddle;
+    hipLaunchKernelGGL(k_sums, dim3(n_items), dim3(256), 0, s, a, band->ring, tw, pats, (const float4*)items, sums);
+    if (int rc = check_launch("zb_sh_coefficients/sums")) return rc;
+    const int nh = n / 2;
+    const int nphi = 2 * n;
+    int G = 2048 / nphi;
+    if (G < 1) G = 1;
+    if (G > nh) G = nh;
+    hipLaunchKernelGGL(k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,
+                       (const float4*)items, (const float2*)sums, spectra);
+    if (int rc = check_launch("zb_sh_coefficients/rings")) return rc;
+    constexpr int TB = 16;
+    hipLaunchKernelGGL(k_legendre<TB>, dim3((n_items + TB - 1) / TB, L), dim3(64), (size_t)TB * nh * 16, s, a, n_items, band->wplm,
+                       (const float4*)spectra, (float2*)alm);
+    if (int rc = check_launch("zb_sh_coefficients/legendre")) return rc;
+    hipLaunchKernelGGL(k_norm, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
+    return check_launch("zb_sh_coefficients/norm");
+}

@@ -1,0 +1,344 @@
+"""The indexing engine on one GPU: device-resident tables and masters, and the batch pipeline
+normalize -> harmonic coefficients -> SO(3) correlation -> Gauss-Newton refinement + NMI ->
+records, all through libzebsd_hip.so on torch's current stream (src/engine.zig:1793-2137).
+
+PyTorch provides device memory and the stream; every kernel is the library's.
+"""
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import master as master_mod
+from . import tables as tables_mod
+from .geometry import Geometry, frame_of
+from .settings import Settings
+
+RECORD = 8
+PIXEL_KIND = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}
+TORCH_PIXEL_KIND = {torch.uint8: 0, torch.uint16: 1, torch.float32: 2} if hasattr(torch, "uint16") else {torch.uint8: 0, torch.float32: 2}
+# device memory the correlation's spectra may take per chunk of items
+SPEC_BUDGET = 4 << 30
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise nat.NativeError("no GPU: the indexing path runs only on the GPU (libzebsd_hip.so), there is no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr():
+    return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+
+def ctypes_void(v):
+    return v if v else None
+
+
+def _dev(a, dev, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(dev)
+
+
+class DeviceBand:
+    """A bandwidth's tables on the device, with the zb_band struct pointing at them."""
+
+    def __init__(self, L, dev):
+        self.host = tables_mod.BandHost(L)
+        h = self.host
+        self.ring = _dev(h.ring, dev)
+        self.wplm = _dev(h.wplm, dev)
+        self.dq = _dev(h.dq, dev)
+        self.tw = _dev(h.tw, dev)
+        self.c = nat.Band(L, h.n, self.ring.data_ptr(), self.wplm.data_ptr(), self.dq.data_ptr(), self.tw.data_ptr())
+
+
+class DevicePhase:
+    """A phase's weight tables at a bandwidth (for one baked master), and its denominator volume
+    per detector geometry."""
+
+    def __init__(self, band, side, sheets, dev):
+        self.host = tables_mod.PhaseHost(band.host, side, sheets)
+        h = self.host
+        self.w_f = _dev(h.w_f, dev)
+        self.w_f2 = _dev(h.w_f2, dev)
+        self.c = nat.Phase(h.fold, h.nc, h.mu_max, h.l_step, self.w_f.data_ptr(), self.w_f2.data_ptr())
+        self.rden = {}
+
+
+class Phase:
+    def __init__(self, m):
+        self.master = m
+        self.baked = {}     # bake key -> (sheets_dev [2, s, s], grad_dev [2, s, s, 4], sheets_host)
+        self.bands = {}     # (L, bake key) -> DevicePhase
+
+
+class Engine:
+    def __init__(self):
+        self.settings = Settings()
+        self.geo = Geometry()
+        self.phases = []
+        self.patterns = None      # numpy (rows, cols, h, w) or a device tensor
+        self.shape = None         # (rows, cols, h, w)
+        self.bands = {}
+        self.stamp = 0
+        self.dev = None
+
+    # ── state ──
+
+    def _device(self):
+        if self.dev is None:
+            self.dev = device()
+            nat.load()
+        return self.dev
+
+    def set_patterns(self, patterns):
+        """(rows, cols, h, w) or (n, h, w) of uint8, uint16 or float32, as a numpy array or a
+        device tensor (kept where it is)."""
+        if isinstance(patterns, torch.Tensor):
+            a = patterns
+            if a.dim() == 3:
+                a = a[None]
+            if a.dim() != 4:
+                raise ValueError("patterns must be (rows, cols, h, w) or (n, h, w)")
+            if a.dtype not in TORCH_PIXEL_KIND:
+                a = a.float()
+            self.patterns = a.contiguous()
+        else:
+            a = np.asarray(patterns)
+            if a.ndim == 3:
+                a = a[None]
+            if a.ndim != 4:
+                raise ValueError("patterns must be (rows, cols, h, w) or (n, h, w)")
+            if a.dtype not in PIXEL_KIND:
+                a = a.astype(np.float32)
+            self.patterns = np.ascontiguousarray(a)
+        self.shape = tuple(int(x) for x in self.patterns.shape)
+        if self.shape[2] < 4 or self.shape[3] < 4:
+            raise ValueError("patterns smaller than 4 x 4")
+
+    def add_phase(self, path):
+        self.phases.append(Phase(master_mod.load(path)))
+        return len(self.phases) - 1
+
+    def add_master(self, m):
+        self.phases.append(Phase(m))
+        return len(self.phases) - 1
+
+    # ── geometry ──
+
+    def pattern_hw(self):
+        if self.shape is None:
+            raise ValueError("no patterns")
+        return self.shape[2], self.shape[3]
+
+    def frame(self):
+        h, w = self.pattern_hw()
+        f = frame_of(self.geo, h, w, self.settings["mask"], self.settings["mask_radius"])
+        c = nat.Frame()
+        c.c0[:] = [float(x) for x in f["c0"]]
+        c.c1[:] = [float(x) for x in f["c1"]]
+        c.c2[:] = [float(x) for x in f["c2"]]
+        c.pat_h, c.pat_w = h, w
+        c.mask[:] = [float(x) for x in f["mask"]]
+        return c
+
+    def mask_tuple(self):
+        h, w = self.pattern_hw()
+        return tuple(float(x) for x in frame_of(self.geo, h, w, self.settings["mask"], self.settings["mask_radius"])["mask"])
+
+    def translation(self):
+        h, w = self.pattern_hw()
+        return self.geo.translation(float(h), float(w))
+
+    def positions(self, cells):
+        """Detector position t (pixels) per map cell: the reference cell's less the beam's travel
+        (src/engine.zig:1588-1613)."""
+        rows, cols = self.shape[0], self.shape[1]
+        cells = np.asarray(cells, np.int64)
+        r = cells // cols
+        c = cells % cols
+        off = self.geo.scan_offset(r, c, 0.5 * (rows - 1), 0.5 * (cols - 1), self.settings["scan_x_sign"], self.settings["scan_y_sign"], 1.0)
+        return (self.translation()[None, :] - off).astype(np.float32)
+
+    def pixel_angle(self):
+        h, _ = self.pattern_hw()
+        if self.settings["pixel_angle_deg"] > 0:
+            return np.radians(self.settings["pixel_angle_deg"])
+        return self.geo.pixel_angle(float(h))
+
+    # ── masters and tables ──
+
+    def bake_key(self, p):
+        return master_mod.bake_sigma(p.master.side, self.pixel_angle(), self.settings["master_smooth"])
+
+    def baked(self, pi):
+        """The phase's master smoothed on the sphere for this pattern size and geometry, renormalized,
+        and its gradient field (src/engine.zig:1702-1788)."""
+        p = self.phases[pi]
+        key = self.bake_key(p)
+        if key in p.baked:
+            return key, p.baked[key]
+        dev = self._device()
+        lib = nat.load()
+        m = p.master
+        side = m.side
+        cur = _dev(m.raw, dev).reshape(-1)
+        host = m.raw
+        passes = master_mod.blur_passes(key)
+        if passes:
+            tmp = torch.empty_like(cur)
+            px_rad = np.sqrt(2 * np.pi) / side
+            for s in passes:
+                nat.check(lib.zb_master_blur(cur.data_ptr(), tmp.data_ptr(), side, float(s * px_rad), stream_ptr()), "master blur")
+                cur, tmp = tmp, cur
+            host = master_mod.normalize_sheets(cur.cpu().numpy().astype(np.float64).reshape(2, side, side))
+            cur = _dev(host, dev).reshape(-1)
+        grad = torch.empty((2 * side * side * 4,), dtype=torch.float32, device=dev)
+        nat.check(lib.zb_master_gradient(cur.data_ptr(), side, grad.data_ptr(), stream_ptr()), "master gradient")
+        p.baked[key] = (cur, grad, np.asarray(host, np.float32).reshape(2, side, side))
+        return key, p.baked[key]
+
+    def band(self, L):
+        if L not in self.bands:
+            self.bands[L] = DeviceBand(L, self._device())
+        return self.bands[L]
+
+    def phase_band(self, pi, L):
+        key, (_, _, host) = self.baked(pi)
+        p = self.phases[pi]
+        k = (L, key)
+        if k not in p.bands:
+            p.bands[k] = DevicePhase(self.band(L), p.master.side, host, self._device())
+        return p.bands[k]
+
+    def rden(self, pi, L):
+        """The denominator volume of the normalized correlation for the reference cell's detector
+        position (src/engine.zig:1839-1849)."""
+        lib = nat.load()
+        band = self.band(L)
+        ph = self.phase_band(pi, L)
+        fr = self.frame()
+        t = self.translation().astype(np.float32)
+        key = (tuple(fr.c0), tuple(fr.c1), tuple(fr.c2), fr.pat_h, fr.pat_w, tuple(fr.mask), tuple(t))
+        if key in ph.rden:
+            return ph.rden[key]
+        dev = self._device()
+        win = torch.tensor([[t[0], t[1], t[2], 0.0]], dtype=torch.float32, device=dev)
+        pairs = band.host.pairs
+        w_alm = torch.empty((1, pairs, 2), dtype=torch.float32, device=dev)
+        w_stats = torch.empty((1, 4), dtype=torch.float32, device=dev)
+        ws = torch.empty((max(lib.zb_sh_workspace(band.c, 1), 1),), dtype=torch.uint8, device=dev)
+        nat.check(lib.zb_sh_coefficients(band.c, fr, None, win.data_ptr(), 1, 4, ws.data_ptr(), ws.numel(), w_alm.data_ptr(),
+                                         w_stats.data_ptr(), stream_ptr()), "window coefficients")
+        h = ph.host
+        n = band.host.n
+        rden = torch.empty((n // 2 + 1) * n * h.nc, dtype=torch.float32, device=dev)
+        ws2 = torch.empty((nat.check(lib.zb_so3_workspace(band.c, ph.c, 1), "so3 workspace"),), dtype=torch.uint8, device=dev)
+        nat.check(lib.zb_so3_denominators(band.c, ph.c, w_alm.data_ptr(), w_stats.data_ptr(), 1, ws2.data_ptr(), ws2.numel(),
+                                          rden.data_ptr(), stream_ptr()), "denominators")
+        ph.rden[key] = rden
+        return rden
+
+    # ── the batch pipeline ──
+
+    def refine_opts(self):
+        s = self.settings
+        o = nat.RefineOpts()
+        o.iters = int(s["refine_iters"]) if s["refine"] else 0
+        o.max_step = float(np.radians(s["refine_step"]))
+        o.scale0, o.scale_min, o.scale_max = 1.0, 0.02, 1e4
+        o.nmi_bins = int(s["nmi_bins"])
+        self.stamp = self.stamp % (1 << 24) + 1
+        o.stamp = self.stamp
+        return o
+
+    def run_batch(self, raw, items, L=None, workspace=None):
+        """The pipeline on device-resident inputs: raw [n, h, w] (uint8/uint16/float32 tensor) and
+        items [n, 4] float32 (t, index into raw). Returns records [n_phases, n, 8] on the device
+        (and the stamp they carry). Nothing is read back."""
+        lib = nat.load()
+        dev = self._device()
+        s = self.settings
+        L = int(L or s["bandwidth"])
+        n = int(items.shape[0])
+        h, w = int(raw.shape[-2]), int(raw.shape[-1])
+        kind = TORCH_PIXEL_KIND[raw.dtype]
+        band = self.band(L)
+        fr = self.frame()
+        mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
+        st = stream_ptr()
+        pats = torch.empty((raw.shape[0], h, w), dtype=torch.float32, device=dev)
+        nat.check(lib.zb_normalize(raw.data_ptr(), kind, raw.shape[0], h, w, mask, pats.data_ptr(), st), "normalize")
+        n_ph = len(self.phases)
+        opts = self.refine_opts()
+        records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
+        phs = [self.phase_band(pi, L) for pi in range(n_ph)]
+        rdens = [self.rden(pi, L) for pi in range(n_ph)]
+        per_item = max(int(lib.zb_so3_workspace(band.c, ph.c, 1)) for ph in phs)
+        chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
+        pairs = band.host.pairs
+        alm = torch.empty((chunk, pairs, 2), dtype=torch.float32, device=dev)
+        stats = torch.empty((chunk, 4), dtype=torch.float32, device=dev)
+        peaks = torch.empty((chunk, 8), dtype=torch.float32, device=dev)
+        need = max(int(lib.zb_sh_workspace(band.c, chunk)), max(int(lib.zb_so3_workspace(band.c, ph.c, chunk)) for ph in phs))
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
+        for c0 in range(0, n, chunk):
+            nt = min(chunk, n - c0)
+            it = items[c0:c0 + nt]
+            nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nt, 3, workspace.data_ptr(), workspace.numel(),
+                                             alm.data_ptr(), stats.data_ptr(), st), "coefficients")
+            for pi, ph in enumerate(phs):
+                nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
+                                               workspace.numel(), peaks.data_ptr(), st), "correlate")
+                _, (_, grad, _) = self.baked(pi)
+                rec = records[pi]
+                nat.check(lib.zb_refine(grad.data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(), 8,
+                                        nt, opts, rec.data_ptr(), c0, st), "refine")
+        return records, opts.stamp
+
+    def _raw_for(self, cells):
+        dev = self._device()
+        rows, cols, h, w = self.shape
+        flat = self.patterns.reshape(rows * cols, h, w)
+        cells = np.asarray(cells, np.int64)
+        if isinstance(flat, torch.Tensor):
+            return flat[torch.as_tensor(cells, device=flat.device)].to(dev).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(flat[cells])).to(dev)
+
+    def index_cells(self, cells, L=None):
+        """Records [n_phases, n, 8] (q, ncc, nmi, step, stamp) for the given map cells; the
+        stamp is checked (src/engine.zig:2127-2137)."""
+        if not self.phases:
+            raise ValueError("no phases")
+        cells = np.asarray(cells, np.int64).reshape(-1)
+        if len(cells) == 0:
+            return np.zeros((len(self.phases), 0, RECORD), np.float32)
+        raw = self._raw_for(cells)
+        t = self.positions(cells)
+        items = np.concatenate([t, np.arange(len(cells), dtype=np.float32)[:, None]], 1)
+        items_d = torch.from_numpy(items).to(self._device())
+        rec, stamp = self.run_batch(raw, items_d, L)
+        out = rec.cpu().numpy()
+        if not np.all(out[..., 7] == np.float32(stamp)):
+            raise RuntimeError("DeviceSkippedBatch: records without this batch's stamp")
+        return out
+
+    def simulate(self, pi, cells, quats):
+        """Simulated patterns of phase pi at the given orientations, as cells would record them
+        (src/engine.zig:2753-2769). Returns float32 (n, h, w)."""
+        lib = nat.load()
+        dev = self._device()
+        h, w = self.pattern_hw()
+        cells = np.asarray(cells, np.int64).reshape(-1)
+        quats = np.asarray(quats, np.float32).reshape(-1, 4)
+        t = self.positions(cells)
+        poses = np.concatenate([t, np.zeros((len(cells), 1), np.float32), quats], 1).astype(np.float32)
+        _, (sheets, _, _) = self.baked(pi)
+        pd = torch.from_numpy(poses).to(dev)
+        out = torch.empty((len(cells), h, w), dtype=torch.float32, device=dev)
+        nat.check(lib.zb_project(sheets.data_ptr(), self.phases[pi].master.side, self.frame(), pd.data_ptr(), len(cells), out.data_ptr(),
+                                 stream_ptr()), "project")
+        return out.cpu().numpy()

@@ -1,0 +1,109 @@
+"""Options of the indexing path, by the reference's names and defaults (src/settings.zig).
+
+`apply` rejects unknown keys and values out of range atomically, as Settings.apply/check does
+(src/settings.zig:404-463). Keys of the reference that belong to stages outside this build
+(pattern filters, dictionary and PCA search, binning, sweeps, the fits) are accepted only at the
+values that leave those stages out, and say so otherwise.
+"""
+import copy
+
+DEFAULTS = {
+    # ── a toy scan (synthesize) ──
+    "toy_seed": 1, "toy_rows": 48, "toy_cols": 64, "toy_grains": 24, "toy_spread_deg": 0.0, "toy_blank": 0.0,
+    "toy_height": 60, "toy_width": 60, "toy_contrast": 1.0, "toy_background_sigma": 0.0, "toy_counts": 20.0,
+    "toy_noise_seed": 0,
+    # ── scan ──
+    "scan_x_sign": -1, "scan_y_sign": -1,
+    # ── resolution ──
+    "binning": 1.0, "full_resolution": "none",
+    # ── averaging and pattern filters (outside this build: only their "off" values) ──
+    "nlpar": False, "static_background": "none", "dynamic_background": "none", "clahe": False, "bandpass": False,
+    # ── mask ──
+    "mask": True, "mask_radius": 1.0,
+    # ── master ──
+    "pixel_angle_deg": 0.0, "master_smooth": 0.3,
+    # ── spherical indexing ──
+    "bandwidth": 64, "method": "spherical",
+    # ── refinement ──
+    "refine": True, "refine_iters": 8, "refine_step": 2.0,
+    # ── phases ──
+    "phase_by": "nmi", "nmi_bins": 16,
+    # ── how the work is cut up ──
+    "batch": 256, "tile_mb": 256, "refine_pixels": 64, "wg_fft": 64, "wg_refine": 64,
+    # ── sweeps and acceptance ──
+    "sweeps": 1, "accept_nmi": 0.0, "accept_z": 15.0, "accept_step_deg": 0.2,
+}
+
+# values a key outside this build may take (the reference's defaults differ for the filters:
+# static_background "average" and dynamic_background "subtract" there)
+_ONLY = {
+    "nlpar": (False,), "static_background": ("none",), "dynamic_background": ("none",), "clahe": (False,),
+    "bandpass": (False,), "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",), "sweeps": (1,),
+}
+
+
+class SettingError(ValueError):
+    pass
+
+
+def check(s):
+    def bad(cond, key):
+        if cond:
+            raise SettingError(f"setting {key}={s[key]!r} is out of range")
+    bad(not (8 <= s["bandwidth"] <= 256), "bandwidth")
+    bad(not (2 <= s["nmi_bins"] <= 32), "nmi_bins")
+    bad(not (1 <= s["batch"] <= 65536), "batch")
+    bad(not (8 <= s["tile_mb"] <= 65536), "tile_mb")
+    bad(not (s["mask_radius"] > 0), "mask_radius")
+    bad(not (s["master_smooth"] >= 0), "master_smooth")
+    bad(not (s["pixel_angle_deg"] >= 0), "pixel_angle_deg")
+    bad(s["refine_iters"] < 0 or s["refine_iters"] > 1000, "refine_iters")
+    bad(not (s["refine_step"] > 0), "refine_step")
+    bad(s["phase_by"] not in ("nmi", "ncc"), "phase_by")
+    bad(not (1 <= s["toy_rows"] <= 4096 and 1 <= s["toy_cols"] <= 4096 and s["toy_grains"] >= 1), "toy_rows")
+    bad(not (8 <= s["toy_height"] <= 1024 and 8 <= s["toy_width"] <= 1024), "toy_height")
+    bad(not (s["toy_spread_deg"] >= 0 and 0 <= s["toy_blank"] <= 1 and 0 < s["toy_contrast"] <= 1 and s["toy_background_sigma"] >= 0), "toy_contrast")
+    bad(not (0 <= s["toy_counts"] <= 1e6), "toy_counts")
+    bad(abs(s["scan_x_sign"]) != 1 or abs(s["scan_y_sign"]) != 1, "scan_x_sign")
+    bad(not (0 <= s["accept_nmi"] <= 1) or not (s["accept_step_deg"] > 0), "accept_nmi")
+    for k in ("wg_fft", "wg_refine"):
+        bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
+    for k, allowed in _ONLY.items():
+        if s[k] not in allowed:
+            raise SettingError(f"setting {k}={s[k]!r} selects a stage outside this build (supported: {allowed[0]!r})")
+
+
+class Settings:
+    def __init__(self):
+        self.values = copy.deepcopy(DEFAULTS)
+
+    def apply(self, **kw):
+        nxt = dict(self.values)
+        for k, v in kw.items():
+            if k not in DEFAULTS:
+                raise SettingError(f"unknown setting {k!r}")
+            d = DEFAULTS[k]
+            if isinstance(d, bool):
+                if not isinstance(v, (bool, int)):
+                    raise SettingError(f"setting {k} takes a boolean")
+                v = bool(v)
+            elif isinstance(d, int):
+                if isinstance(v, float) and v.is_integer():
+                    v = int(v)
+                if not isinstance(v, int) or isinstance(v, bool):
+                    raise SettingError(f"setting {k} takes an integer")
+            elif isinstance(d, float):
+                if not isinstance(v, (int, float)) or isinstance(v, bool):
+                    raise SettingError(f"setting {k} takes a number")
+                v = float(v)
+            elif isinstance(d, str) and not isinstance(v, str):
+                raise SettingError(f"setting {k} takes a string")
+            nxt[k] = v
+        check(nxt)
+        self.values = nxt
+
+    def __getitem__(self, k):
+        return self.values[k]
+
+    def as_dict(self):
+        return dict(self.values)
