@@ -1,0 +1,290 @@
+#!/usr/bin/env python3
+"""Throughput of the indexing hot path on one MI355X.
+
+Workload (the plain FP32 GPU form at scale): 100 000 synthetic 60 x 60 float32 patterns of the
+nickel master (the bundled ebsdsim master, 2 x 501^2), rendered on the GPU at seeded uniform
+random orientations with the pattern centre jittered by U(+-0.01) per pattern and coordinate
+around (0.50, 0.50, 0.55), Poisson noise at a mean of 400 counts; indexed in seeded uneven
+chunks of 1 000 - 8 000 patterns at bandwidth L = 64 with 5 Gauss-Newton iterations. A step is
+one pass over all 100 000 patterns (normalize, harmonic coefficients, SO(3) correlation and
+peak, refinement, NMI, records). The patterns are resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits with a message: this
+build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MASTER = os.path.join(ROOT, "tests", "golden", "Ni_master_20kV.npz")
+PC = (0.50, 0.50, 0.55)
+H = W = 60
+FP32_PEAK_TFLOPS = 157.3   # MI355X published dense FP32 (MFMA and packed VALU)
+HBM_PEAK_GBS = 8000.0
+
+
+# ── algorithmic work per pattern, on the reference's algorithm ──
+
+def spectrum_terms(L, fold, inversion):
+    """Exact nonzero terms of the Wigner-d contraction per pattern: sum over k, m' < L and
+    |mu| <= mu_max of #{l : max(k, m', fold |mu|) <= l < L, l even when the master has an
+    inversion centre}."""
+    mu_max = (L - 1) // fold
+    total = 0
+    ls = np.arange(L)
+    for k in range(L):
+        for mp in range(L):
+            for mu in range(-mu_max, mu_max + 1):
+                lo = max(k, mp, fold * abs(mu))
+                sel = ls[lo:]
+                total += int(np.sum(sel % 2 == 0)) if inversion else len(sel)
+    return total
+
+
+def flops_per_pattern(L, fold, inversion, iters, n_mask):
+    """Per kernel family, in FLOP per pattern (complex multiply-accumulate = 8, radix-2 FFT =
+    5 N log2 N, refinement 120 per masked pixel per pass)."""
+    n = 16
+    while n < 2 * L - 1:
+        n *= 2
+    nc = n // fold
+    nb = n // 2 + 1
+    lg = np.log2
+    f_spec = 8 * spectrum_terms(L, fold, inversion)
+    f_bfft = 5 * n * lg(n) * ((L + 1) // 2) * nc
+    f_slab = 5 * nc * lg(nc) * nb * L + 5 * (n / 2) * lg(n / 2) * nb * nc
+    pairs = L * (L + 1) // 2
+    f_rings = 5 * (2 * n) * lg(2 * n) * (n / 2)
+    f_leg = 4 * pairs * (n / 2)
+    f_ref = 120 * (iters + 1) * n_mask + 120 * n_mask
+    return {"k_spectrum": f_spec + f_bfft, "k_slab0": f_slab, "k_rings": f_rings, "k_legendre": f_leg, "k_refine": f_ref}
+
+
+def mask_count(h, w):
+    r = 0.5 * min(h, w)
+    rr, cc = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    return int(np.sum((rr - 0.5 * (h - 1)) ** 2 + (cc - 0.5 * (w - 1)) ** 2 <= np.float32(r * r)))
+
+
+# ── the workload ──
+
+def make_workload(e, n, seed, counts):
+    """Render n patterns on the GPU (zb_project), Poisson noise on the GPU. Returns the raw
+    patterns [n, H, W] on the device, items [n, 4] (t, pattern index) and the true orientations."""
+    import torch
+    from zebsdsolve_amd import _native as nat, engine
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q[q[:, 0] < 0] *= -1
+    pcs = np.asarray(PC)[None, :] + rng.uniform(-0.01, 0.01, size=(n, 3))
+    R = e.geo.rotation()
+    t = np.stack([R @ np.array([0.5 - p[1] * H, 0.5 - p[0] * W, p[2] * H]) for p in pcs]).astype(np.float32)
+    poses = np.concatenate([t, np.zeros((n, 1), np.float32), q.astype(np.float32)], 1)
+    lib = nat.load()
+    _, (sheets, _, _) = e.baked(0)
+    side = e.phases[0].master.side
+    dev = torch.device("cuda")
+    sims = torch.empty((n, H, W), dtype=torch.float32, device=dev)
+    pd = torch.from_numpy(poses).to(dev)
+    fr = e.frame()
+    step = 16384
+    for c0 in range(0, n, step):
+        c1 = min(n, c0 + step)
+        nat.check(lib.zb_project(sheets.data_ptr(), side, fr, pd[c0:c1].data_ptr(), c1 - c0, sims[c0:c1].data_ptr(), engine.stream_ptr()), "project")
+    lo = sims.min()
+    mean = sims.mean()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    pats = torch.poisson(counts * (sims - lo) / (mean - lo), generator=gen).float()
+    del sims
+    items = np.concatenate([t, np.arange(n, dtype=np.float32)[:, None]], 1)
+    return pats, items, q
+
+
+def chunks_of(n, seed, lo=1000, hi=8000):
+    rng = np.random.default_rng(seed + 1)
+    out = []
+    c0 = 0
+    while c0 < n:
+        k = int(rng.integers(lo, hi + 1))
+        out.append((c0, min(n, c0 + k)))
+        c0 += k
+    return out
+
+
+# ── the CPU baseline: the oracle's float64 restatement, one thread ──
+
+def cpu_baseline(pats_host, items, sheets, side, L, iters, budget_s=20.0, max_patterns=64):
+    from threadpoolctl import threadpool_limits
+    from oracle import pipeline as op
+    R = op.bruker_R(70.0, 0.0)
+    fr = op.frame(R, H, W)
+    t0 = op.translation(R, PC, float(H), float(W))
+    f, f2, fold, _ = op.harmonics(sheets, side, L)
+    so = op.So3(L, fold)
+    wa, wst = op.coefficients(None, fr, t0, L, center=False, norm=False, window=True)
+    rden = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])
+    grad = op.gradient(sheets, side)
+    done = 0
+    with threadpool_limits(limits=1):
+        start = time.perf_counter()
+        while done < max_patterns and time.perf_counter() - start < budget_s:
+            pn = op.normalize(pats_host[done].astype(np.float64), fr["mask"])
+            t = items[done, :3].astype(np.float64)
+            alm, _ = op.coefficients(pn, fr, t, L)
+            q, v0, _, _ = so.peak(so.volume(f, alm), rden)
+            qr, ncc, _ = op.refine(grad, side, fr, pn, t, q, iters=iters)
+            op.nmi(sheets, side, fr, pn, t, qr)
+            done += 1
+        el = time.perf_counter() - start
+    return {"value": done / el, "unit": "patterns/s", "cores": 1, "kind": "port",
+            "sample": f"{done} of the same patterns (60x60, L={L}, {iters} iterations) through oracle/ (float64 numpy, "
+                      f"one thread) in {el:.1f} s; tables built beforehand, as on the GPU"}
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (profiles/<kernel>_pmc.txt: lines 'FETCH_SIZE_bytes_per_dispatch <v>' and
+    'WRITE_SIZE_bytes_per_dispatch <v>'), or None."""
+    p = os.path.join(ROOT, "profiles", f"{kernel}_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    for line in open(p):
+        parts = line.split()
+        if len(parts) == 2 and parts[0].endswith("_bytes_per_dispatch"):
+            vals[parts[0]] = float(parts[1])
+    if "FETCH_SIZE_bytes_per_dispatch" in vals and "WRITE_SIZE_bytes_per_dispatch" in vals:
+        return vals["FETCH_SIZE_bytes_per_dispatch"] + vals["WRITE_SIZE_bytes_per_dispatch"]
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--patterns", type=int, default=100_000)
+    ap.add_argument("--bandwidth", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=5)
+    ap.add_argument("--counts", type=float, default=400.0)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by patterns with no "
+              "exchange in the hot loop; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from zebsdsolve_amd import _native as nat, engine, sym
+    from zebsdsolve_amd.geometry import Geometry
+
+    e = engine.Engine()
+    e.geo = Geometry(pc=PC, sample_tilt_deg=70.0)
+    e.add_phase(MASTER)
+    e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters)
+    e.set_patterns(np.zeros((1, 1, H, W), np.float32))
+    n = args.patterns
+    pats, items, truth = make_workload(e, n, args.seed, args.counts)
+    items_d = torch.from_numpy(items).cuda()
+    chunks = chunks_of(n, args.seed)
+    # per chunk, item indices local to the chunk
+    local = []
+    for c0, c1 in chunks:
+        it = items_d[c0:c1].clone()
+        it[:, 3] = torch.arange(c1 - c0, dtype=torch.float32, device=it.device)
+        local.append(it)
+    ph = e.phase_band(0, args.bandwidth)
+    e.rden(0, args.bandwidth)
+    torch.cuda.synchronize()
+
+    def step():
+        recs = []
+        for (c0, c1), it in zip(chunks, local):
+            r, _ = e.run_batch(pats[c0:c1], it, args.bandwidth)
+            recs.append(r)
+        return recs
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    if not args.no_profile:
+        nat.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        recs = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = nat.profile_report() if not args.no_profile else {}
+    nat.profile(False)
+    el = t1 - t0
+    value = args.steps * n / el
+
+    rec = torch.cat([r[0] for r in recs]).cpu().numpy()
+    rots = e.phases[0].master.rotations
+    dis = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), truth, rots))
+
+    n_mask = mask_count(H, W)
+    fold, inversion = ph.host.fold, ph.host.hm.rules.inversion
+    fpp = flops_per_pattern(args.bandwidth, fold, inversion, args.iters, n_mask)
+    roofline = None
+    kernels = {}
+    if prof:
+        for k, (cnt, ms) in prof.items():
+            kernels[k] = {"launches": cnt, "ms_total": round(ms, 3)}
+        dom = max(prof, key=lambda k: prof[k][1])
+        cnt, ms = prof[dom]
+        per_launch_ms = ms / cnt
+        patterns_per_launch = args.steps * n / cnt
+        if dom in fpp:
+            achieved = fpp[dom] * patterns_per_launch / (per_launch_ms * 1e-3) / 1e12
+            roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(dom),
+                        "kernel": dom, "launch_ms": round(per_launch_ms, 4), "patterns_per_launch": round(patterns_per_launch, 1)}
+        total_alg = sum(fpp.values()) * args.steps * n
+        kernels["_all_algorithmic_TFLOPs"] = round(total_alg / el / 1e12, 3)
+
+    cpu = None
+    if not args.no_cpu:
+        _, (_, _, host_sheets) = e.baked(0)
+        sample = pats[:64].cpu().numpy()
+        cpu = cpu_baseline(sample, items[:64], host_sheets, e.phases[0].master.side, args.bandwidth, args.iters)
+
+    out = {
+        "metric": "indexed patterns/s at <=0.1 deg median misorientation; % of HBM/MFMA roofline",
+        "value": round(value, 1),
+        "unit": "patterns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * el / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Ni master projected on the GPU at seeded random orientations, PC jitter +-0.01, Poisson 400 counts",
+        "config": {"workload": f"{n} patterns {H}x{W} f32, Ni (Fm-3m, bundled ebsdsim master), L={args.bandwidth}, "
+                               f"{args.iters} Gauss-Newton iterations, chunks U[1000, 8000]",
+                   "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks)},
+        "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
+                     "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,6 +153,13 @@ int32_t zb_master_blur(const float* src, float* dst, int32_t side, float sigma_r
  * Replaces Prep.gradient (src/prep.zig:253-260; mastersph.wgsl:124-138). */
 int32_t zb_master_gradient(const float* sheets, int32_t side, float* grad, void* stream);
 
+/* Kernel timing (events around every launch while on). The reference times its dispatches the
+ * same way (Gpu.setProfile / profileReport, src/gpu/gpu.zig:287-313); its C ABI has no entry
+ * point for it. zb_profile(1) clears and starts, zb_profile(0) stops; zb_profile_report waits
+ * for the recorded launches and writes one line "kernel count total_ms" per kernel. */
+int32_t zb_profile(int32_t on);
+int32_t zb_profile_report(char* buf, int32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,10 +37,20 @@ def gl(n):
     return x[::-1].copy(), w[::-1].copy()
 
 
+_PLM = {}
+
+
 def plm(L, x):
     """Orthonormal P_l^m(x) with the Condon-Shortley phase for every stored pair: [pair, len(x)],
-    as Y_l^m(theta, 0) from scipy (src/sh.zig:75-102 computes them by recurrence)."""
+    as Y_l^m(theta, 0) from scipy (src/sh.zig:75-102 computes them by recurrence). Cached."""
     x = np.atleast_1d(np.asarray(x, np.float64))
+    key = (L, x.tobytes())
+    if key not in _PLM:
+        _PLM[key] = _plm(L, x)
+    return _PLM[key]
+
+
+def _plm(L, x):
     theta = np.arccos(np.clip(x, -1, 1))
     ms, ls = pairs(L)
     out = np.empty((len(ms), len(x)))

@@ -46,6 +46,8 @@ _SIGS = {
     "zb_project": ([vp, i32, ctypes.POINTER(Frame), vp, i32, vp, vp], i32),
     "zb_master_blur": ([vp, vp, i32, ctypes.c_float, vp], i32),
     "zb_master_gradient": ([vp, i32, vp, vp], i32),
+    "zb_profile": ([i32], i32),
+    "zb_profile_report": ([ctypes.c_char_p, i32], i32),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -84,3 +86,20 @@ def check(rc, what):
     if rc < 0:
         raise NativeError(f"{what}: {last_error() or rc}")
     return rc
+
+
+def profile(on):
+    load().zb_profile(1 if on else 0)
+
+
+def profile_report():
+    """{kernel: (launches, total ms)} of the launches recorded since profile(True)."""
+    lib = load()
+    n = lib.zb_profile_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.zb_profile_report(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out

@@ -3,6 +3,12 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "common.h"
 
 namespace zb {
@@ -25,9 +31,85 @@ int check_launch(const char* what) {
     return ZB_OK;
 }
 
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+static std::mutex g_prof_mu;
+static std::atomic<bool> g_prof{false};
+static std::vector<ProfRec> g_recs;
+static std::vector<hipEvent_t> g_pool;
+
+static hipEvent_t take_event() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+bool prof_on() { return g_prof.load(std::memory_order_relaxed); }
+
+void prof_begin(const char* name, hipStream_t s) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    ProfRec r{name, take_event(), take_event()};
+    if (r.a) (void)hipEventRecord(r.a, s);
+    g_recs.push_back(r);
+}
+
+void prof_end(hipStream_t s) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (!g_recs.empty() && g_recs.back().b) (void)hipEventRecord(g_recs.back().b, s);
+}
+
 }  // namespace zb
 
 extern "C" const char* zb_version(void) { return "0.1.0"; }
+
+// Kernel timing, the counterpart of the reference's Gpu.setProfile / profileReport
+// (src/gpu/gpu.zig:287-313; python/zebsdsolve/__init__.py:944-951). No counterpart in the C ABI
+// of the reference. on = 1 starts recording (and clears what was recorded), 0 stops.
+extern "C" int32_t zb_profile(int32_t on) {
+    std::lock_guard<std::mutex> g(zb::g_prof_mu);
+    for (auto& r : zb::g_recs) {
+        if (r.a) zb::g_pool.push_back(r.a);
+        if (r.b) zb::g_pool.push_back(r.b);
+    }
+    zb::g_recs.clear();
+    zb::g_prof.store(on != 0);
+    return ZB_OK;
+}
+
+// Waits for the recorded launches and writes "name count total_ms\n" per kernel name into buf
+// (the bytes needed are returned; the text is cut at cap - 1).
+extern "C" int32_t zb_profile_report(char* buf, int32_t cap) {
+    std::lock_guard<std::mutex> g(zb::g_prof_mu);
+    std::map<std::string, std::pair<long, double>> acc;
+    for (auto& r : zb::g_recs) {
+        float ms = 0.f;
+        if (r.a && r.b && hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& e = acc[r.name];
+            e.first += 1;
+            e.second += ms;
+        }
+    }
+    std::string out;
+    char line[256];
+    for (auto& kv : acc) {
+        snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+        out += line;
+    }
+    if (buf && cap > 0) {
+        size_t k = out.size() < (size_t)(cap - 1) ? out.size() : (size_t)(cap - 1);
+        memcpy(buf, out.data(), k);
+        buf[k] = 0;
+    }
+    return (int32_t)out.size();
+}
 
 extern "C" int32_t zb_last_error(char* buf, int32_t cap) {
     int32_t n = (int32_t)strlen(zb::g_err);

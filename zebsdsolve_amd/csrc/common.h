@@ -11,11 +11,25 @@
 // twiddle table length: tw[t] = exp(-2 pi i t / ZB_TW)
 #define ZB_TW 1024
 
+// Launch a kernel, bracketed by timing events when profiling is on.
+#define ZB_LAUNCH(name, kern, grid, block, lds, stream, ...)                      \
+    do {                                                                          \
+        const bool zb_p_ = zb::prof_on();                                         \
+        if (zb_p_) zb::prof_begin(name, stream);                                  \
+        hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);          \
+        if (zb_p_) zb::prof_end(stream);                                          \
+    } while (0)
+
 namespace zb {
 
-// ── error reporting (api.hip) ──
+// ── error reporting and kernel timing (api.hip) ──
 void set_error(const char* fmt, ...);
 int check_launch(const char* what);
+// With profiling on, prof_begin records an event on `s` before a launch and prof_end one after
+// it, under the kernel's name; zb_profile_report sums them per name.
+bool prof_on();
+void prof_begin(const char* name, hipStream_t s);
+void prof_end(hipStream_t s);
 
 // ── quaternions, (w, x, y, z) in (.x, .y, .z, .w), acting as q v q* ──
 __device__ __forceinline__ float4 qmul(float4 a, float4 b) {

@@ -152,11 +152,11 @@ extern "C" int32_t zb_normalize(const void* raw, int32_t pixel, int32_t n, int32
     m.m[2] = mask[2];
     hipStream_t s = (hipStream_t)stream;
     if (pixel == 0)
-        hipLaunchKernelGGL(k_normalize<uint8_t>, dim3(n), dim3(256), 0, s, (const uint8_t*)raw, h, w, m, out);
+        ZB_LAUNCH("k_normalize", k_normalize<uint8_t>, dim3(n), dim3(256), 0, s, (const uint8_t*)raw, h, w, m, out);
     else if (pixel == 1)
-        hipLaunchKernelGGL(k_normalize<uint16_t>, dim3(n), dim3(256), 0, s, (const uint16_t*)raw, h, w, m, out);
+        ZB_LAUNCH("k_normalize", k_normalize<uint16_t>, dim3(n), dim3(256), 0, s, (const uint16_t*)raw, h, w, m, out);
     else
-        hipLaunchKernelGGL(k_normalize<float>, dim3(n), dim3(256), 0, s, (const float*)raw, h, w, m, out);
+        ZB_LAUNCH("k_normalize", k_normalize<float>, dim3(n), dim3(256), 0, s, (const float*)raw, h, w, m, out);
     return check_launch("zb_normalize");
 }
 
@@ -169,7 +169,7 @@ extern "C" int32_t zb_project(const float* master, int32_t side, const zb_frame*
     if (n == 0) return ZB_OK;
     FrameArg fr = frame_arg(frame);
     long total = (long)n * fr.h * fr.w;
-    hipLaunchKernelGGL(k_project, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, master, side, fr, poses, n, out);
+    ZB_LAUNCH("k_project", k_project, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, master, side, fr, poses, n, out);
     return check_launch("zb_project");
 }
 
@@ -179,7 +179,7 @@ extern "C" int32_t zb_master_blur(const float* src, float* dst, int32_t side, fl
         return ZB_EINVAL;
     }
     int total = 2 * side * side;
-    hipLaunchKernelGGL(k_blur, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, src, dst, side, 0.6f * sigma_rad,
+    ZB_LAUNCH("k_blur", k_blur, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, src, dst, side, 0.6f * sigma_rad,
                        1.f / (2.f * sigma_rad * sigma_rad));
     return check_launch("zb_master_blur");
 }
@@ -191,6 +191,6 @@ extern "C" int32_t zb_master_gradient(const float* sheets, int32_t side, float* 
     }
     int total = 2 * side * side;
     float h = (float)(2.5066282746310002 / (double)side);  // sqrt(2 pi) / side
-    hipLaunchKernelGGL(k_gradient, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, sheets, side, h, (float4*)grad);
+    ZB_LAUNCH("k_gradient", k_gradient, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, sheets, side, h, (float4*)grad);
     return check_launch("zb_master_gradient");
 }

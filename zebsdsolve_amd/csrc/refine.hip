@@ -281,7 +281,7 @@ extern "C" int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* f
     a.scale_min = o->scale_min;
     a.scale_max = o->scale_max;
     a.stamp = (float)o->stamp;
-    hipLaunchKernelGGL(k_refine, dim3(n_items), dim3(64), 0, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items, seeds,
+    ZB_LAUNCH("k_refine", k_refine, dim3(n_items), dim3(64), 0, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items, seeds,
                        n_items, records);
     return check_launch("zb_refine");
 }

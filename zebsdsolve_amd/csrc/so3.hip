@@ -411,7 +411,7 @@ static int launch_spectrum(const SoArg& a, int n_items, const float* w, const fl
     const dim3 grid((unsigned)((long)n_items * a.Lh * groups));
     const size_t lds = (size_t)a.Q * a.n * 8;
     const int kpt = (a.L + 63) / 64;
-#define ZB_SPEC(K, QQ) hipLaunchKernelGGL((k_spectrum<K, QQ>), grid, dim3(64), lds, s, a, n_items, (const float2*)w, dq, (const float2*)alm, tw, U)
+#define ZB_SPEC(K, QQ) ZB_LAUNCH("k_spectrum", (k_spectrum<K, QQ>), grid, dim3(64), lds, s, a, n_items, (const float2*)w, dq, (const float2*)alm, tw, U)
     if (a.Q == 8) {
         if (kpt == 1) ZB_SPEC(1, 8); else if (kpt == 2) ZB_SPEC(2, 8); else if (kpt == 3) ZB_SPEC(3, 8); else ZB_SPEC(4, 8);
     } else if (a.Q == 4) {
@@ -445,16 +445,16 @@ extern "C" int32_t zb_so3_correlate(const zb_band* band, const zb_phase* phase, 
     float2* U = (float2*)work;
     unsigned long long* best = (unsigned long long*)((char*)work + align256(u_bytes(a, n_items)));
     if (int rc = launch_spectrum(a, n_items, phase->w_f, band->dq, alm, tw, U, s)) return rc;
-    hipLaunchKernelGGL(k_clear_u64, dim3((n_items + 255) / 256), dim3(256), 0, s, best, n_items);
+    ZB_LAUNCH("k_clear_u64", k_clear_u64, dim3((n_items + 255) / 256), dim3(256), 0, s, best, n_items);
     const size_t lds = (size_t)(a.n / 2) * a.nc * 8;
     if (lds > 65536 && hipFuncSetAttribute((const void*)k_slab<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
         set_error("slab of %zu bytes: LDS attribute refused", lds);
         return ZB_ELAUNCH;
     }
-    hipLaunchKernelGGL(k_slab<0>, dim3((unsigned)((long)n_items * a.nb)), dim3(256), lds, s, a, n_items, (const float2*)U, tw,
+    ZB_LAUNCH("k_slab0", k_slab<0>, dim3((unsigned)((long)n_items * a.nb)), dim3(256), lds, s, a, n_items, (const float2*)U, tw,
                        (const float2*)rden, rden_div, best, (float2*)nullptr);
     if (int rc = check_launch("zb_so3_correlate/slab")) return rc;
-    hipLaunchKernelGGL(k_finish, dim3(n_items), dim3(64), 0, s, a, n_items, (const float2*)U, tw, (const float2*)rden, rden_div,
+    ZB_LAUNCH("k_finish", k_finish, dim3(n_items), dim3(64), 0, s, a, n_items, (const float2*)U, tw, (const float2*)rden, rden_div,
                        (const unsigned long long*)best, peaks);
     return check_launch("zb_so3_correlate/finish");
 }
@@ -486,12 +486,12 @@ extern "C" int32_t zb_so3_denominators(const zb_band* band, const zb_phase* phas
     float* vs[2] = {vf, vf2};
     for (int i = 0; i < 2; ++i) {
         if (int rc = launch_spectrum(a, n_windows, ws[i], band->dq, w_alm, tw, U, s)) return rc;
-        hipLaunchKernelGGL(k_slab<1>, dim3((unsigned)((long)n_windows * a.nb)), dim3(256), lds, s, a, n_windows, (const float2*)U, tw,
+        ZB_LAUNCH("k_slab1", k_slab<1>, dim3((unsigned)((long)n_windows * a.nb)), dim3(256), lds, s, a, n_windows, (const float2*)U, tw,
                            (const float2*)nullptr, 1, (unsigned long long*)nullptr, (float2*)vs[i]);
         if (int rc = check_launch("zb_so3_denominators/volume")) return rc;
     }
     const long per_item = (long)a.nb * a.n * a.nc;
     const long total = per_item * n_windows;
-    hipLaunchKernelGGL(k_rden, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, vf, vf2, w_stats, per_item, total, rden);
+    ZB_LAUNCH("k_rden", k_rden, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, vf, vf2, w_stats, per_item, total, rden);
     return check_launch("zb_so3_denominators/rden");
 }

@@ -248,20 +248,20 @@ extern "C" int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame
     float2* sums = (float2*)ws;
     float4* spectra = (float4*)(ws + ((int64_t)n_items * 8 + 255) / 256 * 256);
     const float2* tw = (const float2*)band->twiddle;
-    hipLaunchKernelGGL(k_sums, dim3(n_items), dim3(256), 0, s, a, band->ring, tw, pats, (const float4*)items, sums);
+    ZB_LAUNCH("k_sums", k_sums, dim3(n_items), dim3(256), 0, s, a, band->ring, tw, pats, (const float4*)items, sums);
     if (int rc = check_launch("zb_sh_coefficients/sums")) return rc;
     const int nh = n / 2;
     const int nphi = 2 * n;
     int G = 2048 / nphi;
     if (G < 1) G = 1;
     if (G > nh) G = nh;
-    hipLaunchKernelGGL(k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,
+    ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,
                        (const float4*)items, (const float2*)sums, spectra);
     if (int rc = check_launch("zb_sh_coefficients/rings")) return rc;
     constexpr int TB = 16;
-    hipLaunchKernelGGL(k_legendre<TB>, dim3((n_items + TB - 1) / TB, L), dim3(64), (size_t)TB * nh * 16, s, a, n_items, band->wplm,
+    ZB_LAUNCH("k_legendre", k_legendre<TB>, dim3((n_items + TB - 1) / TB, L), dim3(64), (size_t)TB * nh * 16, s, a, n_items, band->wplm,
                        (const float4*)spectra, (float2*)alm);
     if (int rc = check_launch("zb_sh_coefficients/legendre")) return rc;
-    hipLaunchKernelGGL(k_norm, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
+    ZB_LAUNCH("k_norm", k_norm, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
     return check_launch("zb_sh_coefficients/norm");
 }
