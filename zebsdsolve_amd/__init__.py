@@ -1,0 +1,400 @@
+"""zebsdsolve_amd: the EBSD indexing hot path of zebsdsolve on one AMD MI355X (gfx950).
+
+The surface mirrors the reference's Python package (python/zebsdsolve/__init__.py, api.py) for
+the indexing path::
+
+    import zebsdsolve_amd as ze
+
+    res = ze.index(patterns, ["Ni_master_20kV.npz"], pc=(0.5, 0.5, 0.55), bandwidth=64)
+    res.quaternions, res.ncc, res.phase
+
+`Solver` is the engine driven a step at a time: set patterns (or synthesize a toy scan), set the
+geometry, add phases, index. Every kernel runs from libzebsd_hip.so (hand-written HIP for gfx950)
+on torch's current stream; there is no CPU path.
+
+Conventions (as the reference's): quaternions (w, x, y, z) map sample directions to crystal
+directions; Bunge angles in radians; map cells numbered row-major.
+"""
+import numpy as np
+
+from . import _native
+from . import rot as _rot
+from . import sym as _sym
+from .api import Result, index
+from .settings import SettingError
+
+__version__ = "0.1.0"
+
+METHODS = ("spherical",)
+
+
+class Error(RuntimeError):
+    """An engine call failed; the message says which and why."""
+
+
+def _hash_u32(x):
+    """PCG-style integer hash (detector.wgsl:284-288), on uint32 arrays."""
+    x = np.asarray(x, np.uint64) & 0xFFFFFFFF
+    v = (x * 747796405 + 2891336453) & 0xFFFFFFFF
+    w = (((v >> ((v >> 28) + 4)) ^ v) * 277803737) & 0xFFFFFFFF
+    return ((w >> 22) ^ w) & 0xFFFFFFFF
+
+
+def _unit_rand(x):
+    return (_hash_u32(x) >> 8).astype(np.float64) / 16777216.0
+
+
+def decoys(n, draw):
+    """Uniform random orientations (Shoemake) for items 0..n-1 of draw `draw`, a different one
+    per item and draw (detector.wgsl decoy, 296-308)."""
+    b = np.arange(n, dtype=np.uint64)
+    s = _hash_u32(b + 1) ^ ((draw * 0x9E3779B9) & 0xFFFFFFFF)
+    u1 = _unit_rand(s)
+    a2 = _unit_rand(s + 1) * 2 * np.pi
+    a3 = _unit_rand(s + 2) * 2 * np.pi
+    ra, rb = np.sqrt(1 - u1), np.sqrt(u1)
+    return np.stack([ra * np.sin(a2), ra * np.cos(a2), rb * np.sin(a3), rb * np.cos(a3)], -1).astype(np.float32)
+
+
+class Solver:
+    """One engine on one GPU. Use as a context manager or call close()."""
+
+    def __init__(self):
+        from . import engine
+        self._e = engine.Engine()
+        self._chance = None
+        self._chance_key = None
+        self._res = None
+        self._idx = None
+
+    def close(self):
+        self._e = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ── settings and geometry ──
+
+    def configure(self, **settings):
+        """Set options by name (zebsdsolve_amd/settings.py lists them with the reference's names)."""
+        try:
+            self._e.settings.apply(**settings)
+        except SettingError as ex:
+            raise Error(f"configure: {ex}") from None
+        return self
+
+    @property
+    def settings(self):
+        return self._e.settings.as_dict()
+
+    @property
+    def geometry(self):
+        return self._e.geo.as_dict()
+
+    def set_geometry(self, **fields):
+        """Set geometry fields: pc (3), sample_tilt_deg, detector_tilt_deg, rot (3), px_um,
+        step_x_um, step_y_um."""
+        g = self._e.geo
+        for k, v in fields.items():
+            if k in ("pc", "rot"):
+                v = tuple(float(x) for x in v)
+                if len(v) != 3:
+                    raise Error(f"geometry field {k} takes three numbers")
+                setattr(g, k, v)
+            elif k in ("sample_tilt_deg", "detector_tilt_deg", "px_um", "step_x_um", "step_y_um"):
+                setattr(g, k, float(v))
+            else:
+                raise Error(f"geometry has no field {k!r}")
+        return self
+
+    # ── scan ──
+
+    def set_patterns(self, patterns):
+        """Patterns (rows, cols, h, w) or (n, h, w): uint8, uint16 or float32, a numpy array or a
+        device tensor (kept on the device)."""
+        try:
+            self._e.set_patterns(patterns)
+        except ValueError as ex:
+            raise Error(str(ex)) from None
+        self._res = None
+        return self
+
+    @property
+    def scan(self):
+        if self._e.shape is None:
+            raise Error("no scan")
+        rows, cols, h, w = self._e.shape
+        p = self._e.patterns
+        dt = str(p.dtype).replace("torch.", "")
+        return {"rows": rows, "cols": cols, "pattern_shape": (h, w), "dtype": dt}
+
+    def shape(self, full=False):
+        return self._e.pattern_hw()
+
+    @property
+    def pattern_shape(self):
+        return self.shape()
+
+    # ── phases ──
+
+    def add_phase(self, path):
+        """Load a master pattern (ebsdsim .npz); returns its index."""
+        try:
+            return self._e.add_phase(str(path))
+        except (OSError, ValueError) as ex:
+            raise Error(f"add_phase: {ex}") from None
+
+    def clear_phases(self):
+        self._e.phases = []
+
+    @property
+    def phases(self):
+        out = []
+        for p in self._e.phases:
+            m = p.master
+            out.append({"name": m.name, "side": m.side, "point_group": m.point_group, "laue": m.laue,
+                        "rotations": len(m.rotations), "voltage_kv": m.voltage_kv, "lattice": m.lattice})
+        return out
+
+    def master(self, phase=0):
+        """The phase's master as indexing compares patterns with it (smoothed for the pattern size
+        and geometry): float32 [2, side, side]."""
+        _, (_, _, host) = self._e.baked(phase)
+        return host.copy()
+
+    def simulate(self, quaternions, cells=None, phase=0, full=False):
+        """Simulated patterns (n, h, w) of a phase at the given orientations, each as map cell
+        cells[k] would record it."""
+        q = np.asarray(quaternions, np.float32).reshape(-1, 4)
+        cells = np.zeros(len(q), np.int64) if cells is None else np.asarray(cells, np.int64).reshape(-1)
+        if len(cells) != len(q):
+            raise Error("simulate: one cell per quaternion")
+        return self._e.simulate(phase, cells, q)
+
+    def synthesize(self, **settings):
+        """A toy scan from the phases loaded (settings toy_*), in place of any scan: grains by the
+        nearest of toy_grains random points, each a random orientation and phase, every cell
+        simulated under the present geometry, scaled to darkest pixel 0 and mean 1, blended with a
+        background, and Poisson-counted at toy_counts (src/engine.zig:2771-2898). Returns the
+        answer: quaternions (rows, cols, 4), phase (rows, cols; -1 blank), grain, geometry.
+        The random streams are numpy's (the reference draws from Zig's)."""
+        if settings:
+            self.configure(**settings)
+        s = self.settings
+        if not self._e.phases:
+            raise Error("synthesize: no phases")
+        rows, cols, h, w = s["toy_rows"], s["toy_cols"], s["toy_height"], s["toy_width"]
+        n = rows * cols
+        self._e.set_patterns(np.zeros((rows, cols, h, w), np.float32))
+        r = np.random.default_rng(s["toy_seed"])
+        ng = min(s["toy_grains"], n)
+        gx = r.random(ng) * cols
+        gy = r.random(ng) * rows
+        gq = _rot.qstd(r.normal(size=(ng, 4)))
+        gp = r.integers(0, len(self._e.phases), ng)
+        yy, xx = np.meshgrid(np.arange(rows) + 0.5, np.arange(cols) + 0.5, indexing="ij")
+        d = (gx[None, None, :] - xx[..., None]) ** 2 + (gy[None, None, :] - yy[..., None]) ** 2
+        grain = np.argmin(d, axis=-1).reshape(-1)
+        q = gq[grain]
+        spread = np.radians(s["toy_spread_deg"]) / np.sqrt(3.0)
+        if spread > 0:
+            q = _rot.qstd(_rot.qmul(_rot.so3_exp(spread * r.normal(size=(n, 3))), q))
+        phase = gp[grain].astype(np.int32)
+        if s["toy_blank"] > 0:
+            phase = np.where(r.random(n) < s["toy_blank"], -1, phase).astype(np.int32)
+        data = np.zeros((n, h, w), np.float64)
+        for p in range(len(self._e.phases)):
+            sel = np.nonzero(phase == p)[0]
+            for c0 in range(0, len(sel), 16384):
+                cs = sel[c0:c0 + 16384]
+                data[cs] = self._e.simulate(p, cs, q[cs].astype(np.float32))
+        crystals = phase >= 0
+        mean = data[crystals].mean() if crystals.any() else 0.0
+        low = data[crystals].min() if crystals.any() else 0.0
+        gain = 1.0 / (mean - low) if crystals.any() and mean > low else 0.0
+        back = np.ones((h, w))
+        if s["toy_background_sigma"] > 0:
+            sg = s["toy_background_sigma"] * w
+            by, bx = np.meshgrid(np.arange(h) + 0.5 - 0.5 * h, np.arange(w) + 0.5 - 0.5 * w, indexing="ij")
+            back = np.exp(-(bx * bx + by * by) / (2 * sg * sg))
+            back *= h * w / back.sum()
+        signal = np.where((crystals & (gain > 0))[:, None, None], (data - low) * gain, 1.0)
+        inten = (1 - s["toy_contrast"]) * back[None] + s["toy_contrast"] * signal
+        rn = np.random.default_rng(s["toy_noise_seed"] if s["toy_noise_seed"] else (s["toy_seed"] ^ 0x9E3779B97F4A7C15))
+        out = rn.poisson(s["toy_counts"] * inten).astype(np.float32) if s["toy_counts"] > 0 else inten.astype(np.float32)
+        self._e.set_patterns(out.reshape(rows, cols, h, w))
+        self._res = None
+        return {"quaternions": q.astype(np.float32).reshape(rows, cols, 4), "phase": phase.reshape(rows, cols),
+                "grain": grain.astype(np.int32).reshape(rows, cols), "geometry": self.geometry}
+
+    # ── chance and acceptance ──
+
+    def _calibrate(self):
+        """What NMI chance gives per phase: about 256 patterns of the map on a lattice, each at 8
+        random orientations, median and 1.4826 x median deviation (src/engine.zig:2202-2292)."""
+        e = self._e
+        key = (e.shape, tuple(e.geo.as_dict()["pc"]), e.geo.sample_tilt_deg, e.geo.detector_tilt_deg, tuple(e.geo.rot), e.geo.px_um,
+               len(e.phases), e.settings["nmi_bins"], e.settings["mask"], e.settings["mask_radius"], e.settings["master_smooth"], id(e.patterns))
+        if self._chance is not None and self._chance_key == key:
+            return self._chance
+        rows, cols = e.shape[0], e.shape[1]
+        region = rows * cols
+        n = min(region, 256)
+        cells = np.array([(2 * j + 1) * region // (2 * n) for j in range(n)], np.int64)
+        med, spread = [], []
+        for pi in range(len(e.phases)):
+            vals = []
+            for d in range(8):
+                vals.append(e.nmi_at(pi, cells, decoys(n, d)))
+            c = np.sort(np.concatenate(vals).astype(np.float32))
+            m = c[len(c) // 2]
+            dev = np.sort(np.abs(c - m))
+            med.append(float(m))
+            spread.append(max(1.4826 * float(dev[len(c) // 2]), 1e-6))
+        self._chance = (np.array(med, np.float32), np.array(spread, np.float32))
+        self._chance_key = key
+        return self._chance
+
+    def chance(self, phase=0):
+        med, spread = self._calibrate()
+        return float(med[phase]), float(spread[phase])
+
+    # ── indexing ──
+
+    def index_cells(self, cells, method=None, bandwidth=None, dict_grid=None):
+        """Index map cells without storing anything: arrays (phases, n[, 4]) of quaternions (raw,
+        not reduced), ncc, nmi, step (radians of the last improving refinement step) and z (NMI
+        above chance in chance's standard deviations) (python/zebsdsolve/__init__.py:666-683)."""
+        if method not in (None, "spherical"):
+            raise Error(f"index_cells: method {method!r} is outside this build (spherical only)")
+        if not self._e.phases:
+            raise Error("index_cells: no phases")
+        rec = self._e.index_cells(np.asarray(cells, np.int64).reshape(-1), bandwidth)
+        med, spread = self._calibrate()
+        z = (rec[..., 5] - med[:, None]) / spread[:, None]
+        return {"quaternions": rec[..., :4].copy(), "ncc": rec[..., 4].copy(), "nmi": rec[..., 5].copy(), "step": rec[..., 6].copy(), "z": z}
+
+    def _sweep_bandwidth(self, i):
+        s = self.settings
+        return s["bandwidth"]
+
+    def index_begin(self):
+        if not self._e.phases:
+            raise Error("index: no phases")
+        rows, cols = self._e.shape[0], self._e.shape[1]
+        n = rows * cols
+        self._res = {"quat": np.zeros((n, 4), np.float32), "ncc": np.zeros(n, np.float32), "nmi": np.zeros(n, np.float32),
+                     "phase": np.full(n, -1, np.int32), "z": np.zeros(n, np.float32), "sweep": np.full(n, -1, np.int32),
+                     "settled": np.zeros(n, bool)}
+        self._todo = np.arange(n, dtype=np.int64)
+        self._done = 0
+        self._accepted = 0
+
+    def index_step(self):
+        """Index the next few batches; returns the cells left (0: done) (src/engine.zig:2426-2465)."""
+        if self._res is None:
+            self.index_begin()
+        left = len(self._todo) - self._done
+        if left == 0:
+            return 0
+        s = self.settings
+        n = min(left, max(s["batch"], 1) * max(2, min(16, 4096 // max(s["batch"], 1))) if s["batch"] < 4096 else left)
+        n = max(n, min(left, 16384))
+        cells = self._todo[self._done:self._done + n]
+        out = self.index_cells(cells)
+        self._land(cells, out)
+        self._done += n
+        return len(self._todo) - self._done
+
+    def _land(self, cells, out):
+        """Each cell keeps the better match, the accepted ones their sweep (src/engine.zig:2390-2419)."""
+        s = self.settings
+        res = self._res
+        settle = np.radians(s["accept_step_deg"])
+        score = out["nmi"] if s["phase_by"] == "nmi" else out["ncc"]
+        n_ph = score.shape[0]
+        p = np.zeros(score.shape[1], np.int64)
+        best = np.full(score.shape[1], -np.inf, np.float32)
+        for k in range(n_ph):
+            better = score[k] > best
+            p = np.where(better, k, p)
+            best = np.where(better, score[k], best)
+        ar = np.arange(len(cells))
+        nmi, z, step = out["nmi"][p, ar], out["z"][p, ar], out["step"][p, ar]
+        good = nmi >= s["accept_nmi"] if s["accept_nmi"] > 0 else z >= s["accept_z"]
+        take = (res["phase"][cells] < 0) | (good & (step <= settle)) | (z > res["z"][cells])
+        for k in range(n_ph):
+            sel = take & (p == k)
+            if not sel.any():
+                continue
+            q = _sym.to_fz(out["quaternions"][k, ar[sel]].astype(np.float64), self._e.phases[k].master.rotations)
+            c = cells[sel]
+            res["quat"][c] = q.astype(np.float32)
+            res["ncc"][c] = out["ncc"][k, ar[sel]]
+            res["nmi"][c] = nmi[sel]
+            res["z"][c] = z[sel]
+            res["phase"][c] = k
+            res["settled"][c] = step[sel] <= settle
+        kept_good = res["nmi"][cells] >= s["accept_nmi"] if s["accept_nmi"] > 0 else res["z"][cells] >= s["accept_z"]
+        newly = (res["sweep"][cells] < 0) & kept_good & res["settled"][cells]
+        res["sweep"][cells[newly]] = 0
+        self._accepted += int(newly.sum())
+
+    @property
+    def index_status(self):
+        total = 0 if self._res is None else len(self._todo)
+        return {"sweep": 0, "sweeps": 1, "bandwidth": self.settings["bandwidth"], "dict_grid": 0, "done": getattr(self, "_done", 0),
+                "total": total, "region": total, "accepted": getattr(self, "_accepted", 0), "method": "spherical"}
+
+    def index(self, progress=None):
+        """Index the whole map a few batches at a time; progress(status) after each step, False
+        from it stops. Returns results()."""
+        self.index_begin()
+        while True:
+            left = self.index_step()
+            if progress is not None and progress(self.index_status) is False:
+                break
+            if left == 0:
+                break
+        return self.results()
+
+    def results(self):
+        """quaternions (rows, cols, 4) in each phase's fundamental zone, euler (Bunge, radians),
+        ncc, nmi, phase (-1: not indexed), z, sweep (-1: no accepted match)."""
+        if self._res is None:
+            raise Error("results: nothing indexed")
+        rows, cols = self._e.shape[0], self._e.shape[1]
+        r = self._res
+        return {"quaternions": r["quat"].reshape(rows, cols, 4).copy(),
+                "euler": _rot.to_bunge(r["quat"].astype(np.float64)).astype(np.float32).reshape(rows, cols, 3),
+                "ncc": r["ncc"].reshape(rows, cols).copy(), "nmi": r["nmi"].reshape(rows, cols).copy(),
+                "phase": r["phase"].reshape(rows, cols).copy(), "z": r["z"].reshape(rows, cols).copy(),
+                "sweep": r["sweep"].reshape(rows, cols).copy()}
+
+    # ── symmetry ──
+
+    def reduce(self, quaternions, phase=0):
+        """Each orientation's fundamental-zone representative under the phase's rotations."""
+        q = np.asarray(quaternions, np.float64)
+        return _sym.to_fz(q.reshape(-1, 4), self._e.phases[phase].master.rotations).reshape(q.shape).astype(np.float32)
+
+    def disorientation(self, a, b, phase=0):
+        """Disorientation angles (radians) between orientations a and b of one phase."""
+        a = np.asarray(a, np.float64).reshape(-1, 4)
+        b = np.asarray(b, np.float64).reshape(-1, 4)
+        return _sym.disorientation(a, b, self._e.phases[phase].master.rotations)
+
+    # ── timing ──
+
+    def profile(self, on=True):
+        _native.profile(on)
+
+    def profile_report(self):
+        """{kernel: (launches, total ms)} since profile(True)."""
+        return _native.profile_report()
+
+
+__all__ = ["Solver", "Result", "index", "Error", "decoys", "__version__"]

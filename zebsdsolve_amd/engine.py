@@ -326,6 +326,30 @@ class Engine:
             raise RuntimeError("DeviceSkippedBatch: records without this batch's stamp")
         return out
 
+    def nmi_at(self, pi, cells, quats):
+        """NMI of each cell's pattern with phase pi's master at the given orientation (no
+        refinement): zb_refine with no iterations. Returns float32 [n]."""
+        lib = nat.load()
+        dev = self._device()
+        cells = np.asarray(cells, np.int64).reshape(-1)
+        n = len(cells)
+        raw = self._raw_for(cells)
+        h, w = self.pattern_hw()
+        pats = torch.empty((n, h, w), dtype=torch.float32, device=dev)
+        mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
+        st = stream_ptr()
+        nat.check(lib.zb_normalize(raw.data_ptr(), TORCH_PIXEL_KIND[raw.dtype], n, h, w, mask, pats.data_ptr(), st), "normalize")
+        t = self.positions(cells)
+        items = torch.from_numpy(np.concatenate([t, np.arange(n, dtype=np.float32)[:, None]], 1)).to(dev)
+        seeds = torch.from_numpy(np.ascontiguousarray(np.asarray(quats, np.float32).reshape(n, 4))).to(dev)
+        o = self.refine_opts()
+        o.iters = 0
+        rec = torch.empty((n, RECORD), dtype=torch.float32, device=dev)
+        _, (_, grad, _) = self.baked(pi)
+        nat.check(lib.zb_refine(grad.data_ptr(), self.phases[pi].master.side, self.frame(), pats.data_ptr(), items.data_ptr(),
+                                seeds.data_ptr(), 4, n, o, rec.data_ptr(), 0, st), "nmi")
+        return rec[:, 5].cpu().numpy()
+
     def simulate(self, pi, cells, quats):
         """Simulated patterns of phase pi at the given orientations, as cells would record them
         (src/engine.zig:2753-2769). Returns float32 (n, h, w)."""
