@@ -1,0 +1,96 @@
+"""The Solver / index surface on the GPU against ground truth, with the reference's own thresholds
+(tests/test_api.py): toy scans recovered to median < 0.15 deg and max < 1 deg, every cell indexed
+in the fundamental zone; noise cells far below chance + 15 sigma and patterns far above; two
+candidate phases told apart; results independent of the batch size."""
+import numpy as np
+import pytest
+
+from tests.conftest import GAN, NI, gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+def _solver():
+    import zebsdsolve_amd as ze
+    s = ze.Solver()
+    s.set_geometry(pc=(0.44, 0.24, 0.53), sample_tilt_deg=70, detector_tilt_deg=0, px_um=70 * 8, step_x_um=20, step_y_um=20)
+    return s
+
+
+def test_toy_scan_is_recovered_in_the_fundamental_zone():
+    import zebsdsolve_amd as ze
+    s = _solver()
+    s.add_phase(NI)
+    truth = s.synthesize(toy_rows=8, toy_cols=12, toy_grains=96, toy_counts=400, toy_seed=3)
+    r = s.index()
+    q = r["quaternions"].reshape(-1, 4)
+    d = np.degrees(s.disorientation(q, truth["quaternions"].reshape(-1, 4)))
+    assert np.median(d) < 0.15 and d.max() < 1.0, (np.median(d), d.max())
+    assert (r["phase"] == 0).all()
+    assert (r["quaternions"][..., 0] >= np.cos(np.radians(62.81) / 2) - 1e-4).all()
+    assert r["euler"].shape == (8, 12, 3) and r["ncc"].shape == (8, 12)
+    # the peak alone is within the grid's reach
+    s.configure(refine=False)
+    r0 = s.index()
+    d0 = np.degrees(s.disorientation(r0["quaternions"].reshape(-1, 4), truth["quaternions"].reshape(-1, 4)))
+    assert np.median(d0) < 1.5 and np.percentile(d0, 95) < 3.0
+    s.configure(refine=True)
+    # batch-size independence (test_api.py:511-513)
+    one = s.index_cells(np.arange(96))
+    part = s.index_cells(np.arange(40, 56))
+    assert np.allclose(one["quaternions"][0, 40:56], part["quaternions"][0], atol=1e-6)
+    with pytest.raises(ze.Error):
+        s.configure(no_such_option=1)
+    with pytest.raises(ze.Error):
+        s.configure(bandwidth=3)
+    assert s.settings["bandwidth"] == 64
+
+
+def test_noise_is_not_accepted_and_patterns_are():
+    s = _solver()
+    s.add_phase(NI)
+    truth = s.synthesize(toy_rows=8, toy_cols=12, toy_grains=96, toy_counts=400, toy_seed=5)
+    rng = np.random.default_rng(0)
+    pats = s._e.patterns.reshape(96, 60, 60).copy()
+    noise = rng.choice(96, 32, replace=False)
+    pats[noise] = rng.normal(400, 20, size=(32, 60, 60)).astype(np.float32)
+    s.set_patterns(pats.reshape(8, 12, 60, 60))
+    r = s.index()
+    z = r["z"].reshape(-1)
+    is_noise = np.zeros(96, bool)
+    is_noise[noise] = True
+    assert z[~is_noise].min() > 15 and z[is_noise].max() < 8, (z[~is_noise].min(), z[is_noise].max())
+    assert ((r["sweep"].reshape(-1) >= 0) == ~is_noise).all()
+    one = s.index_cells(np.arange(96))
+    assert np.degrees(one["step"][0][~is_noise]).max() < 0.2
+
+
+def test_two_phases_are_told_apart():
+    s = _solver()
+    s.add_phase(NI)
+    s.add_phase(GAN)
+    truth = s.synthesize(toy_rows=8, toy_cols=12, toy_grains=96, toy_counts=400, toy_seed=9)
+    r = s.index()
+    ph = r["phase"].reshape(-1)
+    tp = truth["phase"].reshape(-1)
+    assert (ph == tp).mean() > 0.97
+    ok = ph == tp
+    d = []
+    for p in (0, 1):
+        sel = ok & (tp == p)
+        if sel.any():
+            d.append(np.degrees(s.disorientation(r["quaternions"].reshape(-1, 4)[sel], truth["quaternions"].reshape(-1, 4)[sel], phase=p)))
+    d = np.concatenate(d)
+    assert np.median(d) < 0.3
+
+
+def test_index_function_returns_a_result():
+    import zebsdsolve_amd as ze
+    s = _solver()
+    s.add_phase(NI)
+    truth = s.synthesize(toy_rows=4, toy_cols=5, toy_grains=20, toy_counts=400, toy_seed=2)
+    pats = s._e.patterns
+    res = ze.index(pats, [NI], pc=(0.44, 0.24, 0.53), px_um=560, step_x_um=20, step_y_um=20, bandwidth=48)
+    assert res.quaternions.shape == (4, 5, 4) and res.phase.shape == (4, 5)
+    d = np.degrees(s.disorientation(res.quaternions.reshape(-1, 4), truth["quaternions"].reshape(-1, 4)))
+    assert np.median(d) < 0.15
