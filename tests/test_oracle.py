@@ -236,3 +236,27 @@ def test_normalize_is_zero_mean_unit_norm_under_the_mask():
     r, c = np.meshgrid(np.arange(16), np.arange(20), indexing="ij")
     m = op.in_mask(r, c, fr["mask"])
     assert abs(p[m].mean()) < 1e-15 and abs((p * p).sum() - 1) < 1e-12 and np.all(p[~m] == 0)
+
+
+def test_oracle_recovers_known_orientations_end_to_end():
+    # the nickel master, two patterns at known orientations: the correlation peak lands within the
+    # grid's reach (L = 32: 5.6 deg steps) and the refinement on it converges to the truth
+    from tests import cases
+    from tests.conftest import NI
+    from zebsdsolve_amd import master, sym
+    m = master.load(NI)
+    L = 32
+    pats, truth = cases.simulate(m.raw, m.side, 2, seed=3)
+    fr, t = cases.oracle_frame(60, 60)
+    f, f2, fold, _ = op.harmonics(m.raw, m.side, L)
+    so = op.So3(L, fold)
+    wa, wst = op.coefficients(None, fr, t, L, center=False, norm=False, window=True)
+    rden = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])
+    grad = op.gradient(m.raw, m.side)
+    for b in range(2):
+        pn = op.normalize(pats[b].astype(np.float64), fr["mask"])
+        alm, _ = op.coefficients(pn, fr, t, L)
+        q, v0, _, _ = so.peak(so.volume(f, alm), rden)
+        assert np.degrees(sym.disorientation(q, truth[b], m.rotations)) < 3.0
+        qr, ncc, _ = op.refine(grad, m.side, fr, pn, t, q, iters=8)
+        assert np.degrees(sym.disorientation(qr, truth[b], m.rotations)) < 0.02 and ncc > 0.99
