@@ -68,6 +68,25 @@ typedef struct zb_phase {
     const float* w_f2; /* the same for the master's square */
 } zb_phase;
 
+/* A phase's correlation as a GEMM table at a bandwidth: the reference's transforms along b and c
+ * of the spectrum (so3.wgsl spectrum, a_peak_m) folded into a pattern-independent table,
+ *   Q[j][m'][c][l] = sum_mu f_{l, fold mu} d^l_{m', fold mu}(2 pi j / n) e^{-2 pi i mu c / nc},
+ * so that V[j][m'][c] = sum_{l >= m'} Q[j][m'][c][l] conj(g_{l m'}). Stored as the matrix-core
+ * B operand: q[((j L + m') slots + s) 2 + r][cols] f32 for l = s l_step, row r = 0:
+ * [Re Q | Im Q | 0], r = 1: [Im Q | -Re Q | 0] (c innermost, cols = 32 ceil(2 nc / 32)).
+ * Built once per phase and bandwidth (zebsdsolve_amd/tables.py VTableHost). No counterpart in
+ * the reference's tables; it replaces W_f of spherical.PhaseGpu (src/spherical.zig:192-242) on
+ * this path. */
+typedef struct zb_vtable {
+    int32_t L;
+    int32_t n;      /* fft_size(L), at most 128 on this path */
+    int32_t nc;     /* n / fold */
+    int32_t l_step; /* 2 when the master has no odd l, else 1 */
+    int32_t slots;  /* ceil(L / l_step) */
+    int32_t cols;
+    const float* q; /* [n/2 + 1][L][slots][2][cols] */
+} zb_vtable;
+
 /* Gauss-Newton refinement options (detector.RefineOptions, src/detector.zig:57-70). */
 typedef struct zb_refine_opts {
     int32_t iters;    /* Gauss-Newton iterations (0: score the seeds only) */
@@ -124,6 +143,26 @@ int32_t zb_so3_correlate(const zb_band* band, const zb_phase* phase, const float
  * Replaces Spherical.denominators (src/spherical.zig:446-464: so3.wgsl a_volume, rden_build). */
 int32_t zb_so3_denominators(const zb_band* band, const zb_phase* phase, const float* w_alm, const float* w_stats,
                             int32_t n_windows, void* work, int64_t work_bytes, float* rden, void* stream);
+
+/* Bytes of workspace zb_vcorr_correlate (denominators = 0) or zb_vcorr_denominators (1) needs
+ * for n_items. */
+int64_t zb_vcorr_workspace(const zb_vtable* table, int32_t n_items, int32_t denominators);
+
+/* zb_so3_correlate on the matrix cores: V = the table times each item's coefficients (float32
+ * MFMA), then the real transform along a, times the reciprocal denominator, argmax (ties: lowest
+ * index (j nc + c) n + a) and the parabolic sub-cell peak, as zb_so3_correlate.
+ * twiddle: [1024][2] (zb_band.twiddle); rden: [*][n/2+1][n][nc] f32 in natural order (a, then c
+ * innermost), as zb_vcorr_denominators writes it; peaks: [n_items][8] out, as zb_so3_correlate.
+ * Replaces Spherical.correlate (src/spherical.zig:401-432: so3.wgsl spectrum, a_peak_m,
+ * peak_reduce, peak_finish_m). */
+int32_t zb_vcorr_correlate(const zb_vtable* table, const float* twiddle, const float* alm, int32_t n_items, const float* rden,
+                           int32_t rden_div, void* work, int64_t work_bytes, float* peaks, void* stream);
+
+/* zb_so3_denominators on the matrix cores, from the tables of the master (t_f) and of its
+ * square (t_f2): rden [n_windows][n/2+1][n][nc] out, natural order.
+ * Replaces Spherical.denominators (src/spherical.zig:446-464: so3.wgsl a_volume, rden_build). */
+int32_t zb_vcorr_denominators(const zb_vtable* t_f, const zb_vtable* t_f2, const float* w_alm, const float* w_stats,
+                              int32_t n_windows, void* work, int64_t work_bytes, float* rden, void* stream);
 
 /* Gauss-Newton refinement of each item's orientation on the detector, then its NMI, into one
  * record per item: records[(out0 + b) * 8 ..] = best q (w, x, y, z), NCC, NMI, last improving

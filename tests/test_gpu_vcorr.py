@@ -1,0 +1,123 @@
+"""The matrix-core correlation (zb_vcorr_*) against the float64 oracle and against the slab path
+(zb_so3_*), for a centrosymmetric cubic master (fold 4, even l only) and a hexagonal one without
+inversion (fold 2, every l), at L = 32 and at the benchmark's L = 64.
+
+Tolerances: float32 GEMM and transforms against float64, stated at each assertion.
+"""
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.conftest import GAN, NI, gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+CASES = [("Ni", NI, 32, 10), ("GaN", GAN, 32, 10), ("Ni", NI, 64, 4)]
+
+
+def _setup(path, L, N):
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import engine
+    from zebsdsolve_amd.geometry import Geometry
+
+    e = engine.Engine()
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(path)
+    e.settings.apply(bandwidth=L)
+    e.set_patterns(np.zeros((1, N, 60, 60), np.float32))
+    _, (_, _, host) = e.baked(0)
+    side = e.phases[0].master.side
+    pats, truth = cases.simulate(host, side, N, seed=21)
+    e.set_patterns(pats.reshape(1, N, 60, 60))
+    fr, t = cases.oracle_frame(60, 60)
+    torch.cuda.synchronize()
+    return e, op, host, side, pats, truth, fr, t
+
+
+def _coefficients(e, L, pats):
+    import torch
+    from zebsdsolve_amd import _native as nat
+    from zebsdsolve_amd.engine import stream_ptr
+    lib = nat.load()
+    N = pats.shape[0]
+    band = e.band(L)
+    raw = torch.from_numpy(pats).cuda()
+    pn = torch.empty_like(raw)
+    mask = (nat.ctypes.c_float * 3)(*e.mask_tuple())
+    nat.check(lib.zb_normalize(raw.data_ptr(), 2, N, 60, 60, mask, pn.data_ptr(), stream_ptr()), "normalize")
+    t = e.positions(np.arange(N))
+    items = torch.from_numpy(np.concatenate([t, np.arange(N, dtype=np.float32)[:, None]], 1)).cuda()
+    alm = torch.empty((N, band.host.pairs, 2), dtype=torch.float32, device="cuda")
+    stats = torch.empty((N, 4), dtype=torch.float32, device="cuda")
+    ws = torch.empty((lib.zb_sh_workspace(band.c, N),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_sh_coefficients(band.c, e.frame(), pn.data_ptr(), items.data_ptr(), N, 3, ws.data_ptr(), ws.numel(), alm.data_ptr(),
+                                     stats.data_ptr(), stream_ptr()), "coefficients")
+    return alm, pn.cpu().numpy()
+
+
+@pytest.mark.parametrize("name,path,L,N", CASES, ids=[f"{c[0]}-L{c[2]}" for c in CASES])
+def test_vcorr_against_oracle_and_slab_path(name, path, L, N):
+    import torch
+    from zebsdsolve_amd import _native as nat, sym
+    from zebsdsolve_amd.engine import stream_ptr
+    print(f"vcorr case {name} L={L} N={N}", flush=True)
+    e, op, host, side, pats, truth, fr, t = _setup(path, L, N)
+    lib = nat.load()
+    ph = e.phase_band(0, L)
+    band = e.band(L)
+    f, f2, fold, inv = op.harmonics(host, side, L)
+    assert fold == ph.host.fold and (ph.host.l_step == 2) == inv
+    so = op.So3(L, fold)
+    wa, wst = op.coefficients(None, fr, t, L, center=False, norm=False, window=True)
+    rden_ref = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])          # [j][c][a]
+    vr = e.vrden(0, L).cpu().numpy().reshape(so.n // 2 + 1, so.n, so.nc)     # [j][a][c]
+    rden = np.transpose(vr, (0, 2, 1))
+    big = rden_ref > 0
+    # the same cells are defined; reciprocal denominators agree to float32 GEMM and transforms
+    assert np.mean((rden > 0) == big) > 0.999
+    rel = np.abs(rden[big] - rden_ref[big]) / rden_ref[big]
+    assert np.median(rel) < 1e-4 and np.percentile(rel, 99) < 2e-3, (np.median(rel), np.percentile(rel, 99))
+
+    alm, pats_n = _coefficients(e, L, pats)
+    (vt, _), _ = ph.vtables()
+    peaks = torch.empty((N, 8), dtype=torch.float32, device="cuda")
+    ws = torch.empty((lib.zb_vcorr_workspace(vt, N, 0),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), alm.data_ptr(), N, e.vrden(0, L).data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                     peaks.data_ptr(), stream_ptr()), "vcorr")
+    pk = peaks.cpu().numpy()
+    # the slab path on the same coefficients
+    peaks2 = torch.empty((N, 8), dtype=torch.float32, device="cuda")
+    ws2 = torch.empty((lib.zb_so3_workspace(band.c, ph.c, N),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), N, e.rden(0, L).data_ptr(), N, ws2.data_ptr(), ws2.numel(), peaks2.data_ptr(),
+                                   stream_ptr()), "so3")
+    pk2 = peaks2.cpu().numpy()
+    rots = e.phases[0].master.rotations
+    step = 360.0 / so.n
+    for b in range(N):
+        ref_alm, _ = op.coefficients(pats_n[b].astype(np.float64), fr, t, L)
+        C = so.volume(f, ref_alm)
+        q, v0, ang, idx = so.peak(C, rden_ref)
+        # the oracle's peak: score to float32 accuracy, orientation within 1/100 of a grid step
+        assert abs(pk[b, 4] - v0) < 2e-4 * abs(v0), (b, pk[b, 4], v0)
+        d = np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), q, rots))
+        assert d < 0.01 * step, (b, d)
+        # the slab path: the same cell, score and sub-cell offsets to float32 rounding
+        assert abs(pk[b, 4] - pk2[b, 4]) < 2e-5 * abs(pk2[b, 4]), (b, pk[b, 4], pk2[b, 4])
+        d2 = np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), pk2[b, :4].astype(np.float64), rots))
+        assert d2 < 0.005 * step, (b, d2)
+
+
+def test_vcorr_argument_checks():
+    import torch
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
+    bad = nat.VTable(64, 256, 64, 1, 64, 128, 0)
+    assert lib.zb_vcorr_workspace(bad, 4, 0) < 0
+    assert "n <= 128" in nat.last_error()
+    x = torch.zeros(16, device="cuda")
+    good = nat.VTable(32, 64, 16, 2, 16, 32, x.data_ptr())
+    need = lib.zb_vcorr_workspace(good, 4, 0)
+    assert need > 0
+    rc = lib.zb_vcorr_correlate(good, x.data_ptr(), x.data_ptr(), 4, x.data_ptr(), 4, x.data_ptr(), need - 1, x.data_ptr(), None)
+    assert rc == -2 and "workspace" in nat.last_error()

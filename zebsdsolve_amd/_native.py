@@ -28,6 +28,10 @@ class Phase(ctypes.Structure):
     _fields_ = [("fold", i32), ("nc", i32), ("mu_max", i32), ("l_step", i32), ("w_f", vp), ("w_f2", vp)]
 
 
+class VTable(ctypes.Structure):
+    _fields_ = [("L", i32), ("n", i32), ("nc", i32), ("l_step", i32), ("slots", i32), ("cols", i32), ("q", vp)]
+
+
 class RefineOpts(ctypes.Structure):
     _fields_ = [("iters", i32), ("max_step", ctypes.c_float), ("scale0", ctypes.c_float), ("scale_min", ctypes.c_float),
                 ("scale_max", ctypes.c_float), ("nmi_bins", i32), ("stamp", ctypes.c_uint32)]
@@ -42,6 +46,9 @@ _SIGS = {
     "zb_so3_workspace": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), i32], i64),
     "zb_so3_correlate": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_so3_denominators": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, vp, i32, vp, i64, vp, vp], i32),
+    "zb_vcorr_workspace": ([ctypes.POINTER(VTable), i32, i32], i64),
+    "zb_vcorr_correlate": ([ctypes.POINTER(VTable), vp, vp, i32, vp, i32, vp, i64, vp, vp], i32),
+    "zb_vcorr_denominators": ([ctypes.POINTER(VTable), ctypes.POINTER(VTable), vp, vp, i32, vp, i64, vp, vp], i32),
     "zb_refine": ([vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
     "zb_project": ([vp, i32, ctypes.POINTER(Frame), vp, i32, vp, vp], i32),
     "zb_master_blur": ([vp, vp, i32, ctypes.c_float, vp], i32),

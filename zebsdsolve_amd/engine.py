@@ -65,6 +65,27 @@ class DevicePhase:
         self.w_f2 = _dev(h.w_f2, dev)
         self.c = nat.Phase(h.fold, h.nc, h.mu_max, h.l_step, self.w_f.data_ptr(), self.w_f2.data_ptr())
         self.rden = {}
+        self.band = band
+        self.dev = dev
+        self._vt = None
+        self.vrden = {}
+
+    def vtables(self):
+        """The GEMM tables of the master and of its square (zb_vtable), built on first use."""
+        if self._vt is None:
+            out = []
+            for coef in (self.host.hm.f, self.host.hm.f2):
+                t = tables_mod.VTableHost(self.band.host, self.host, coef)
+                q = _dev(t.table, self.dev)
+                out.append((nat.VTable(t.L, t.n, t.nc, t.l_step, t.slots, t.cols, q.data_ptr()), q))
+            self._vt = out
+        return self._vt
+
+
+def mfma_correlation(L, n):
+    """Whether the matrix-core correlation takes this bandwidth (its transform along a is held in
+    registers: n <= 128)."""
+    return n <= 128
 
 
 class Phase:
@@ -84,6 +105,9 @@ class Engine:
         self.bands = {}
         self.stamp = 0
         self.dev = None
+        # "mfma": the correlation as a GEMM on the matrix cores (bandwidths with n <= 128);
+        # "slab": the spectrum + slab transforms (every bandwidth, and the fallback above 128)
+        self.correlator = "mfma"
 
     # ── state ──
 
@@ -213,33 +237,66 @@ class Engine:
             p.bands[k] = DevicePhase(self.band(L), p.master.side, host, self._device())
         return p.bands[k]
 
-    def rden(self, pi, L):
-        """The denominator volume of the normalized correlation for the reference cell's detector
-        position (src/engine.zig:1839-1849)."""
+    def _window(self, band):
+        """Harmonic coefficients and statistics of the reference cell's window (ones under the
+        mask)."""
         lib = nat.load()
-        band = self.band(L)
-        ph = self.phase_band(pi, L)
         fr = self.frame()
         t = self.translation().astype(np.float32)
-        key = (tuple(fr.c0), tuple(fr.c1), tuple(fr.c2), fr.pat_h, fr.pat_w, tuple(fr.mask), tuple(t))
-        if key in ph.rden:
-            return ph.rden[key]
         dev = self._device()
         win = torch.tensor([[t[0], t[1], t[2], 0.0]], dtype=torch.float32, device=dev)
-        pairs = band.host.pairs
-        w_alm = torch.empty((1, pairs, 2), dtype=torch.float32, device=dev)
+        w_alm = torch.empty((1, band.host.pairs, 2), dtype=torch.float32, device=dev)
         w_stats = torch.empty((1, 4), dtype=torch.float32, device=dev)
         ws = torch.empty((max(lib.zb_sh_workspace(band.c, 1), 1),), dtype=torch.uint8, device=dev)
         nat.check(lib.zb_sh_coefficients(band.c, fr, None, win.data_ptr(), 1, 4, ws.data_ptr(), ws.numel(), w_alm.data_ptr(),
                                          w_stats.data_ptr(), stream_ptr()), "window coefficients")
-        h = ph.host
+        return w_alm, w_stats
+
+    def _geo_key(self):
+        fr = self.frame()
+        t = self.translation().astype(np.float32)
+        return (tuple(fr.c0), tuple(fr.c1), tuple(fr.c2), fr.pat_h, fr.pat_w, tuple(fr.mask), tuple(t))
+
+    def rden(self, pi, L):
+        """The denominator volume of the normalized correlation for the reference cell's detector
+        position (src/engine.zig:1839-1849), in the slab path's order."""
+        lib = nat.load()
+        band = self.band(L)
+        ph = self.phase_band(pi, L)
+        key = self._geo_key()
+        if key in ph.rden:
+            return ph.rden[key]
+        w_alm, w_stats = self._window(band)
+        dev = self._device()
         n = band.host.n
-        rden = torch.empty((n // 2 + 1) * n * h.nc, dtype=torch.float32, device=dev)
+        rden = torch.empty((n // 2 + 1) * n * ph.host.nc, dtype=torch.float32, device=dev)
         ws2 = torch.empty((nat.check(lib.zb_so3_workspace(band.c, ph.c, 1), "so3 workspace"),), dtype=torch.uint8, device=dev)
         nat.check(lib.zb_so3_denominators(band.c, ph.c, w_alm.data_ptr(), w_stats.data_ptr(), 1, ws2.data_ptr(), ws2.numel(),
                                           rden.data_ptr(), stream_ptr()), "denominators")
         ph.rden[key] = rden
         return rden
+
+    def vrden(self, pi, L):
+        """The same volume for the matrix-core path, in natural order [j][a][c]."""
+        lib = nat.load()
+        band = self.band(L)
+        ph = self.phase_band(pi, L)
+        key = self._geo_key()
+        if key in ph.vrden:
+            return ph.vrden[key]
+        w_alm, w_stats = self._window(band)
+        dev = self._device()
+        (tf, _), (tf2, _) = ph.vtables()
+        n = band.host.n
+        rden = torch.empty((n // 2 + 1) * n * ph.host.nc, dtype=torch.float32, device=dev)
+        ws = torch.empty((nat.check(lib.zb_vcorr_workspace(tf, 1, 1), "vcorr workspace"),), dtype=torch.uint8, device=dev)
+        nat.check(lib.zb_vcorr_denominators(tf, tf2, w_alm.data_ptr(), w_stats.data_ptr(), 1, ws.data_ptr(), ws.numel(), rden.data_ptr(),
+                                            stream_ptr()), "vcorr denominators")
+        ph.vrden[key] = rden
+        return rden
+
+    def use_mfma(self, L):
+        return self.correlator == "mfma" and mfma_correlation(L, self.band(L).host.n)
 
     # ── the batch pipeline ──
 
@@ -275,14 +332,21 @@ class Engine:
         opts = self.refine_opts()
         records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
         phs = [self.phase_band(pi, L) for pi in range(n_ph)]
-        rdens = [self.rden(pi, L) for pi in range(n_ph)]
-        per_item = max(int(lib.zb_so3_workspace(band.c, ph.c, 1)) for ph in phs)
+        mfma = self.use_mfma(L)
+        if mfma:
+            rdens = [self.vrden(pi, L) for pi in range(n_ph)]
+            vts = [ph.vtables()[0][0] for ph in phs]
+            corr_ws = [lambda k, vt=vt: int(lib.zb_vcorr_workspace(vt, k, 0)) for vt in vts]
+        else:
+            rdens = [self.rden(pi, L) for pi in range(n_ph)]
+            corr_ws = [lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k)) for ph in phs]
+        per_item = max(f(1) for f in corr_ws)
         chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
         pairs = band.host.pairs
         alm = torch.empty((chunk, pairs, 2), dtype=torch.float32, device=dev)
         stats = torch.empty((chunk, 4), dtype=torch.float32, device=dev)
         peaks = torch.empty((chunk, 8), dtype=torch.float32, device=dev)
-        need = max(int(lib.zb_sh_workspace(band.c, chunk)), max(int(lib.zb_so3_workspace(band.c, ph.c, chunk)) for ph in phs))
+        need = max(int(lib.zb_sh_workspace(band.c, chunk)), max(f(chunk) for f in corr_ws))
         if workspace is None or workspace.numel() < need:
             workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
         for c0 in range(0, n, chunk):
@@ -291,8 +355,12 @@ class Engine:
             nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nt, 3, workspace.data_ptr(), workspace.numel(),
                                              alm.data_ptr(), stats.data_ptr(), st), "coefficients")
             for pi, ph in enumerate(phs):
-                nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
-                                               workspace.numel(), peaks.data_ptr(), st), "correlate")
+                if mfma:
+                    nat.check(lib.zb_vcorr_correlate(vts[pi], band.tw.data_ptr(), alm.data_ptr(), nt, rdens[pi].data_ptr(), nt,
+                                                     workspace.data_ptr(), workspace.numel(), peaks.data_ptr(), st), "vcorr correlate")
+                else:
+                    nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
+                                                   workspace.numel(), peaks.data_ptr(), st), "correlate")
                 _, (_, grad, _) = self.baked(pi)
                 rec = records[pi]
                 nat.check(lib.zb_refine(grad.data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(), 8,

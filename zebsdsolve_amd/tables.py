@@ -76,3 +76,78 @@ def volume_natural(vol, n, nc, n_items=1):
     out[:, :, c[None, :, None].repeat(h, 2)[0], (2 * p)[None, :].repeat(nc, 0)] = np.transpose(v[..., 0], (0, 1, 3, 2))
     out[:, :, c[None, :, None].repeat(h, 2)[0], (2 * p + 1)[None, :].repeat(nc, 0)] = np.transpose(v[..., 1], (0, 1, 3, 2))
     return out
+
+
+def d_beta(L, fold, wigner, betas):
+    """D[j, l, m', mu] = d^l_{m', fold mu}(beta_j) for m' in [0, L), |mu| <= (L-1)/fold, from the
+    table at pi/2 by d^l_{m'm}(beta) = i^{m'-m} sum_k d^l_{km'}(pi/2) d^l_{km}(pi/2) e^{-ik beta}
+    (src/sh.zig:508-518), with d^l_{-k,m} = (-1)^{l+m} d^l_{k,m} folding k < 0 onto k > 0."""
+    mu_max = (L - 1) // fold
+    mus = np.arange(-mu_max, mu_max + 1)
+    ms = fold * mus
+    am = np.abs(ms)
+    betas = np.asarray(betas, np.float64)
+    ks = np.arange(L)
+    out = np.zeros((len(betas), L, L, len(mus)))
+    cosk = np.cos(np.outer(betas, ks))           # [j, k]
+    sink = np.sin(np.outer(betas, ks))
+    for l in range(L):
+        # P[k, m'] = d^l_{k, m'} (k, m' >= 0); R[k, mu] = d^l_{k, m} for signed m
+        P = wigner[:l + 1, :, l]                                  # [k, m']
+        R = wigner[:l + 1, am, l].copy()                          # [k, mu], |m|
+        neg = ms < 0
+        # d^l_{k,-|m|} = (-1)^{l+k} d^l_{k,|m|}
+        R[:, neg] *= np.where((l + np.arange(l + 1)) % 2 == 1, -1.0, 1.0)[:, None]
+        wk = np.where(np.arange(l + 1) == 0, 1.0, 2.0)
+        # parity of m' + m decides whether the +-k terms add to a cosine or a sine
+        Sc = np.einsum("km,jk,ku->jmu", P, cosk[:, :l + 1] * wk, R, optimize=True)
+        Ss = np.einsum("km,jk,ku->jmu", P, sink[:, :l + 1] * wk, R, optimize=True)
+        mp = np.arange(L)[:, None]
+        par = (mp + ms[None, :]) % 2
+        # i^{m'-m} (sum_k ... e^{-ik beta}): even parity -> i^{m'-m} Sc (real), odd -> i^{m'-m} (-i Ss)
+        ph = (mp - ms[None, :]) % 4
+        even = np.where(ph == 0, 1.0, np.where(ph == 2, -1.0, 0.0))
+        # i^p (-i) = i^{p-1}: p = 1 gives 1, p = 3 gives -1 (p even has even parity)
+        odd = np.where(ph == 1, 1.0, np.where(ph == 3, -1.0, 0.0))
+        # (P's columns m' > l and R's columns |m| > l are zero in the table: so are these)
+        out[:, l] = np.where(par[None] == 0, Sc * even[None], Ss * odd[None])
+    return out
+
+
+class VTableHost:
+    """The correlation as one GEMM per (j, m'): V[j, m', c] = sum_l Q[j, m', c, l] conj(g_{l,m'}),
+    with Q[j, m', c, l] = sum_mu f_{l, fold mu} d^l_{m', fold mu}(beta_j) e^{-2 pi i mu c / nc}
+    (the transforms along b and c of the reference's spectrum folded into a pattern-independent
+    table). Laid out as the MFMA B operand: per (j, m', slot s) two rows (k = re, im) of
+    `cols` columns, [V_re | V_im | zero pad]:
+        row re: [Q_re, Q_im];   row im: [Q_im, -Q_re]
+    so that (g_re, g_im) . (row re, row im) gives (V_re, V_im). Slot s holds l = s l_step."""
+
+    def __init__(self, band, phase, coef):
+        L, n = band.L, band.n
+        fold, nc = phase.fold, phase.nc
+        self.L, self.n, self.nc, self.fold, self.l_step = L, n, nc, fold, phase.l_step
+        self.slots = (L + self.l_step - 1) // self.l_step
+        self.cols = 32 * ((2 * nc + 31) // 32)
+        nb = n // 2 + 1
+        betas = 2 * np.pi * np.arange(nb) / n
+        D = d_beta(L, fold, band.wigner, betas)                   # [j, l, m', mu]
+        mu_max = (L - 1) // fold
+        mus = np.arange(-mu_max, mu_max + 1)
+        F = np.zeros((L, len(mus)), np.complex128)
+        for i, mu in enumerate(mus):
+            for l in range(abs(fold * mu), L):
+                F[l, i] = sh.signed_coef(coef, L, l, fold * mu)
+        E = np.exp(-2j * np.pi * np.outer(mus, np.arange(nc)) / nc)   # [mu, c]
+        ls = np.arange(self.slots) * self.l_step
+        Q = np.einsum("su,jsmu,uc->jmcs", F[ls], D[:, ls], E, optimize=True)      # [j, m', c, s]
+        # zero where l < m'
+        Q *= (ls[None, None, None, :] >= np.arange(L)[None, :, None, None])
+        self.Q = Q
+        T = np.zeros((nb, L, self.slots, 2, self.cols), np.float32)
+        Qt = np.transpose(Q, (0, 1, 3, 2))                          # [j, m', s, c]
+        T[..., 0, :nc] = Qt.real
+        T[..., 0, nc:2 * nc] = Qt.imag
+        T[..., 1, :nc] = Qt.imag
+        T[..., 1, nc:2 * nc] = -Qt.real
+        self.table = T
