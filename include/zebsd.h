@@ -71,12 +71,15 @@ typedef struct zb_phase {
 /* A phase's correlation as a GEMM table at a bandwidth: the reference's transforms along b and c
  * of the spectrum (so3.wgsl spectrum, a_peak_m) folded into a pattern-independent table,
  *   Q[j][m'][c][l] = sum_mu f_{l, fold mu} d^l_{m', fold mu}(2 pi j / n) e^{-2 pi i mu c / nc},
- * so that V[j][m'][c] = sum_{l >= m'} Q[j][m'][c][l] conj(g_{l m'}). Stored as the matrix-core
- * B operand: q[((j L + m') slots + s) 2 + r][cols] f32 for l = s l_step, row r = 0:
- * [Re Q | Im Q | 0], r = 1: [Im Q | -Re Q | 0] (c innermost, cols = 32 ceil(2 nc / 32)).
- * Built once per phase and bandwidth (zebsdsolve_amd/tables.py VTableHost). No counterpart in
- * the reference's tables; it replaces W_f of spherical.PhaseGpu (src/spherical.zig:192-242) on
- * this path. */
+ * so that V[j][m'][c] = sum_{l >= m'} Q[j][m'][c][l] conj(g_{l m'}). As matrix-core operands,
+ * per (j, m') and degree slot s (l = s l_step) two rows of `cols` floats: row 0 holds
+ * (Re Q_c, Im Q_c) at columns (2c, 2c + 1), row 1 holds (Im Q_c, -Re Q_c), columns past 2 nc
+ * zero. With (cols, S, nblk) from zb_vtable_shape, q stores them tiled for 16-byte loads: per
+ * (j, m', column group g < cols / (32 nblk)), [S/4][nblk][64][4] floats, element
+ * [t][b][32 r + x][i] = row r, slot 4t + i, column 32 (g nblk + b) + x (zero for slots past the
+ * last). Built once per phase and bandwidth (zebsdsolve_amd/tables.py VTableHost). No
+ * counterpart in the reference's tables; it replaces W_f of spherical.PhaseGpu
+ * (src/spherical.zig:192-242) on this path. */
 typedef struct zb_vtable {
     int32_t L;
     int32_t n;      /* fft_size(L), at most 128 on this path */
@@ -84,8 +87,12 @@ typedef struct zb_vtable {
     int32_t l_step; /* 2 when the master has no odd l, else 1 */
     int32_t slots;  /* ceil(L / l_step) */
     int32_t cols;
-    const float* q; /* [n/2 + 1][L][slots][2][cols] */
+    const float* q; /* tiled, (n/2 + 1) L S 2 cols floats */
 } zb_vtable;
+
+/* The tiling of zb_vtable.q for nc points along c and `slots` degree slots: cols, S (slots per
+ * wave, a power of two >= slots, at least 8) and nblk (column blocks of 32 per wave). */
+int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* S, int32_t* nblk);
 
 /* Gauss-Newton refinement options (detector.RefineOptions, src/detector.zig:57-70). */
 typedef struct zb_refine_opts {
@@ -151,7 +158,7 @@ int64_t zb_vcorr_workspace(const zb_vtable* table, int32_t n_items, int32_t deno
 /* zb_so3_correlate on the matrix cores: V = the table times each item's coefficients (float32
  * MFMA), then the real transform along a, times the reciprocal denominator, argmax (ties: lowest
  * index (j nc + c) n + a) and the parabolic sub-cell peak, as zb_so3_correlate.
- * twiddle: [1024][2] (zb_band.twiddle); rden: [*][n/2+1][n][nc] f32 in natural order (a, then c
+ * twiddle: [1024][2] (zb_band.twiddle); rden: [*][n/2+1][nc][n] f32 in natural order (a
  * innermost), as zb_vcorr_denominators writes it; peaks: [n_items][8] out, as zb_so3_correlate.
  * Replaces Spherical.correlate (src/spherical.zig:401-432: so3.wgsl spectrum, a_peak_m,
  * peak_reduce, peak_finish_m). */
@@ -159,7 +166,7 @@ int32_t zb_vcorr_correlate(const zb_vtable* table, const float* twiddle, const f
                            int32_t rden_div, void* work, int64_t work_bytes, float* peaks, void* stream);
 
 /* zb_so3_denominators on the matrix cores, from the tables of the master (t_f) and of its
- * square (t_f2): rden [n_windows][n/2+1][n][nc] out, natural order.
+ * square (t_f2): rden [n_windows][n/2+1][nc][n] out, natural order (a innermost).
  * Replaces Spherical.denominators (src/spherical.zig:446-464: so3.wgsl a_volume, rden_build). */
 int32_t zb_vcorr_denominators(const zb_vtable* t_f, const zb_vtable* t_f2, const float* w_alm, const float* w_stats,
                               int32_t n_windows, void* work, int64_t work_bytes, float* rden, void* stream);

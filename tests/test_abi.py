@@ -49,6 +49,10 @@ def test_bad_arguments_fail_with_a_message_and_launch_nothing():
     ph = nat.Phase(3, 32, 21, 2, None, None)  # fold 3 is not a power of two
     band = nat.Band(64, 128, None, None, None, None)
     assert lib.zb_so3_workspace(ctypes.byref(band), ctypes.byref(ph), 4) == -1
+    vt = nat.VTable(96, 256, 64, 1, 96, 128, None)  # n = 256: past the register transform
+    assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "n <= 128" in nat.last_error()
+    rc = lib.zb_vcorr_correlate(ctypes.byref(vt), None, None, 4, None, 4, None, 0, None, None)
+    assert rc == -1 and "zb_vcorr_correlate" in nat.last_error()
     opts = nat.RefineOpts(5, 0.035, 1.0, 0.02, 1e4, 64, 1)  # 64 NMI bins: too many
     rc = lib.zb_refine(None, 501, ctypes.byref(nat.Frame()), None, None, None, 8, 4, ctypes.byref(opts), None, 0, None)
     assert rc == -1
@@ -61,5 +65,37 @@ def test_flop_model_reproduces_the_exact_term_counts():
     assert bench.spectrum_terms(48, 4, True) == 330912
     assert bench.spectrum_terms(64, 2, False) == 4280640
     assert bench.mask_count(60, 60) == 2828 and bench.mask_count(120, 120) == 11304
-    f = bench.flops_per_pattern(64, 4, True, 5, 2828)
-    assert abs(sum(f.values()) / 1e6 - 23.85) < 0.01
+    per, total = bench.flops_per_pattern(64, 4, True, 5, 2828)
+    # 23.85 MFLOP per pattern at L = 64, 5 iterations, 60 x 60 (spectrum 8.38, transforms 11.91,
+    # harmonic analysis 1.19, refinement and NMI 2.38)
+    assert abs(total / 1e6 - 23.85) < 0.01
+    # both correlation paths carry the same terms
+    assert per["k_vgemm"] + per["k_apeak"] == per["k_spectrum"] + per["k_slab0"]
+    assert abs((per["k_vgemm"] + per["k_apeak"]) / 1e6 - (8.38 + 11.91)) < 0.01
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
+def test_table_tiling_rule_is_the_librarys():
+    """The host builds the matrix-core table in the tiling the kernel launch chooses: the two
+    implementations of the rule agree, and the tiled layout holds each element where the header
+    says."""
+    import numpy as np
+    from zebsdsolve_amd import _native as nat, tables
+    lib = nat.load()
+    for nc in (2, 4, 8, 16, 32, 64):
+        for slots in (1, 5, 8, 9, 16, 17, 32, 33, 64):
+            c, S, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+            assert lib.zb_vtable_shape(nc, slots, ctypes.byref(c), ctypes.byref(S), ctypes.byref(b)) == 0
+            assert (c.value, S.value, b.value) == tables.vgemm_shape(nc, slots)
+    assert lib.zb_vtable_shape(48, 8, ctypes.byref(c), ctypes.byref(S), ctypes.byref(b)) == -1
+    rng = np.random.default_rng(0)
+    nb, L, slots, cols = 3, 4, 5, 64
+    T = rng.normal(size=(nb, L, slots, 2, cols)).astype(np.float32)
+    _, S, nblk = tables.vgemm_shape(32, slots)
+    t = tables.tile_vtable(T, S, nblk).reshape(nb, L, cols // (32 * nblk), S // 4, nblk, 64, 4)
+    for _ in range(200):
+        j, m, s, r, col = rng.integers(nb), rng.integers(L), rng.integers(S), rng.integers(2), rng.integers(cols)
+        g, rest = divmod(col, 32 * nblk)
+        bb, x = divmod(rest, 32)
+        want = T[j, m, s, r, col] if s < slots else 0.0
+        assert t[j, m, g, s // 4, bb, 32 * r + x, s % 4] == want

@@ -71,8 +71,7 @@ def test_vcorr_against_oracle_and_slab_path(name, path, L, N):
     so = op.So3(L, fold)
     wa, wst = op.coefficients(None, fr, t, L, center=False, norm=False, window=True)
     rden_ref = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])          # [j][c][a]
-    vr = e.vrden(0, L).cpu().numpy().reshape(so.n // 2 + 1, so.n, so.nc)     # [j][a][c]
-    rden = np.transpose(vr, (0, 2, 1))
+    rden = e.vrden(0, L).cpu().numpy().reshape(so.n // 2 + 1, so.nc, so.n)  # [j][c][a]
     big = rden_ref > 0
     # the same cells are defined; reciprocal denominators agree to float32 GEMM and transforms
     assert np.mean((rden > 0) == big) > 0.999
@@ -121,3 +120,41 @@ def test_vcorr_argument_checks():
     assert need > 0
     rc = lib.zb_vcorr_correlate(good, x.data_ptr(), x.data_ptr(), 4, x.data_ptr(), 4, x.data_ptr(), need - 1, x.data_ptr(), None)
     assert rc == -2 and "workspace" in nat.last_error()
+
+
+@pytest.mark.parametrize("L,N", [(32, 300), (64, 200)])
+def test_vcorr_matches_slab_path_over_many_tiles(L, N):
+    """Several item tiles and a ragged last one (N not a multiple of 64): random unit-norm
+    coefficients through both correlators."""
+    import torch
+    from zebsdsolve_amd import _native as nat, engine, sym
+    from zebsdsolve_amd.geometry import Geometry
+    from zebsdsolve_amd.engine import stream_ptr
+    print(f"vcorr tiles L={L} N={N}", flush=True)
+    e = engine.Engine()
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(NI)
+    e.settings.apply(bandwidth=L)
+    e.set_patterns(np.zeros((1, 1, 60, 60), np.float32))
+    lib = nat.load()
+    ph = e.phase_band(0, L)
+    band = e.band(L)
+    (vt, _), _ = ph.vtables()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    g = torch.randn((N, band.host.pairs, 2), device="cuda", generator=gen)
+    g /= g.norm(dim=(1, 2), keepdim=True)
+    p1 = torch.empty((N, 8), device="cuda")
+    p2 = torch.empty((N, 8), device="cuda")
+    ws = torch.empty((lib.zb_vcorr_workspace(vt, N, 0),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), g.data_ptr(), N, e.vrden(0, L).data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                     p1.data_ptr(), stream_ptr()), "vcorr")
+    ws2 = torch.empty((lib.zb_so3_workspace(band.c, ph.c, N),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_so3_correlate(band.c, ph.c, g.data_ptr(), N, e.rden(0, L).data_ptr(), N, ws2.data_ptr(), ws2.numel(), p2.data_ptr(),
+                                   stream_ptr()), "so3")
+    a, b = p1.cpu().numpy(), p2.cpu().numpy()
+    # the best score of every item to float32 rounding of the two summation orders
+    assert np.all(np.abs(a[:, 4] - b[:, 4]) <= 2e-5 * np.abs(b[:, 4])), np.abs(a[:, 4] - b[:, 4]).max()
+    # the same orientation, except where two cells score within rounding of each other
+    d = np.degrees(sym.disorientation(a[:, :4].astype(np.float64), b[:, :4].astype(np.float64), e.phases[0].master.rotations))
+    assert np.mean(d < 0.01) > 0.98, np.sort(d)[-5:]

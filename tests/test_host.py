@@ -142,6 +142,42 @@ def test_harmonics_and_weights_match_the_oracle(ni):
                 assert abs(W[mi, l, k] - ref) < 1e-9
 
 
+@pytest.mark.parametrize("path", [NI, GAN], ids=["Ni", "GaN"])
+def test_gemm_table_reproduces_the_oracle_volume(path):
+    """The matrix-core path's table, applied the way the MFMA applies it (row re times Re g plus
+    row im times Im g, over the degree slots l >= m'), then the transform along a, equals the
+    oracle's volume, whose b-axis rotation matrices come from the eigenvector route (not the
+    factorization at pi/2 the table uses)."""
+    m = master.load(path)
+    L = 16
+    band = tables.BandHost(L)
+    ph = tables.PhaseHost(band, m.side, m.raw)
+    vt = tables.VTableHost(band, ph, ph.hm.f)
+    so = op.So3(L, ph.fold)
+    rng = np.random.default_rng(4)
+    g = rng.normal(size=band.pairs) + 1j * rng.normal(size=band.pairs)
+    T = vt.table.astype(np.float64)                              # [j, m', s, 2, cols]
+    nc, n = vt.nc, vt.n
+    V = np.zeros((so.nb, L, nc), complex)
+    for mp in range(L):
+        for s_ in range(vt.slots):
+            l = s_ * vt.l_step
+            if l < mp:
+                continue
+            gv = g[sh.pair(L, mp, l)]
+            out = gv.real * T[:, mp, s_, 0] + gv.imag * T[:, mp, s_, 1]   # [j, cols]
+            V[:, mp] += out[:, 0:2 * nc:2] + 1j * out[:, 1:2 * nc:2]
+    Ea = np.exp(-2j * np.pi * np.outer(np.arange(L), np.arange(n)) / n)
+    k = np.where(np.arange(L) == 0, 1.0, 2.0)
+    V[:, 0] = V[:, 0].real
+    C = np.real(np.einsum("jpc,pa->jca", V * k[None, :, None], Ea))
+    ref = so.volume(ph.hm.f, g)
+    # float32 table entries: 1e-6 of the volume's scale
+    assert np.abs(C - ref).max() < 1e-6 * np.abs(ref).max()
+    # and in float64 (the table before rounding) the two routes agree to rounding
+    assert vt.cols >= 2 * nc and vt.slots == (L + vt.l_step - 1) // vt.l_step
+
+
 def test_bake_sigma_quantized_to_five_percent_steps():
     # 60 rows at pcz 0.55: a pixel is 1 / 33 rad; the 501-texel sheet spreads 2 pi sr over 501^2
     s = master.bake_sigma(501, 1.0 / 33.0, 0.3)

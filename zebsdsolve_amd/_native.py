@@ -46,6 +46,7 @@ _SIGS = {
     "zb_so3_workspace": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), i32], i64),
     "zb_so3_correlate": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_so3_denominators": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, vp, i32, vp, i64, vp, vp], i32),
+    "zb_vtable_shape": ([i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
     "zb_vcorr_workspace": ([ctypes.POINTER(VTable), i32, i32], i64),
     "zb_vcorr_correlate": ([ctypes.POINTER(VTable), vp, vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_vcorr_denominators": ([ctypes.POINTER(VTable), ctypes.POINTER(VTable), vp, vp, i32, vp, i64, vp, vp], i32),

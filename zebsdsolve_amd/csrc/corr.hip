@@ -11,49 +11,232 @@
 // left per pattern is a complex GEMM per (j, m') with the item's coefficients (K = its degrees l
 // >= m'), and the real transform along a.
 //
-//   k_vgemm   one wave per (m', tile of 32 MB items), looping over every j: the items'
-//             coefficients for m' are the MFMA A operand (rows = items, k = (re, im) of one
-//             degree), held in registers across all j; the table rows are the B operand
-//             ([Q_re, Q_im] and [Q_im, -Q_re] so one real MFMA per degree gives V_re | V_im).
-//             v_mfma_f32_32x32x2_f32 (float32 in, float32 accumulate). V goes to HBM as two
-//             planes, [item][j][m'][c], each write a run of columns.
-//   k_apeak   one lane per (item, j, c) column: the column's L orders into the half-length
+//   k_vgemm   one wave per (m', column group, tile of 32 MB items), looping over every j: the
+//             items' coefficients for m' are the MFMA B operand (columns = items, k = (re,
+//             im) of one degree), held in registers across all j; the table rows are the A
+//             operand (rows = V columns, (re, im) interleaved, so one real MFMA per degree
+//             gives V as complex pairs), loaded one j ahead. v_mfma_f32_32x32x2_f32 (float32
+//             in, float32 accumulate). V goes to HBM as complex pairs [j][m'][item][c]: a lane
+//             holds four adjacent columns of one item per accumulator group (one 16-byte
+//             store), from a scalar base plus a per-lane offset that does not change with j.
+//   k_apeak   one lane per (j, item, c) column: the column's L orders into the half-length
 //             complex input of the real transform along a, a radix-2 FFT of h = n/2 points in
-//             registers (twiddles are compile-time constants), times the reciprocal denominator,
-//             the best (value, lowest index) of the column, of the slab (xor shuffles over its
-//             nc lanes), of the item (one 64-bit atomicMax). Or the volume itself (denominators).
+//             registers (twiddles are compile-time constants, the trivial ones skipped), times
+//             the reciprocal denominator, the column's maximum, the slab's (xor shuffles over its
+//             nc lanes); only a slab that reaches the item's best so far looks up its lowest
+//             index of that value and offers (value, index) with one 64-bit atomicMax. Or the
+//             volume itself (denominators).
 //   k_vfinish the peak's six neighbours summed directly from V (already along c), parabolic
 //             offsets and the orientation, as so3.hip's k_finish.
 //
-// Volumes here (rden, and the window's) are in natural order, transposed: [j][a][c].
+// Volumes here (rden, and the window's) are in natural order [j][c][a]; V's items are padded to
+// a multiple of 64 and its rows to the table's column count, so that no store needs a bounds
+// check.
 #include "common.h"
 #include "tw128.h"
 
 namespace zb {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Raw buffer resources (gfx9 family: dword 3 = 0x00020000, 32-bit data format): a scalar base
+// and a 32-bit per-lane offset, so the per-lane address arithmetic of a long unrolled run of
+// loads or stores is one offset register, and the hardware bounds every access by `bytes`.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf(const void* base, long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL), 0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float2 buf_f32x2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return make_float2(__uint_as_float(u.x), __uint_as_float(u.y));
+}
+__device__ __forceinline__ void buf_st_f32(float x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, 0);
+}
+
+// items per V row are padded to a multiple of this (the largest tile)
+constexpr int kItemPad = 64;
 
 struct VArg {
-    int L, n, log2n, nc, log2nc, nb, slots, cols, l_step, pairs, tiles, n_items;
+    int L, n, log2n, nc, log2nc, nb, slots, cols, l_step, pairs, tiles, n_items, ni_pad;
 };
 
 __device__ __forceinline__ int xcd_block(int id, int G) {
     // blocks are dealt to the 8 XCDs round-robin: give each XCD a contiguous run of the logical
-    // order, so one XCD's L2 holds the table rows of a few m'
-    return (G % 8 == 0) ? (id % 8) * (G / 8) + id / 8 : id;
+    // order (XCD x takes G / 8 blocks, one more when x < G % 8), so one XCD's L2 holds the table
+    // rows of a few m'
+    const int q = G >> 3, r = G & 7, x = id & 7;
+    return x * q + min(x, r) + (id >> 3);
 }
 
+// One wave's GEMM for orders m' whose first live degree slot is at least S0: every slot from S0
+// to S - 1 is multiplied (the items' coefficients are zero below their first slot and past the
+// last one, and so is the table below l = m'), so no branch stands between a table load and its
+// MFMA. The table rows of j + 1 are loaded while j's products run.
+template <int MB, int NBLK, int S, int S0>
+__device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int cg, const float* __restrict__ tab, const float (&areg)[MB][S],
+                                          float* __restrict__ v) {
+    constexpr int NQ = (S - S0) / 4;                     // groups of four slots from S0
+    const int lane = threadIdx.x;
+    const int r32 = lane & 31;
+    const int half = lane >> 5;
+    const int ncg = a.cols / (NBLK * 32);
+    // the tiled table: per (j, m', cg) [S/4][NBLK][64 lanes][4 slots], 16 bytes per lane
+    const size_t blk = (size_t)S * NBLK * 64;
+    const float* tm = tab + ((size_t)mp * ncg + cg) * blk;
+    const size_t total = (size_t)a.nb * a.L * ncg * blk;
+    const auto trs = buf(tm, (long long)(total - ((size_t)mp * ncg + cg) * blk) * 4);
+    const unsigned jstride = (unsigned)((size_t)a.L * ncg * blk * 4);
+    const unsigned tl = (unsigned)lane * 16u;
+    // the table rows are the A operand (rows = V columns), the coefficients the B operand
+    // (columns = items): a lane's four consecutive accumulators are four adjacent V columns of
+    // one item, stored as one 16-byte write
+    floatx4 bc[NQ][NBLK], bn[NQ][NBLK];
+    auto load = [&](floatx4 (&b)[NQ][NBLK], int j) {
+        const unsigned sj = (unsigned)j * jstride;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb)
+                b[k][nb] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                           trs, (int)tl, (int)(sj + (unsigned)(((S0 / 4 + k) * NBLK + nb) * 1024)), 0));
+    };
+    load(bn, 0);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k)
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb) bc[k][nb] = bn[k][nb];
+    // V row of one (j, m'): ni_pad items x cols floats (2 nc, or 32 when nc < 16: the table's
+    // zero columns are stored too, so no store needs a predicate)
+    const int w = a.cols;
+    const int c0 = cg * NBLK * 32;
+    const size_t vrow = (size_t)a.ni_pad * w;
+    // per-lane byte offsets inside a row, the same for every j
+    unsigned loff[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + r32) * w + c0 + half * 4);
+    // one j: prefetch the next j's rows, the products, the copy of the prefetched rows, the
+    // stores. Two accumulator sets alternate between consecutive j, so the products of j + 1
+    // never wait for the stores of j to read their registers.
+    // KS partial sums per output tile, so that at least four independent MFMA chains are in
+    // flight (a chain's next product waits for the previous one)
+    constexpr int KS = (MB * NBLK >= 4) ? 1 : 4 / (MB * NBLK);
+    typedef floatx16 Acc[MB][NBLK][KS];
+    auto step = [&](int j, Acc& acc, Acc& other) {
+        // the last j reloads its own rows: no branch, so the waits count exactly
+        load(bn, min(j + 1, a.nb - 1));
+        // keep the loads ahead of the products (the scheduler would otherwise sink them to
+        // their use and expose their latency)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+                for (int p = 0; p < KS; ++p)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[mb][nb][p][r] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                    for (int nb = 0; nb < NBLK; ++nb) {
+                        floatx16& c = acc[mb][nb][(4 * k + i) % KS];
+                        c = __builtin_amdgcn_mfma_f32_32x32x2f32(bc[k][nb][i], areg[mb][S0 + 4 * k + i], c, 0, 0, 0);
+                    }
+        // the other set (stored one step ago) stays live across these products, so the two
+        // sets get their own registers
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+                for (int p = 0; p < KS; ++p) asm volatile("" ::"a"(other[mb][nb][p]));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb) bc[k][nb] = bn[k][nb];
+        // C[V column][item]: the lane holds item r32, V columns (r / 4) 8 + half 4 + r % 4
+        const auto vrs = buf(v + ((size_t)j * a.L + mp) * vrow, (long long)vrow * 4);
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    floatx4 q4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int p = 0; p < KS; ++p) {
+                        const floatx16& x = acc[mb][nb][p];
+                        q4 += floatx4{x[4 * g], x[4 * g + 1], x[4 * g + 2], x[4 * g + 3]};
+                    }
+                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, q4), vrs, (int)loff[mb], (int)((nb * 32 + g * 8) * 4), 0);
+                }
+    };
+    Acc acc0, acc1;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+            for (int p = 0; p < KS; ++p)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc1[mb][nb][p][r] = 0.f;
+    // an odd count repeats the last j (the same values to the same place)
+    for (int j = 0; j < a.nb; j += 2) {
+        step(j, acc0, acc1);
+        step(min(j + 1, a.nb - 1), acc1, acc0);
+    }
+}
+
+// The bucket of four slots the first live slot falls in picks the unrolled variant.
+template <int MB, int NBLK, int S, int S0 = 0>
+__device__ __forceinline__ void vgemm_bucket(int bucket, const VArg& a, int mp, int tile, int cg, const float* __restrict__ tab,
+                                             const float (&areg)[MB][S], float* __restrict__ v) {
+    if constexpr (S0 < S) {
+        if (bucket == S0 / 4) vgemm_run<MB, NBLK, S, S0>(a, mp, tile, cg, tab, areg, v);
+        else vgemm_bucket<MB, NBLK, S, S0 + 4>(bucket, a, mp, tile, cg, tab, areg, v);
+    }
+}
+
+// V: [j][m'][ni_pad][cols] floats, complex pairs (re, im) for c < nc.
 template <int MB, int NBLK, int S>
-__global__ void __launch_bounds__(64) k_vgemm(VArg a, const float* __restrict__ tab, const float2* __restrict__ coef,
-                                              float* __restrict__ vre, float* __restrict__ vim) {
-    const int logical = xcd_block(blockIdx.x, gridDim.x);
-    const int mp = logical / a.tiles;
-    const int tile = logical - mp * a.tiles;
+__global__ void __launch_bounds__(64) k_vgemm(VArg a, const float* __restrict__ tab, const float2* __restrict__ coef, float* __restrict__ v) {
+    // blocks are dealt to the 8 XCDs round-robin (block id mod 8). The work of an m' falls with
+    // m' (fewer live slots), so when L is a multiple of 8 each XCD takes the m' of a snake
+    // order (x, 15 - x, 16 + x, 31 - x, ...) and the XCDs get equal work; within an XCD the
+    // blocks run m' by m', tiles innermost, so its L2 holds one m's table rows at a time.
+    const int ncg = a.cols / (NBLK * 32);
+    const int per_m = a.tiles * ncg;
+    int mp, rem;
+    if ((a.L & 7) == 0) {
+        const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+        const int mi = k / per_m;
+        rem = k - mi * per_m;
+        mp = 8 * mi + ((mi & 1) ? 7 - x : x);
+    } else {
+        const int logical = xcd_block(blockIdx.x, gridDim.x);
+        mp = logical / per_m;
+        rem = logical - mp * per_m;
+    }
+    const int cg = rem / a.tiles;
+    const int tile = rem - cg * a.tiles;
     const int lane = threadIdx.x;
     const int half = lane >> 5;
     const int r32 = lane & 31;
     const int s0 = (mp + a.l_step - 1) / a.l_step;
     const int pbase = mp * a.L - (mp * (mp - 1)) / 2 - mp;  // pair(mp, l) = pbase + l
+    // A: rows = items, k = (re, im) of one degree slot; zero outside [s0, slots) and past the
+    // last item
     float areg[MB][S];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
@@ -61,55 +244,15 @@ __global__ void __launch_bounds__(64) k_vgemm(VArg a, const float* __restrict__ 
         const bool ok = item < a.n_items;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            float v = 0.f;
+            float x = 0.f;
             if (ok && s >= s0 && s < a.slots) {
                 const float2 g = coef[(size_t)item * a.pairs + pbase + s * a.l_step];
-                v = half ? g.y : g.x;
+                x = half ? g.y : g.x;
             }
-            areg[mb][s] = v;
+            areg[mb][s] = x;
         }
     }
-    const size_t per_item = (size_t)a.nb * a.L * a.nc;
-    for (int j = 0; j < a.nb; ++j) {
-        floatx16 acc[MB][NBLK];
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-            for (int nb = 0; nb < NBLK; ++nb)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) acc[mb][nb][r] = 0.f;
-        const float* tj = tab + ((size_t)j * a.L + mp) * a.slots * 2 * a.cols + half * a.cols + r32;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (s >= s0 && s < a.slots) {
-                float bv[NBLK];
-#pragma unroll
-                for (int nb = 0; nb < NBLK; ++nb) bv[nb] = tj[(size_t)s * 2 * a.cols + nb * 32];
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-                    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[mb][s], bv[nb], acc[mb][nb], 0, 0, 0);
-            }
-        }
-        // C[item row][col]: lane holds column r32 of rows (r/4) 8 + half 4 + r % 4
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-            for (int nb = 0; nb < NBLK; ++nb) {
-                const int col = nb * 32 + r32;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int item = (tile * MB + mb) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
-                    if (item < a.n_items) {
-                        const size_t at = (size_t)item * per_item + ((size_t)j * a.L + mp) * a.nc;
-                        if (col < a.nc)
-                            vre[at + col] = acc[mb][nb][r];
-                        else if (col < 2 * a.nc)
-                            vim[at + col - a.nc] = acc[mb][nb][r];
-                    }
-                }
-            }
-    }
+    vgemm_bucket<MB, NBLK, S>(min(s0, S - 1) >> 2, a, mp, tile, cg, tab, areg, v);
 }
 
 __device__ __forceinline__ unsigned ord_bits(float v) {
@@ -130,42 +273,60 @@ __device__ __forceinline__ int brev_c(int x) {
     return r;
 }
 
-// exp(-2 pi i t / N) from the 128-point constants (N divides 128)
-template <int N>
-__device__ __forceinline__ float2 twc(int t) {
-    const int k = (t * (128 / N)) & 127;
-    return make_float2(kTwCos[k], -kTwSin[k]);
+// v exp(-2 pi i k / 128) for a k known once the loops are unrolled: the multiplications by 1,
+// -i, -1, i and the odd eighth roots of unity reduce to moves and one product per part
+__device__ __forceinline__ float2 twmul(float2 v, int k) {
+    k &= 127;
+    constexpr float r = 0.70710678118654752f;
+    if (k == 0) return v;
+    if (k == 32) return make_float2(v.y, -v.x);
+    if (k == 64) return make_float2(-v.x, -v.y);
+    if (k == 96) return make_float2(-v.y, v.x);
+    if (k == 16) return make_float2((v.x + v.y) * r, (v.y - v.x) * r);
+    if (k == 48) return make_float2((v.y - v.x) * r, -(v.x + v.y) * r);
+    if (k == 80) return make_float2(-(v.x + v.y) * r, (v.x - v.y) * r);
+    if (k == 112) return make_float2((v.x - v.y) * r, (v.x + v.y) * r);
+    return cmul(v, make_float2(kTwCos[k], -kTwSin[k]));
+}
+
+// Z_q = (X_q + X_{q+h}) + i w_q (X_q - X_{q+h}) with X_{q+h} = conj X_{h-q}, given x = X_q,
+// yc = conj X_{h-q} and w_q = exp(-2 pi i k / 128)
+__device__ __forceinline__ float2 zterm(float2 x, float2 yc, int k) {
+    const float2 t = twmul(csub(x, yc), k);
+    return make_float2(x.x + yc.x - t.y, x.y + yc.y + t.x);
 }
 
 template <int H, int LH, int MODE>
-__global__ void __launch_bounds__(64) k_apeak(VArg a, const float* __restrict__ vre, const float* __restrict__ vim,
-                                              const float* __restrict__ rden, int rden_div, unsigned long long* __restrict__ best,
-                                              float* __restrict__ vol) {
+__global__ void __launch_bounds__(64) k_apeak(VArg a, const float2* __restrict__ v, const float2* __restrict__ rden, int rden_div,
+                                              unsigned long long* __restrict__ best, float2* __restrict__ vol) {
     const int lane = threadIdx.x;
     const int sl = lane >> a.log2nc;
     const int c = lane & (a.nc - 1);
-    const long slab = (long)blockIdx.x * (64 >> a.log2nc) + sl;
-    const bool live = slab < (long)a.n_items * a.nb;
-    const long sc = live ? slab : 0;
-    const int b = (int)(sc / a.nb);
-    const int j = (int)(sc - (long)b * a.nb);
-    const size_t base = ((size_t)b * a.nb + j) * a.L * a.nc + c;
-    float2 X[H];
-#pragma unroll
-    for (int q = 0; q < H; ++q) X[q] = (live && q < a.L) ? make_float2(vre[base + (size_t)q * a.nc], vim[base + (size_t)q * a.nc]) : make_float2(0.f, 0.f);
-    // the real transform along a (C = V_0 + 2 Re sum_{m' > 0} V_m' e^{-i m' a}) from a complex one
-    // of half its length: Z_q = (X_q + X_{q+h}) + i e^{-2 pi i q / n} (X_q - X_{q+h}),
-    // X_{q+h} = conj X_{h-q}, X_0 real
+    // slabs (j, item b) over the padded items, items innermost: a block's slabs share j (ni_pad
+    // is a multiple of the slabs per block) and read adjacent items' rows
+    const int per_block = 64 >> a.log2nc;
+    const long slab0 = (long)blockIdx.x * per_block;
+    const int j = (int)(slab0 / a.ni_pad);
+    const int b = (int)(slab0 - (long)j * a.ni_pad) + sl;
+    const bool live = b < a.n_items;
+    const int wc = a.cols / 2;                              // complex pairs per item row
+    const unsigned rs = (unsigned)(a.ni_pad * wc) * 8u;    // bytes per m' row
+    const auto vrs = buf(v + (size_t)j * a.L * a.ni_pad * wc, (long long)a.L * rs);
+    const unsigned vl = (unsigned)(b * wc + c) * 8u;
     float2 z[H];
-    z[0] = make_float2(X[0].x, X[0].x);
 #pragma unroll
-    for (int q = 1; q < H; ++q) {
-        const float2 x = X[q];
-        const float2 y = make_float2(X[H - q].x, -X[H - q].y);
-        const float2 w = twc<2 * H>(q);
-        const float2 d = csub(x, y);
-        z[q] = make_float2(x.x + y.x - (w.x * d.y + w.y * d.x), x.y + y.y + (w.x * d.x - w.y * d.y));
+    for (int q = 0; q < H; ++q) z[q] = q < a.L ? buf_f32x2(vrs, vl, (unsigned)q * rs) : make_float2(0.f, 0.f);
+    // the real transform along a (C = Re V_0 + 2 Re sum_{m' > 0} V_m' e^{-i m' a}) from a complex
+    // one of half its length, built in place pairwise (q, h - q); X_0 is taken real
+    constexpr int KS = 64 / H;  // w_q = exp(-2 pi i q / n) = exp(-2 pi i (q KS) / 128)
+    z[0] = make_float2(z[0].x, z[0].x);
+#pragma unroll
+    for (int q = 1; q < H / 2; ++q) {
+        const float2 x = z[q], y = z[H - q];
+        z[q] = zterm(x, make_float2(y.x, -y.y), q * KS);
+        z[H - q] = zterm(y, make_float2(x.x, -x.y), (H - q) * KS);
     }
+    z[H / 2] = zterm(z[H / 2], make_float2(z[H / 2].x, -z[H / 2].y), (H / 2) * KS);
     // radix-2 decimation in frequency: natural in, bit-reversed out
 #pragma unroll
     for (int hh = H / 2; hh >= 1; hh >>= 1) {
@@ -173,45 +334,49 @@ __global__ void __launch_bounds__(64) k_apeak(VArg a, const float* __restrict__ 
         for (int blk = 0; blk < H; blk += 2 * hh) {
 #pragma unroll
             for (int jj = 0; jj < hh; ++jj) {
-                const float2 u = z[blk + jj], v = z[blk + jj + hh];
-                z[blk + jj] = cadd(u, v);
-                z[blk + jj + hh] = cmul(csub(u, v), twc<128>(jj * (64 / hh)));
+                const float2 u = z[blk + jj], w = z[blk + jj + hh];
+                z[blk + jj] = cadd(u, w);
+                z[blk + jj + hh] = twmul(csub(u, w), jj * (64 / hh));
             }
         }
     }
-    const int n = a.n;
+    // z[p] = (C[2f], C[2f + 1]), f = bitrev(p)
     if (MODE == 1) {
         if (!live) return;
-        float* out = vol + ((size_t)b * a.nb + j) * n * a.nc + c;
+        float2* out = vol + (((size_t)b * a.nb + j) * a.nc + c) * H;
 #pragma unroll
-        for (int p = 0; p < H; ++p) {
-            const int f = brev_c<LH>(p);
-            out[(size_t)(2 * f) * a.nc] = z[p].x;
-            out[(size_t)(2 * f + 1) * a.nc] = z[p].y;
-        }
+        for (int p = 0; p < H; ++p) out[brev_c<LH>(p)] = z[p];
         return;
     }
-    const float* rd = rden + ((size_t)(b / rden_div) * a.nb + j) * n * a.nc + c;
-    float bv = -3.4e38f;
-    unsigned bi = 0xffffffffu;
-    const unsigned ibase = (unsigned)(j * a.nc + c) * (unsigned)n;
+    // padding lanes (b >= n_items) read item 0's volume and are masked below
+    const float2* rd = rden + (((size_t)((live ? b : 0) / rden_div) * a.nb + j) * a.nc + c) * H;
+    float m = -3.4e38f;
 #pragma unroll
     for (int p = 0; p < H; ++p) {
-        const int f = brev_c<LH>(p);
-        const float ve = z[p].x * rd[(size_t)(2 * f) * a.nc];
-        const float vo = z[p].y * rd[(size_t)(2 * f + 1) * a.nc];
-        const unsigned ie = ibase + 2u * f;
-        if (ve > bv || (ve == bv && ie < bi)) { bv = ve; bi = ie; }
-        if (vo > bv || (vo == bv && ie + 1 < bi)) { bv = vo; bi = ie + 1; }
+        const float2 r = rd[brev_c<LH>(p)];
+        z[p] = make_float2(z[p].x * r.x, z[p].y * r.y);
+        m = fmaxf(m, fmaxf(z[p].x, z[p].y));
     }
-    if (!live) { bv = -3.4e38f; bi = 0xffffffffu; }
-    for (int o = 1; o < a.nc; o <<= 1) {
-        const float ov = __shfl_xor(bv, o, 64);
-        const unsigned oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    if (!live) m = -3.4e38f;
+    float smax = m;
+    for (int o = 1; o < a.nc; o <<= 1) smax = fmaxf(smax, __shfl_xor(smax, o, 64));
+    // a slab below the item's best so far (a stale read is never above the true best) cannot
+    // hold the argmax; one that reaches it finds its lowest index of that value
+    const unsigned long long cur = live ? __hip_atomic_load(best + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+    const bool need = live && (cur == 0ull || smax >= from_ord((unsigned)(cur >> 32)));
+    unsigned idx = 0xffffffffu;
+    if (need && m == smax) {
+        const unsigned base = (unsigned)(j * a.nc + c) * (unsigned)a.n;
+#pragma unroll
+        for (int p = 0; p < H; ++p) {
+            const unsigned f2 = 2u * (unsigned)brev_c<LH>(p);
+            if (z[p].x == smax) idx = min(idx, base + f2);
+            if (z[p].y == smax) idx = min(idx, base + f2 + 1);
+        }
     }
-    if (live && c == 0) {
-        const unsigned long long key = ((unsigned long long)ord_bits(bv) << 32) | (unsigned long long)(0xffffffffu - bi);
+    for (int o = 1; o < a.nc; o <<= 1) idx = min(idx, (unsigned)__shfl_xor((int)idx, o, 64));
+    if (need && c == 0) {
+        const unsigned long long key = ((unsigned long long)ord_bits(smax) << 32) | (unsigned long long)(0xffffffffu - idx);
         atomicMax(best + b, key);
     }
 }
@@ -227,9 +392,8 @@ __device__ __forceinline__ float vertex_v(float lo, float mid, float hi) {
     return fminf(fmaxf(0.5f * (lo - hi) / den, -0.5f), 0.5f);
 }
 
-__global__ void __launch_bounds__(64) k_vfinish(VArg a, const float* __restrict__ vre, const float* __restrict__ vim, const float2* __restrict__ tw,
-                                                const float* __restrict__ rden, int rden_div, const unsigned long long* __restrict__ best,
-                                                float* __restrict__ peaks) {
+__global__ void __launch_bounds__(64) k_vfinish(VArg a, const float2* __restrict__ v, const float2* __restrict__ tw, const float* __restrict__ rden,
+                                                int rden_div, const unsigned long long* __restrict__ best, float* __restrict__ peaks) {
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     const unsigned long long key = best[b];
@@ -252,13 +416,11 @@ __global__ void __launch_bounds__(64) k_vfinish(VArg a, const float* __restrict_
     const int iam = (ia + n - 1) & (n - 1), iap = (ia + 1) & (n - 1);
     const int ts = ZB_TW / n;
     float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    auto V = [&](int jj, int mp, int c) {
-        const size_t at = (((size_t)b * a.nb + jj) * a.L + mp) * nc + c;
-        return make_float2(vre[at], vim[at]);
-    };
-    auto term = [&](float2 v, int mp, int x) {
-        const float2 e = tw[((mp * x) & (n - 1)) * ts];
-        return v.x * e.x - v.y * e.y;
+    const int wc = a.cols / 2;
+    auto V = [&](int jj, int mp, int cc) { return v[(((size_t)jj * a.L + mp) * a.ni_pad + b) * wc + cc]; };
+    auto term = [&](float2 x, int mp, int at) {
+        const float2 e = tw[((mp * at) & (n - 1)) * ts];
+        return x.x * e.x - x.y * e.y;
     };
     for (int mp = lane; mp < a.L; mp += 64) {
         const float f = mp == 0 ? 1.f : 2.f;
@@ -275,8 +437,8 @@ __global__ void __launch_bounds__(64) k_vfinish(VArg a, const float* __restrict_
 #pragma unroll
     for (int k = 0; k < 6; ++k) s[k] = warp_sum(s[k]);
     if (lane != 0) return;
-    const float* rd = rden + (size_t)(b / rden_div) * a.nb * n * nc;
-    auto R = [&](int jj, int c, int x) { return rd[((size_t)jj * n + x) * nc + c]; };
+    const float* rd = rden + (size_t)(b / rden_div) * a.nb * nc * n;
+    auto R = [&](int jj, int cc, int at) { return rd[((size_t)jj * nc + cc) * n + at]; };
     const float da = vertex_v(s[0] * R(j, iq, iam), v0, s[1] * R(j, iq, iap));
     const float dc = vertex_v(s[2] * R(j, iqm, ia), v0, s[3] * R(j, iqp, ia));
     float dj = 0.f;
@@ -285,6 +447,7 @@ __global__ void __launch_bounds__(64) k_vfinish(VArg a, const float* __restrict_
     const float aa = ((float)ia + da) * step;
     const float bb = ((float)j + dj) * step;
     const float cc = ((float)iq + dc) * step;
+    // S = Rz(a) Ry(b) Rz(c); the orientation is its inverse
     const float4 qa = make_float4(cosf(0.5f * aa), 0.f, 0.f, sinf(0.5f * aa));
     const float4 qb = make_float4(cosf(0.5f * bb), 0.f, sinf(0.5f * bb), 0.f);
     const float4 qc = make_float4(cosf(0.5f * cc), 0.f, 0.f, sinf(0.5f * cc));
@@ -295,14 +458,15 @@ __global__ void __launch_bounds__(64) k_vfinish(VArg a, const float* __restrict_
     out[4] = v0; out[5] = aa; out[6] = bb; out[7] = cc;
 }
 
+// 1 / sqrt(<w, f^2> - <w, f>^2 / area), 0 where that is <= 1e-6 area (so3.wgsl:614-626).
 __global__ void __launch_bounds__(256) k_vrden(const float* __restrict__ vf, const float* __restrict__ vf2, const float* __restrict__ stats,
                                                long per_item, long total, float* __restrict__ out) {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
     const float ar = fmaxf(stats[(i / per_item) * 4], 1e-12f);
     const float m = vf[i];
-    const float v = vf2[i] - m * m / ar;
-    out[i] = v > 1e-6f * ar ? 1.f / sqrtf(v) : 0.f;
+    const float x = vf2[i] - m * m / ar;
+    out[i] = x > 1e-6f * ar ? 1.f / sqrtf(x) : 0.f;
 }
 
 }  // namespace zb
@@ -338,58 +502,83 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
     a.l_step = t->l_step;
     a.pairs = t->L * (t->L + 1) / 2;
     a.n_items = n_items;
+    a.ni_pad = (n_items + kItemPad - 1) / kItemPad * kItemPad;
     a.tiles = 1;
     return ZB_OK;
 }
 
-static int64_t v_bytes(const VArg& a, int n_items) { return (int64_t)n_items * a.nb * a.L * a.nc * 4; }
 static int64_t al256(int64_t v) { return (v + 255) / 256 * 256; }
-// V (two planes) and the per-item best; the denominators add two volumes
-static int64_t vcorr_ws(const VArg& a, int n_items, int volumes) {
-    return 2 * al256(v_bytes(a, n_items)) + al256((int64_t)n_items * 8) + (volumes ? 2 * al256((int64_t)n_items * a.nb * a.n * a.nc * 4) : 0);
+// V for the padded items: cols floats per item row (complex pairs, then the table's zero columns)
+static int64_t v_bytes(const VArg& a) { return (int64_t)a.ni_pad * a.nb * a.L * a.cols * 4; }
+// V and the per-item best; the denominators add two volumes
+static int64_t vcorr_ws(const VArg& a, int volumes) {
+    return al256(v_bytes(a)) + al256((int64_t)a.n_items * 8) + (volumes ? 2 * al256((int64_t)a.n_items * a.nb * a.n * a.nc * 4) : 0);
 }
 
 extern "C" int64_t zb_vcorr_workspace(const zb_vtable* t, int32_t n_items, int32_t denominators) {
     VArg a;
-    if (v_arg(t, n_items, a, "zb_vcorr_workspace")) return ZB_EINVAL;
     if (n_items < 0) {
         set_error("zb_vcorr_workspace: n_items %d", n_items);
         return ZB_EINVAL;
     }
-    return vcorr_ws(a, n_items, denominators != 0);
+    if (v_arg(t, n_items, a, "zb_vcorr_workspace")) return ZB_EINVAL;
+    return vcorr_ws(a, denominators != 0);
 }
 
 template <int MB, int NBLK, int S>
-static void launch_vgemm_t(const VArg& a, const float* q, const float* alm, float* vre, float* vim, hipStream_t s) {
+static void launch_vgemm_t(const VArg& a, const float* q, const float* alm, float* v, hipStream_t s) {
     const int tiles = (a.n_items + 32 * MB - 1) / (32 * MB);
     VArg b = a;
     b.tiles = tiles;
-    ZB_LAUNCH("k_vgemm", (k_vgemm<MB, NBLK, S>), dim3((unsigned)(tiles * a.L)), dim3(64), 0, s, b, q, (const float2*)alm, vre, vim);
+    const int ncg = a.cols / (NBLK * 32);
+    ZB_LAUNCH("k_vgemm", (k_vgemm<MB, NBLK, S>), dim3((unsigned)(tiles * a.L * ncg)), dim3(64), 0, s, b, q, (const float2*)alm, v);
 }
 
-static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* vre, float* vim, hipStream_t s) {
-    const int nblk = a.cols / 32;
-    const int S = a.slots <= 8 ? 8 : a.slots <= 16 ? 16 : a.slots <= 32 ? 32 : 64;
-#define ZB_VG(MB, NB, SS) launch_vgemm_t<MB, NB, SS>(a, q, alm, vre, vim, s)
-    if (nblk == 1) {
-        if (S == 8) ZB_VG(2, 1, 8); else if (S == 16) ZB_VG(2, 1, 16); else if (S == 32) ZB_VG(2, 1, 32); else ZB_VG(1, 1, 64);
-    } else if (nblk == 2) {
-        if (S == 8) ZB_VG(2, 2, 8); else if (S == 16) ZB_VG(2, 2, 16); else if (S == 32) ZB_VG(2, 2, 32); else ZB_VG(1, 2, 64);
+// Register budget per lane (256 architectural VGPRs): coefficients MB S, table rows twice
+// (current and next j) 2 NBLK (S - S0); the accumulators MB NBLK 16 go to the accumulation
+// registers. The choice of (S, NBLK) fixes the table's tiling (zb_vtable_shape).
+static void vgemm_shape(int nc, int slots, int& cols, int& S, int& nblk) {
+    cols = 32 * ((2 * nc + 31) / 32);
+    S = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : 64;
+    nblk = (cols == 32 || S == 64) ? 1 : 2;
+}
+
+static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* v, hipStream_t s) {
+    int cols, S, nblk;
+    vgemm_shape(a.nc, a.slots, cols, S, nblk);
+#define ZB_VG(MB, NB, SS) launch_vgemm_t<MB, NB, SS>(a, q, alm, v, s)
+    if (S == 64) {
+        ZB_VG(1, 1, 64);
+    } else if (nblk == 1) {
+        if (S == 8) ZB_VG(2, 1, 8); else if (S == 16) ZB_VG(2, 1, 16); else ZB_VG(2, 1, 32);
     } else {
-        if (S == 8) ZB_VG(1, 4, 8); else if (S == 16) ZB_VG(1, 4, 16); else if (S == 32) ZB_VG(1, 4, 32); else ZB_VG(1, 4, 64);
+        if (S == 8) ZB_VG(2, 2, 8); else if (S == 16) ZB_VG(2, 2, 16); else ZB_VG(2, 2, 32);
     }
 #undef ZB_VG
     return check_launch("k_vgemm");
 }
 
+extern "C" int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* S, int32_t* nblk) {
+    if (nc < 2 || nc > 64 || (nc & (nc - 1)) || slots < 1 || slots > 64 || !cols || !S || !nblk) {
+        set_error("zb_vtable_shape: nc %d, slots %d", nc, slots);
+        return ZB_EINVAL;
+    }
+    int c, s, b;
+    vgemm_shape(nc, slots, c, s, b);
+    *cols = c;
+    *S = s;
+    *nblk = b;
+    return ZB_OK;
+}
+
 template <int MODE>
-static int launch_apeak(const VArg& a, const float* vre, const float* vim, const float* rden, int rden_div, unsigned long long* best, float* vol,
-                        hipStream_t s) {
+static int launch_apeak(const VArg& a, const float* v, const float* rden, int rden_div, unsigned long long* best, float* vol, hipStream_t s) {
     const int per_block = 64 >> a.log2nc;
-    const long slabs = (long)a.n_items * a.nb;
-    const dim3 grid((unsigned)((slabs + per_block - 1) / per_block));
+    const long slabs = (long)a.ni_pad * a.nb;  // ni_pad is a multiple of per_block
+    const dim3 grid((unsigned)(slabs / per_block));
     const int H = a.n / 2;
-#define ZB_AP(HH, LL) ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE>), grid, dim3(64), 0, s, a, vre, vim, rden, rden_div, best, vol)
+#define ZB_AP(HH, LL) \
+    ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE>), grid, dim3(64), 0, s, a, (const float2*)v, (const float2*)rden, rden_div, best, (float2*)vol)
     if (H == 8) ZB_AP(8, 3); else if (H == 16) ZB_AP(16, 4); else if (H == 32) ZB_AP(32, 5); else ZB_AP(64, 6);
 #undef ZB_AP
     return check_launch("k_apeak");
@@ -408,18 +597,17 @@ extern "C" int32_t zb_vcorr_correlate(const zb_vtable* t, const float* twiddle, 
         set_error("zb_vcorr_correlate: null pointer");
         return ZB_EINVAL;
     }
-    if (work_bytes < vcorr_ws(a, n_items, 0)) {
-        set_error("zb_vcorr_correlate: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)vcorr_ws(a, n_items, 0));
+    if (work_bytes < vcorr_ws(a, 0)) {
+        set_error("zb_vcorr_correlate: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)vcorr_ws(a, 0));
         return ZB_ESPACE;
     }
     hipStream_t s = (hipStream_t)stream;
-    float* vre = (float*)work;
-    float* vim = (float*)((char*)work + al256(v_bytes(a, n_items)));
-    unsigned long long* best = (unsigned long long*)((char*)vim + al256(v_bytes(a, n_items)));
-    if (int rc = launch_vgemm(a, t->q, alm, vre, vim, s)) return rc;
+    float* v = (float*)work;
+    unsigned long long* best = (unsigned long long*)((char*)work + al256(v_bytes(a)));
     ZB_LAUNCH("k_vclear", k_vclear, dim3((n_items + 255) / 256), dim3(256), 0, s, best, n_items);
-    if (int rc = launch_apeak<0>(a, vre, vim, rden, rden_div, best, nullptr, s)) return rc;
-    ZB_LAUNCH("k_vfinish", k_vfinish, dim3(n_items), dim3(64), 0, s, a, vre, vim, (const float2*)twiddle, rden, rden_div,
+    if (int rc = launch_vgemm(a, t->q, alm, v, s)) return rc;
+    if (int rc = launch_apeak<0>(a, v, rden, rden_div, best, nullptr, s)) return rc;
+    ZB_LAUNCH("k_vfinish", k_vfinish, dim3(n_items), dim3(64), 0, s, a, (const float2*)v, (const float2*)twiddle, rden, rden_div,
               (const unsigned long long*)best, peaks);
     return check_launch("zb_vcorr_correlate/finish");
 }
@@ -435,21 +623,20 @@ extern "C" int32_t zb_vcorr_denominators(const zb_vtable* t_f, const zb_vtable* 
         return ZB_EINVAL;
     }
     if (n_windows == 0) return ZB_OK;
-    if (work_bytes < vcorr_ws(a, n_windows, 1)) {
-        set_error("zb_vcorr_denominators: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)vcorr_ws(a, n_windows, 1));
+    if (work_bytes < vcorr_ws(a, 1)) {
+        set_error("zb_vcorr_denominators: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)vcorr_ws(a, 1));
         return ZB_ESPACE;
     }
     hipStream_t s = (hipStream_t)stream;
-    float* vre = (float*)work;
-    float* vim = (float*)((char*)work + al256(v_bytes(a, n_windows)));
-    char* after = (char*)vim + al256(v_bytes(a, n_windows)) + al256((int64_t)n_windows * 8);
+    float* v = (float*)work;
+    char* after = (char*)work + al256(v_bytes(a)) + al256((int64_t)n_windows * 8);
     const int64_t vb = al256((int64_t)n_windows * a.nb * a.n * a.nc * 4);
     float* vf = (float*)after;
     float* vf2 = (float*)(after + vb);
-    if (int rc = launch_vgemm(a, t_f->q, w_alm, vre, vim, s)) return rc;
-    if (int rc = launch_apeak<1>(a, vre, vim, nullptr, 1, nullptr, vf, s)) return rc;
-    if (int rc = launch_vgemm(a2, t_f2->q, w_alm, vre, vim, s)) return rc;
-    if (int rc = launch_apeak<1>(a, vre, vim, nullptr, 1, nullptr, vf2, s)) return rc;
+    if (int rc = launch_vgemm(a, t_f->q, w_alm, v, s)) return rc;
+    if (int rc = launch_apeak<1>(a, v, nullptr, 1, nullptr, vf, s)) return rc;
+    if (int rc = launch_vgemm(a2, t_f2->q, w_alm, v, s)) return rc;
+    if (int rc = launch_apeak<1>(a, v, nullptr, 1, nullptr, vf2, s)) return rc;
     const long per_item = (long)a.nb * a.n * a.nc;
     const long total = per_item * n_windows;
     ZB_LAUNCH("k_vrden", k_vrden, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, vf, vf2, w_stats, per_item, total, rden);

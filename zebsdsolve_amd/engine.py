@@ -76,7 +76,7 @@ class DevicePhase:
             out = []
             for coef in (self.host.hm.f, self.host.hm.f2):
                 t = tables_mod.VTableHost(self.band.host, self.host, coef)
-                q = _dev(t.table, self.dev)
+                q = _dev(t.tiled, self.dev)
                 out.append((nat.VTable(t.L, t.n, t.nc, t.l_step, t.slots, t.cols, q.data_ptr()), q))
             self._vt = out
         return self._vt

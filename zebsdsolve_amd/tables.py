@@ -114,14 +114,41 @@ def d_beta(L, fold, wigner, betas):
     return out
 
 
+def vgemm_shape(nc, slots):
+    """(cols, S, nblk) of the matrix-core correlation for nc points along c and `slots` degree
+    slots, by the rule its kernel launch follows (zb_vtable_shape in the C ABI): cols = 32
+    ceil(2 nc / 32) table columns, S = the power of two >= slots (at least 8) slots per wave,
+    nblk column blocks of 32 per wave (1 when cols = 32 or S = 64, else 2)."""
+    cols = 32 * ((2 * nc + 31) // 32)
+    S = 8
+    while S < slots:
+        S *= 2
+    nblk = 1 if (cols == 32 or S == 64) else 2
+    return cols, S, nblk
+
+
+def tile_vtable(T, S, nblk):
+    """The natural table [nb][L][slots][2][cols] in the layout the kernel loads 16 bytes per lane
+    from: per (j, m', column group cg of nblk x 32 columns), [S/4][nblk][64 lanes][4], lane =
+    32 r + x holding row r (re, im) of column cg nblk 32 + b 32 + x for the four slots
+    4 sq .. 4 sq + 3 (zero past the last slot)."""
+    nb, L, slots, _, cols = T.shape
+    ncg = cols // (32 * nblk)
+    Tp = np.zeros((nb, L, S, 2, cols), np.float32)
+    Tp[:, :, :slots] = T
+    Tp = Tp.reshape(nb, L, S // 4, 4, 2, ncg, nblk, 32)
+    return np.ascontiguousarray(np.transpose(Tp, (0, 1, 5, 2, 6, 4, 7, 3))).reshape(-1)
+
+
 class VTableHost:
     """The correlation as one GEMM per (j, m'): V[j, m', c] = sum_l Q[j, m', c, l] conj(g_{l,m'}),
     with Q[j, m', c, l] = sum_mu f_{l, fold mu} d^l_{m', fold mu}(beta_j) e^{-2 pi i mu c / nc}
     (the transforms along b and c of the reference's spectrum folded into a pattern-independent
     table). Laid out as the MFMA B operand: per (j, m', slot s) two rows (k = re, im) of
-    `cols` columns, [V_re | V_im | zero pad]:
-        row re: [Q_re, Q_im];   row im: [Q_im, -Q_re]
-    so that (g_re, g_im) . (row re, row im) gives (V_re, V_im). Slot s holds l = s l_step."""
+    `cols` columns, c-major with (re, im) interleaved, then zero pad:
+        row re: column 2c = Re Q_c, 2c + 1 = Im Q_c;   row im: 2c = Im Q_c, 2c + 1 = -Re Q_c
+    so that Re g (row re) + Im g (row im) gives V = Q conj(g) as (re, im) pairs. Slot s holds
+    l = s l_step."""
 
     def __init__(self, band, phase, coef):
         L, n = band.L, band.n
@@ -146,8 +173,10 @@ class VTableHost:
         self.Q = Q
         T = np.zeros((nb, L, self.slots, 2, self.cols), np.float32)
         Qt = np.transpose(Q, (0, 1, 3, 2))                          # [j, m', s, c]
-        T[..., 0, :nc] = Qt.real
-        T[..., 0, nc:2 * nc] = Qt.imag
-        T[..., 1, :nc] = Qt.imag
-        T[..., 1, nc:2 * nc] = -Qt.real
+        T[..., 0, 0:2 * nc:2] = Qt.real
+        T[..., 0, 1:2 * nc:2] = Qt.imag
+        T[..., 1, 0:2 * nc:2] = Qt.imag
+        T[..., 1, 1:2 * nc:2] = -Qt.real
         self.table = T
+        _, self.S, self.nblk = vgemm_shape(nc, self.slots)
+        self.tiled = tile_vtable(T, self.S, self.nblk)
