@@ -182,6 +182,21 @@ int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* frame, const
                   const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* opts, float* records,
                   int32_t out0, void* stream);
 
+/* Flag of zb_sums: zero the sums of the shift directions (orientation refinement's view). */
+#define ZB_SUMS_ROTATION_ONLY 2
+
+/* The 43 sums of the six-direction fit of each item's pose against its pattern: count, sum M,
+ * sum M^2, sum M e, then for d = (q^-1 grad M / |u|, c x grad M) (a shift of the rays' origin
+ * along the sample's x, y, z in pixels; a crystal rotation about x, y, z in radians) sum d_i,
+ * sum M d_i, sum e d_i (i < 6) and the upper triangle of sum d_i d_j, row by row (21).
+ * NCC, its gradient and Gauss-Newton matrix over the six directions follow from them
+ * (planefit.zig:177-201 localOf; zebsdsolve_amd/fit.py). poses: [n][8] = (t, pattern index, q);
+ * sums: [n][43] out.
+ * Replaces Detector.sumsOf (src/detector.zig:144-159: detector.wgsl sums, sums_reduce), with
+ * the bilinear master lookup. */
+int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
+                int32_t flags, float* sums, void* stream);
+
 /* Simulated patterns: the master (bilinear, [2][side][side] f32) in direction q . ray of every
  * pixel. poses: [n][8] = (t, pattern index, q); out: [n][h][w].
  * Replaces Detector.project (src/detector.zig:137-142: detector.wgsl project). */

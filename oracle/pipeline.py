@@ -400,6 +400,52 @@ def ncc_grad_hess(s):
     return ncc, grad, hess
 
 
+def sums6(grad_field, side, fr, pat, t, q):
+    """The 43 sums of the six-direction fit (detector.wgsl:433-494): count, sum M, sum M^2,
+    sum M e, then per direction i (a shift of the rays' origin along the sample's x, y, z, in
+    pixels; a crystal rotation about x, y, z) sum d_i, sum M d_i, sum e d_i, and the upper
+    triangle of sum d_i d_j row by row. d = (q^-1 grad M / |u|, c x grad M) for the pixel ray u,
+    c = q u / |u|."""
+    r, c, idx = masked_pixels(fr)
+    u = fr["c0"] * r[:, None] + fr["c1"] * c[:, None] + np.asarray(t)
+    dist = np.linalg.norm(u, axis=1, keepdims=True)
+    cr = qrot(q, u / dist)
+    m = sheet_at(grad_field, side, cr)
+    a = np.cross(cr, m[:, :3])
+    w = qrot(qconj(q), m[:, :3]) / dist
+    d = np.concatenate([w, a], 1)
+    e = np.asarray(pat, np.float64).reshape(-1)[idx]
+    M = m[:, 3]
+    out = [len(e), M.sum(), (M * M).sum(), (M * e).sum()]
+    out += list(d.sum(0)) + list((M[:, None] * d).sum(0)) + list((e[:, None] * d).sum(0))
+    dd = d.T @ d
+    out += [dd[i, j] for i in range(6) for j in range(i, 6)]
+    return np.asarray(out)
+
+
+def local6(s):
+    """NCC, its gradient and Gauss-Newton matrix over the six directions from the 43 sums
+    (planefit.zig:177-201 localOf). Returns (ok, ncc, g[6], H[6, 6])."""
+    s = np.asarray(s, np.float64)
+    n = max(s[0], 1.0)
+    mean = s[1] / n
+    var_n = s[2] - n * mean * mean
+    if not var_n > 1e-12:
+        return False, 0.0, np.zeros(6), np.zeros((6, 6))
+    inv = 1 / np.sqrt(var_n)
+    ncc = s[3] * inv
+    sd = s[4:10]
+    au = (s[10:16] - mean * sd) * inv
+    g = (s[16:22] - ncc * au) * inv
+    H = np.zeros((6, 6))
+    o = 22
+    for i in range(6):
+        for j in range(i, 6):
+            H[i, j] = H[j, i] = (s[o] - sd[i] * sd[j] / n - au[i] * au[j]) * inv * inv
+            o += 1
+    return True, ncc, g, H
+
+
 def refine(grad_field, side, fr, pat, t, q0, iters=8, max_step=np.radians(2.0), scale0=1.0, scale_min=0.02, scale_max=1e4):
     """Gauss-Newton on the orientation with the curvature rescaled from each gain against its
     promise, accept/reject against the best point (detector.wgsl:529-659). Returns

@@ -1,0 +1,103 @@
+"""The six-direction fit's sums (zb_sums) and the differentiable NCC built on them, against the
+float64 oracle (whose gradient its own test pins to central differences of the NCC).
+
+Tolerances: float32 sums over 2828 pixels against float64, stated at each assertion.
+"""
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.conftest import NI, gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+N = 8
+
+
+@pytest.fixture(scope="module")
+def fitcase():
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import engine, fit
+    from zebsdsolve_amd.geometry import Geometry
+    e = engine.Engine()
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(NI)
+    e.set_patterns(np.zeros((1, N, 60, 60), np.float32))
+    _, (_, grad_d, host) = e.baked(0)
+    side = e.phases[0].master.side
+    pats, truth = cases.simulate(host, side, N, seed=17, counts=400.0)
+    e.set_patterns(pats.reshape(1, N, 60, 60))
+    fr, t = cases.oracle_frame(60, 60)
+    field = fit.PoseField.for_cells(e, 0, np.arange(N))
+    rng = np.random.default_rng(5)
+    # poses near the truth: the detector position off by up to a pixel, the crystal by ~0.5 deg
+    tt = t[None, :] + rng.uniform(-1, 1, size=(N, 3))
+    qq = np.stack([op.qmul(op.qexp(rng.normal(size=3) * 0.005), truth[b]) for b in range(N)])
+    poses = np.concatenate([tt, np.arange(N)[:, None], qq], 1).astype(np.float32)
+    torch.cuda.synchronize()
+    return {"e": e, "op": op, "fit": fit, "field": field, "fr": fr, "poses": poses, "side": side,
+            "grad": grad_d.cpu().numpy().reshape(-1, 4), "pats_n": field.pats.cpu().numpy()}
+
+
+def test_sums_match_the_oracle(fitcase):
+    op, field = fitcase["op"], fitcase["field"]
+    print("zb_sums vs oracle", flush=True)
+    got = field.sums(fitcase["poses"]).cpu().numpy().astype(np.float64)
+    for b in range(N):
+        p = fitcase["poses"][b].astype(np.float64)
+        ref = op.sums6(fitcase["grad"], fitcase["side"], fitcase["fr"], fitcase["pats_n"][b].astype(np.float64), p[:3], p[4:8])
+        # each sum to float32 accumulation over 2828 terms: 1e-4 of the largest sum of its kind
+        scale = np.abs(ref).max()
+        assert got[b, 0] == ref[0]
+        assert np.abs(got[b] - ref).max() < 1e-4 * scale, (b, np.abs(got[b] - ref).max(), scale)
+        ok, n1, g1, H1 = op.local6(ref)
+        ok2, n2, g2, H2 = fitcase["fit"].local_of(got[b])
+        assert ok and ok2
+        # the NCC to 1e-5, its gradient to 1e-3 of its size
+        assert abs(n1 - n2) < 1e-5
+        assert np.abs(g1 - g2).max() < 1e-3 * np.abs(g1).max()
+
+
+def test_rotation_only_zeroes_the_shift_sums(fitcase):
+    field = fitcase["field"]
+    a = field.sums(fitcase["poses"]).cpu().numpy()
+    b = field.sums(fitcase["poses"], flags=fitcase["fit"].ROTATION_ONLY).cpu().numpy()
+    shift = np.array([(4 <= i < 7) or (10 <= i < 13) or (16 <= i < 19) or (22 <= i < 37) for i in range(43)])
+    assert np.all(b[:, shift] == 0) and np.array_equal(a[:, ~shift], b[:, ~shift])
+
+
+def test_differentiable_ncc_backward_is_the_oracle_gradient(fitcase):
+    import torch
+    op, fit = fitcase["op"], fitcase["fit"]
+    rng = np.random.default_rng(6)
+    delta0 = np.concatenate([rng.uniform(-0.3, 0.3, size=(N, 3)), rng.normal(size=(N, 3)) * 0.002], 1)
+    delta = torch.tensor(delta0, dtype=torch.float64, requires_grad=True)
+    ncc = fit.PoseNcc.apply(delta, fitcase["field"], fitcase["poses"])
+    ncc.sum().backward()
+    gd = delta.grad.numpy()
+    moved = fit.moved(fitcase["poses"], delta0)
+    for b in range(N):
+        pb = fitcase["pats_n"][b].astype(np.float64)
+
+        def f(dd):
+            p = fit.moved(fitcase["poses"][b:b + 1], dd[None])[0]
+            return op.local6(op.sums6(fitcase["grad"], fitcase["side"], fitcase["fr"], pb, p[:3], p[4:8]))[1]
+
+        assert abs(ncc[b].item() - f(delta0[b])) < 1e-5
+        # the float64 gradient at the moved pose, carried to delta through exp's left Jacobian
+        _, _, g, _ = op.local6(op.sums6(fitcase["grad"], fitcase["side"], fitcase["fr"], pb, moved[b, :3], moved[b, 4:8]))
+        J = fit.left_jacobian(delta0[b, 3:][None])[0]
+        want = np.concatenate([g[:3], J.T @ g[3:]])
+        assert np.abs(gd[b] - want).max() < 1e-3 * np.abs(want).max(), (b, gd[b], want)
+        # and central differences of the float64 NCC in delta (1e-5 of the pattern height in
+        # pixels, 1e-4 rad): the bilinear master is only piecewise smooth, so 10 %
+        fd = np.zeros(6)
+        for i in range(6):
+            h = 60 * 1e-5 if i < 3 else 1e-4
+            e = np.zeros(6)
+            e[i] = h
+            fd[i] = (f(delta0[b] + e) - f(delta0[b] - e)) / (2 * h)
+        assert np.abs(fd[:3] - gd[b, :3]).max() < 0.1 * np.abs(fd[:3]).max(), (b, fd, gd[b])
+        assert np.abs(fd[3:] - gd[b, 3:]).max() < 0.1 * np.abs(fd[3:]).max(), (b, fd, gd[b])
+    assert moved.shape == (N, 8)

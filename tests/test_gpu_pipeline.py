@@ -185,3 +185,58 @@ def test_end_to_end_and_refinement(case):
         assert abs(rec[b, 4] - ncc) < 1e-3, (b, rec[b, 4], ncc)
         nm = op.nmi(case["sheets"], case["side"], case["fr"], pats_n[b].astype(np.float64), case["t"], rec[b, :4].astype(np.float64))
         assert abs(rec[b, 5] - nm) < 2e-3, (b, rec[b, 5], nm)
+
+
+@pytest.mark.parametrize("h,w,mask_on", [(40, 56, False), (80, 80, True)], ids=["40x56-nomask", "80x80-mask"])
+def test_refinement_on_other_frames(case, h, w, mask_on):
+    """zb_refine on a non-square frame with the mask off, and on a frame whose masked pixels are
+    more than a lane keeps for the NMI (5024 > 48 x 64), against the oracle from the same seeds
+    (the truth turned by about 1 degree)."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import _native as nat, engine, sym
+    from zebsdsolve_amd.geometry import Geometry
+    print(f"refine frame {h}x{w} mask={mask_on}", flush=True)
+    n = 6
+    e = engine.Engine()
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(NI)
+    e.settings.apply(mask=mask_on)
+    e.set_patterns(np.zeros((1, n, h, w), np.float32))
+    _, (_, grad_d, host) = e.baked(0)
+    side = e.phases[0].master.side
+    R = op.bruker_R(70.0, 0.0)
+    fr = op.frame(R, h, w, mask_on=mask_on)
+    t = op.translation(R, cases.PC, float(h), float(w))
+    rng = np.random.default_rng(3)
+    truth = cases.random_quaternions(rng, n)
+    pats = np.stack([op.project(host, side, fr, t, q) for q in truth]).astype(np.float32)
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    half = np.radians(1.0) / 2
+    turn = np.concatenate([np.full((n, 1), np.cos(half)), axis * np.sin(half)], 1)
+    seeds = np.stack([op.qmul(turn[i], truth[i]) for i in range(n)])
+    lib = nat.load()
+    raw = torch.from_numpy(pats).cuda()
+    pn = torch.empty_like(raw)
+    mask = (nat.ctypes.c_float * 3)(*e.mask_tuple())
+    st = engine.stream_ptr()
+    nat.check(lib.zb_normalize(raw.data_ptr(), 2, n, h, w, mask, pn.data_ptr(), st), "normalize")
+    items = torch.from_numpy(np.concatenate([np.tile(t.astype(np.float32), (n, 1)), np.arange(n, dtype=np.float32)[:, None]], 1)).cuda()
+    sd = torch.from_numpy(seeds.astype(np.float32)).cuda()
+    opts = nat.RefineOpts(4, float(np.radians(2.0)), 1.0, 0.02, 1e4, 16, 9)
+    rec = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    nat.check(lib.zb_refine(grad_d.data_ptr(), side, e.frame(), pn.data_ptr(), items.data_ptr(), sd.data_ptr(), 4, n, opts, rec.data_ptr(), 0, st),
+              "refine")
+    r = rec.cpu().numpy()
+    assert np.all(r[:, 7] == 9.0)
+    pnh = pn.cpu().numpy()
+    g = grad_d.cpu().numpy().reshape(-1, 4)
+    rots = e.phases[0].master.rotations
+    for b in range(n):
+        q, ncc, _ = op.refine(g, side, fr, pnh[b].astype(np.float64), t, seeds[b], iters=4)
+        # float32 sums over the frame against float64: the same iterate to 0.02 deg, NCC to 1e-3
+        assert np.degrees(sym.disorientation(r[b, :4].astype(np.float64), q, rots)) < 0.02, b
+        assert abs(r[b, 4] - ncc) < 1e-3, (b, r[b, 4], ncc)
+        nm = op.nmi(host, side, fr, pnh[b].astype(np.float64), t, r[b, :4].astype(np.float64))
+        assert abs(r[b, 5] - nm) < 2e-3, (b, r[b, 5], nm)

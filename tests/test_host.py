@@ -265,3 +265,44 @@ def test_rotation_conversions_round_trip():
         assert np.allclose(rot.from_mat(rot.to_mat(qq)), qq, atol=1e-12)
     v = rng.normal(size=3)
     assert np.allclose(rot.to_mat(q[0]) @ v, rot.qapply(q[0], v))
+
+
+def test_six_direction_local_fit_matches_the_oracle():
+    """fit.local_of (batched, float64) against the oracle's localOf restatement, which its own
+    test pins to central differences of the NCC."""
+    from zebsdsolve_amd import fit
+    rng = np.random.default_rng(8)
+    # sums of a random set of pixels (M, e, d) so that the variances are positive
+    batch = []
+    for _ in range(5):
+        M = rng.normal(size=500)
+        e = rng.normal(size=500)
+        d = rng.normal(size=(500, 6))
+        s = [500, M.sum(), (M * M).sum(), (M * e).sum()] + list(d.sum(0)) + list((M[:, None] * d).sum(0)) + list((e[:, None] * d).sum(0))
+        dd = d.T @ d
+        s += [dd[i, j] for i in range(6) for j in range(i, 6)]
+        batch.append(s)
+    batch = np.asarray(batch)
+    ok, ncc, g, H = fit.local_of(batch)
+    for b in range(5):
+        ok1, n1, g1, H1 = op.local6(batch[b])
+        assert ok[b] and ok1
+        assert abs(ncc[b] - n1) < 1e-12 and np.abs(g[b] - g1).max() < 1e-12 and np.abs(H[b] - H1).max() < 1e-12
+    flat = batch[0].copy()
+    flat[2] = flat[1] ** 2 / flat[0]
+    assert not fit.local_of(flat[None])[0][0]
+
+
+def test_left_jacobian_of_exp():
+    from zebsdsolve_amd import fit
+    rng = np.random.default_rng(9)
+    for phi in [np.zeros(3), rng.normal(size=3) * 0.3, rng.normal(size=3) * 2.0]:
+        J = fit.left_jacobian(phi[None])[0]
+        h = 1e-6
+        for i in range(3):
+            dphi = np.zeros(3)
+            dphi[i] = h
+            q = rot.qmul(rot.so3_exp(phi + dphi), rot.qconj(rot.so3_exp(phi)))
+            q = q * np.sign(q[0])
+            # the rotation vector of exp(phi + h e_i) exp(phi)^-1 is J h e_i to first order
+            assert np.abs(2 * q[1:] / h - J[:, i]).max() < 1e-5

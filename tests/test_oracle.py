@@ -206,6 +206,36 @@ def test_refinement_gradient_and_matrix_from_sums_are_the_ncc_derivatives():
     assert np.all(np.linalg.eigvalsh(H) > -1e-9)
 
 
+def test_six_direction_gradient_from_sums_is_the_ncc_derivative():
+    # the fit's gradient over a shift of the rays' origin (h = 1e-5 of the pattern height, in
+    # pixels) and a crystal rotation (h = 1e-4 rad), from the 43 sums, against central
+    # differences of the NCC; its rotation block equals the orientation refinement's
+    sheets, side = _smooth_master(301)
+    grad_field = op.gradient(sheets, side)
+    h_px = 30
+    fr = op.frame(op.bruker_R(), h_px, h_px)
+    t0 = op.translation(op.bruker_R(), (0.5, 0.5, 0.55), float(h_px), float(h_px))
+    q0 = op.qexp(np.array([0.3, -0.2, 0.5]))
+    pat = op.normalize(op.project(grad_field[:, 3].reshape(2, side, side), side, fr, t0 + np.array([0.3, -0.2, 0.1]),
+                                  op.qexp(np.array([0.31, -0.19, 0.52]))), fr["mask"])
+
+    def ncc_at(t, q):
+        return op.local6(op.sums6(grad_field, side, fr, pat, t, q))[1]
+
+    ok, ncc, g, H = op.local6(op.sums6(grad_field, side, fr, pat, t0, q0))
+    assert ok and abs(ncc - ncc_at(t0, q0)) == 0
+    hs = 1e-5 * h_px
+    fd_shift = np.array([(ncc_at(t0 + hs * e, q0) - ncc_at(t0 - hs * e, q0)) / (2 * hs) for e in np.eye(3)])
+    hr = 1e-4
+    fd_rot = np.array([(ncc_at(t0, op.qmul(op.qexp(hr * e), q0)) - ncc_at(t0, op.qmul(op.qexp(-hr * e), q0))) / (2 * hr) for e in np.eye(3)])
+    assert np.abs(fd_shift - g[:3]).max() < 0.02 * np.abs(g[:3]).max()
+    assert np.abs(fd_rot - g[3:]).max() < 0.02 * np.abs(g[3:]).max()
+    assert np.all(np.linalg.eigvalsh(H) > -1e-9)
+    # the rotation block is the orientation refinement's
+    n3, g3, H3 = op.ncc_grad_hess(op.rot_sums(grad_field, side, fr, pat, t0, q0))
+    assert abs(n3 - ncc) < 1e-12 and np.abs(g3 - g[3:]).max() < 1e-12 and np.abs(H3 - H[3:, 3:]).max() < 1e-12
+
+
 def test_nmi_is_the_histogram_definition():
     rng = np.random.default_rng(4)
     sheets, side = _smooth_master(101)

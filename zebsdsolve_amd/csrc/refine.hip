@@ -36,38 +36,64 @@ __device__ __forceinline__ float wsum(float v) {
 
 __device__ __forceinline__ float3 wsum3(float3 v) { return make_float3(wsum(v.x), wsum(v.y), wsum(v.z)); }
 
-__device__ Sums rot_sums(const RefArg& a, const float4* __restrict__ mg, const float* __restrict__ pat, float4 q, float3 t, int lane) {
+// The masked pixels of the frame, as pixel indices in row-major order, built by each block into
+// LDS (the same list for every item): the passes then walk only the pixels under the mask, 64 at
+// a time, instead of the whole frame with a quarter of the lanes idle.
+__device__ int build_list(const RefArg& a, unsigned short* list, int lane) {
+    int total = 0;
+    for (int r0 = 0; r0 < a.h; r0 += 64) {
+        const int r = r0 + lane;
+        int cnt = 0;
+        if (r < a.h) {
+            const float dr = (float)r - a.mask[0];
+            for (int c = 0; c < a.w; ++c) {
+                const float dc = (float)c - a.mask[1];
+                cnt += (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2]) ? 1 : 0;
+            }
+        }
+        // exclusive prefix over the 64 rows of this group
+        int inc = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        int at = total + inc - cnt;
+        if (r < a.h) {
+            const float dr = (float)r - a.mask[0];
+            for (int c = 0; c < a.w; ++c) {
+                const float dc = (float)c - a.mask[1];
+                if (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2]) list[at++] = (unsigned short)(r * a.w + c);
+            }
+        }
+        total += __shfl(inc, 63, 64);
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ Sums rot_sums(const RefArg& a, const float4* __restrict__ mg, const float* __restrict__ pat, const unsigned short* list, int n_px,
+                         float inv_w, float4 q, float3 t, int lane) {
     Sums s;
     s.n = s.m = s.mm = s.me = 0.f;
     s.sd = s.sv = s.se = s.hd = s.ho = make_float3(0.f, 0.f, 0.f);
-    const int hw = a.h * a.w;
-    int row = lane / a.w, col = lane - (lane / a.w) * a.w;
-    const int step_r = 64 / a.w, step_c = 64 - (64 / a.w) * a.w;
-    for (int pix = lane; pix < hw; pix += 64) {
-        const float fr = (float)row, fc = (float)col;
-        const float dr = fr - a.mask[0], dc = fc - a.mask[1];
-        if (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2]) {
-            float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
-            const float3 c = qrot(q, normalize3(u));
-            const float4 m = sheet_bilinear<float4>(mg, a.side, c);
-            const float3 d = cross3(c, make_float3(m.x, m.y, m.z));
-            const float e = pat[pix];
-            s.n += 1.f;
-            s.m += m.w;
-            s.mm += m.w * m.w;
-            s.me += m.w * e;
-            s.sd = f3add(s.sd, d);
-            s.sv = f3add(s.sv, f3s(d, m.w));
-            s.se = f3add(s.se, f3s(d, e));
-            s.hd = f3add(s.hd, make_float3(d.x * d.x, d.y * d.y, d.z * d.z));
-            s.ho = f3add(s.ho, make_float3(d.x * d.y, d.x * d.z, d.y * d.z));
-        }
-        row += step_r;
-        col += step_c;
-        if (col >= a.w) {
-            col -= a.w;
-            row += 1;
-        }
+    for (int k = lane; k < n_px; k += 64) {
+        const int pix = list[k];
+        const int row = (int)(((float)pix + 0.5f) * inv_w);
+        const float fr = (float)row, fc = (float)(pix - row * a.w);
+        float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
+        const float3 c = qrot(q, normalize3(u));
+        const float4 m = sheet_bilinear<float4>(mg, a.side, c);
+        const float3 d = cross3(c, make_float3(m.x, m.y, m.z));
+        const float e = pat[pix];
+        s.n += 1.f;
+        s.m += m.w;
+        s.mm += m.w * m.w;
+        s.me += m.w * e;
+        s.sd = f3add(s.sd, d);
+        s.sv = f3add(s.sv, f3s(d, m.w));
+        s.se = f3add(s.se, f3s(d, e));
+        s.hd = f3add(s.hd, make_float3(d.x * d.x, d.y * d.y, d.z * d.z));
+        s.ho = f3add(s.ho, make_float3(d.x * d.y, d.x * d.z, d.y * d.z));
     }
     s.n = wsum(s.n);
     s.m = wsum(s.m);
@@ -95,12 +121,21 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
     return make_float3(det3(g, m1, m2) / det, det3(m0, g, m2) / det, det3(m0, m1, g) / det);
 }
 
+// MAXP > 0: each lane keeps the master values of its (at most MAXP) pixels at the best
+// orientation from the first NMI pass for the second; MAXP = 0: the second pass looks them up
+// again.
+template <int MAXP>
 __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restrict__ mg, const float* __restrict__ pats,
                                                const float4* __restrict__ items, const float* __restrict__ seeds, int n_items,
                                                float* __restrict__ records) {
-    __shared__ unsigned hist[32 * 32];
+    extern __shared__ unsigned lds[];
+    unsigned* hist = lds;                                          // [32][32]
+    unsigned short* list = (unsigned short*)(lds + 32 * 32);       // the masked pixels
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
+    for (int i = lane; i < 32 * 32; i += 64) hist[i] = 0u;
+    const int n_px = build_list(a, list, lane);
+    const float inv_w = 1.f / (float)a.w;
     const float4 it = items[b];
     const float3 t = make_float3(it.x, it.y, it.z);
     const float* pat = pats + (size_t)(int)it.w * a.h * a.w;
@@ -116,7 +151,7 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
     float last_step = 0.f, promised = 0.f, step_size = 0.f, accepted = 0.f;
 
     for (int itn = 0; itn <= a.iters; ++itn) {
-        const Sums s = rot_sums(a, mg, pat, cur, t, lane);
+        const Sums s = rot_sums(a, mg, pat, list, n_px, inv_w, cur, t, lane);
         // NCC, its gradient and Gauss-Newton matrix over the rotation (detector.wgsl:561-583)
         const float n = fmaxf(s.n, 1.f);
         const float mean = s.m / n;
@@ -175,20 +210,40 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
     // normalized mutual information of the pattern and the master at the best orientation,
     // each binned between its own extremes under the mask (detector.wgsl:739-832)
     const int nb = a.nmi_bins;
-    const int hw = a.h * a.w;
-    float emin = 3.4e38f, emax = -3.4e38f, smin = 3.4e38f, smax = -3.4e38f;
-    for (int i = lane; i < nb * nb; i += 64) hist[i] = 0u;
-    for (int pix = lane; pix < hw; pix += 64) {
-        const float fr = (float)(pix / a.w), fc = (float)(pix % a.w);
-        const float dr = fr - a.mask[0], dc = fc - a.mask[1];
-        if (!(a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2])) continue;
-        const float e = pat[pix];
+    auto sim_at = [&](int pix) {
+        const int row = (int)(((float)pix + 0.5f) * inv_w);
+        const float fr = (float)row, fc = (float)(pix - row * a.w);
         float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
-        const float sim = sheet_bilinear<float4>(mg, a.side, qrot(qbest, normalize3(u))).w;
-        emin = fminf(emin, e);
-        emax = fmaxf(emax, e);
-        smin = fminf(smin, sim);
-        smax = fmaxf(smax, sim);
+        return sheet_bilinear<float4>(mg, a.side, qrot(qbest, normalize3(u))).w;
+    };
+    float emin = 3.4e38f, emax = -3.4e38f, smin = 3.4e38f, smax = -3.4e38f;
+    float simv[MAXP > 0 ? MAXP : 1];
+    if constexpr (MAXP > 0) {
+#pragma unroll
+        for (int i = 0; i < MAXP; ++i) {
+            const int k = lane + 64 * i;
+            simv[i] = 0.f;
+            if (k < n_px) {
+                const int pix = list[k];
+                const float e = pat[pix];
+                const float sim = sim_at(pix);
+                simv[i] = sim;
+                emin = fminf(emin, e);
+                emax = fmaxf(emax, e);
+                smin = fminf(smin, sim);
+                smax = fmaxf(smax, sim);
+            }
+        }
+    } else {
+        for (int k = lane; k < n_px; k += 64) {
+            const int pix = list[k];
+            const float e = pat[pix];
+            const float sim = sim_at(pix);
+            emin = fminf(emin, e);
+            emax = fmaxf(emax, e);
+            smin = fminf(smin, sim);
+            smax = fmaxf(smax, sim);
+        }
     }
     for (int o = 1; o < 64; o <<= 1) {
         emin = fminf(emin, __shfl_xor(emin, o, 64));
@@ -196,19 +251,24 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
         smin = fminf(smin, __shfl_xor(smin, o, 64));
         smax = fmaxf(smax, __shfl_xor(smax, o, 64));
     }
-    __syncthreads();
     const float ke = (float)nb / fmaxf(emax - emin, 1e-20f);
     const float ks = (float)nb / fmaxf(smax - smin, 1e-20f);
-    for (int pix = lane; pix < hw; pix += 64) {
-        const float fr = (float)(pix / a.w), fc = (float)(pix % a.w);
-        const float dr = fr - a.mask[0], dc = fc - a.mask[1];
-        if (!(a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2])) continue;
-        const float e = pat[pix];
-        float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
-        const float sim = sheet_bilinear<float4>(mg, a.side, qrot(qbest, normalize3(u))).w;
+    auto bin = [&](float e, float sim) {
         const int be = min((int)((e - emin) * ke), nb - 1);
         const int bs = min((int)((sim - smin) * ks), nb - 1);
         atomicAdd(&hist[be * nb + bs], 1u);
+    };
+    if constexpr (MAXP > 0) {
+#pragma unroll
+        for (int i = 0; i < MAXP; ++i) {
+            const int k = lane + 64 * i;
+            if (k < n_px) bin(pat[list[k]], simv[i]);
+        }
+    } else {
+        for (int k = lane; k < n_px; k += 64) {
+            const int pix = list[k];
+            bin(pat[pix], sim_at(pix));
+        }
     }
     __syncthreads();
     float hj = 0.f, tot = 0.f, hm = 0.f;
@@ -246,9 +306,122 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
     }
 }
 
+// The 43 sums of the six-direction fit for each item's pose (detector.wgsl:433-494 sums +
+// sums_reduce): count, sum M, sum M^2, sum M e, then for d = (q^-1 grad M / |u|, c x grad M)
+// (a shift of the rays' origin in pixels, a crystal rotation) sum d_i, sum M d_i, sum e d_i and
+// the upper triangle of sum d_i d_j. One wave per item over the masked-pixel list; every lane
+// ends with the same totals (xor butterflies) and the lanes write them out in parallel.
+// rotation_only zeroes the shift's sums, as sums_reduce does for orientation refinement.
+constexpr int kSums = 43;
+
+__global__ void __launch_bounds__(64) k_sums6(RefArg a, const float4* __restrict__ mg, const float* __restrict__ pats, const float* __restrict__ poses,
+                                              int rotation_only, float* __restrict__ out) {
+    extern __shared__ unsigned lds[];
+    unsigned short* list = (unsigned short*)lds;
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n_px = build_list(a, list, lane);
+    const float inv_w = 1.f / (float)a.w;
+    const float* ps = poses + (size_t)b * 8;
+    const float3 t = make_float3(ps[0], ps[1], ps[2]);
+    const float* pat = pats + (size_t)(int)ps[3] * a.h * a.w;
+    const float4 q = make_float4(ps[4], ps[5], ps[6], ps[7]);
+    const float4 qi = make_float4(q.x, -q.y, -q.z, -q.w);
+    float s[kSums];
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) s[i] = 0.f;
+    for (int k = lane; k < n_px; k += 64) {
+        const int pix = list[k];
+        const int row = (int)(((float)pix + 0.5f) * inv_w);
+        const float fr = (float)row, fc = (float)(pix - row * a.w);
+        const float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
+        const float dist = sqrtf(dot3(u, u));
+        const float id = 1.f / dist;
+        const float3 c = qrot(q, make_float3(u.x * id, u.y * id, u.z * id));
+        const float4 m = sheet_bilinear<float4>(mg, a.side, c);
+        const float3 gm = make_float3(m.x, m.y, m.z);
+        const float3 ac = cross3(c, gm);
+        const float3 w = qrot(qi, gm);
+        float d[6] = {w.x * id, w.y * id, w.z * id, ac.x, ac.y, ac.z};
+        const float e = pat[pix];
+        const float val = m.w;
+        s[0] += 1.f;
+        s[1] += val;
+        s[2] += val * val;
+        s[3] += val * e;
+        int o = 22;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            s[4 + i] += d[i];
+            s[10 + i] += val * d[i];
+            s[16 + i] += e * d[i];
+#pragma unroll
+            for (int j = i; j < 6; ++j) s[o++] += d[i] * d[j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) s[i] = wsum(s[i]);
+    // lane i writes sum i (the shift's sums zero for orientation refinement)
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) {
+        if (lane == i) {
+            const bool shift = (i >= 4 && i < 7) || (i >= 10 && i < 13) || (i >= 16 && i < 19) || (i >= 22 && i < 37);
+            out[(size_t)b * kSums + i] = (rotation_only && shift) ? 0.f : s[i];
+        }
+    }
+}
+
 }  // namespace zb
 
 using namespace zb;
+
+static int ref_frame(const zb_frame* frame, int side, RefArg& a, size_t& lds, const char* who) {
+    for (int i = 0; i < 3; ++i) {
+        a.c0[i] = frame->c0[i];
+        a.c1[i] = frame->c1[i];
+        a.c2[i] = frame->c2[i];
+        a.mask[i] = frame->mask[i];
+    }
+    a.h = frame->pat_h;
+    a.w = frame->pat_w;
+    a.side = side;
+    if ((long)a.h * a.w > 65536) {
+        set_error("%s: %d x %d pixels: at most 65536", who, a.h, a.w);
+        return ZB_EINVAL;
+    }
+    int n_px = 0;
+    for (int r = 0; r < a.h; ++r)
+        for (int c = 0; c < a.w; ++c) {
+            const float dr = (float)r - a.mask[0], dc = (float)c - a.mask[1];
+            n_px += (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2]) ? 1 : 0;
+        }
+    lds = ((size_t)n_px * 2 + 15) / 16 * 16;
+    return n_px;
+}
+
+extern "C" int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
+                           int32_t flags, float* sums, void* stream) {
+    if (!frame || side < 2 || n_items < 0 || frame->pat_h < 2 || frame->pat_w < 2 || (flags & ~ZB_SUMS_ROTATION_ONLY)) {
+        set_error("zb_sums: bad arguments");
+        return ZB_EINVAL;
+    }
+    if (n_items == 0) return ZB_OK;
+    if (!mgrad || !pats || !poses || !sums) {
+        set_error("zb_sums: null pointer");
+        return ZB_EINVAL;
+    }
+    RefArg a = {};
+    size_t lds = 0;
+    const int n_px = ref_frame(frame, side, a, lds, "zb_sums");
+    if (n_px < 0) return n_px;
+    if (lds > 64 * 1024) {
+        set_error("zb_sums: %d masked pixels: at most %d", n_px, 32 * 1024);
+        return ZB_EINVAL;
+    }
+    ZB_LAUNCH("k_sums6", k_sums6, dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, poses,
+              (flags & ZB_SUMS_ROTATION_ONLY) ? 1 : 0, sums);
+    return check_launch("zb_sums");
+}
 
 extern "C" int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* items, const float* seeds,
                              int32_t seed_stride, int32_t n_items, const zb_refine_opts* o, float* records, int32_t out0, void* stream) {
@@ -262,16 +435,11 @@ extern "C" int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* f
         set_error("zb_refine: null pointer");
         return ZB_EINVAL;
     }
-    RefArg a;
-    for (int i = 0; i < 3; ++i) {
-        a.c0[i] = frame->c0[i];
-        a.c1[i] = frame->c1[i];
-        a.c2[i] = frame->c2[i];
-        a.mask[i] = frame->mask[i];
-    }
-    a.h = frame->pat_h;
-    a.w = frame->pat_w;
-    a.side = side;
+    RefArg a = {};
+    size_t list_bytes = 0;
+    // the masked pixels (the list each block builds) and the LDS they take
+    const int n_px = ref_frame(frame, side, a, list_bytes, "zb_refine");
+    if (n_px < 0) return n_px;
     a.iters = o->iters;
     a.nmi_bins = o->nmi_bins;
     a.seed_stride = seed_stride;
@@ -281,7 +449,16 @@ extern "C" int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* f
     a.scale_min = o->scale_min;
     a.scale_max = o->scale_max;
     a.stamp = (float)o->stamp;
-    ZB_LAUNCH("k_refine", k_refine, dim3(n_items), dim3(64), 0, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items, seeds,
-                       n_items, records);
+    const size_t lds = 32 * 32 * 4 + list_bytes;
+    if (lds > 64 * 1024) {
+        set_error("zb_refine: %d masked pixels: at most %d", n_px, (64 * 1024 - 4096) / 2);
+        return ZB_EINVAL;
+    }
+    if (n_px <= 64 * 48)
+        ZB_LAUNCH("k_refine", (k_refine<48>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items,
+                  seeds, n_items, records);
+    else
+        ZB_LAUNCH("k_refine", (k_refine<0>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items,
+                  seeds, n_items, records);
     return check_launch("zb_refine");
 }
