@@ -9,8 +9,8 @@
 //   k_rings    ring y with its mirror n-1-y as one complex line (real: the ring, imaginary: the
 //              mirror), mean-subtracted and windowed, FFT along phi in LDS, split apart; of the
 //              orders m < L the sum and difference of the two rings' spectra are kept
-//   k_legendre a_lm = sum_y wplm[y, pair] (sum or difference by the parity of l + m), a tile of
-//              items a block so the Legendre table is read once per tile
+//   k_legendre a_lm = sum_y wplm[y, pair] (sum or difference by the parity of l + m): per order
+//              m and tile of 16 items, two GEMMs on the matrix cores (one per parity)
 //   k_norm     unit harmonic norm and the item's stats
 // Workspace: per item the window sums (float2) and the ring spectra [L][n/2] float4 (m-major,
 // so that k_legendre reads them contiguously).
@@ -124,41 +124,55 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
     }
 }
 
-// a_lm for a tile of TB items at one order m: thread l - m (sphere.wgsl:209-226).
-template <int TB>
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// a_lm of 16 items at one order m on the matrix cores (sphere.wgsl:209-226): per parity of l + m
+// a GEMM with rows = (item, re / im) of the ring-pair spectra (N + S for even l + m, N - S for
+// odd), k = the ring y, columns = the degrees of that parity, against the Legendre table.
+// v_mfma_f32_32x32x2_f32: A lane -> (row lane % 32, k lane / 32), B lane -> (k lane / 32,
+// column lane % 32), C register r -> (row (r / 4) 8 + (lane / 32) 4 + r % 4, column lane % 32).
 __global__ void __launch_bounds__(64) k_legendre(SphArg a, int n_items, const float* __restrict__ wplm, const float4* __restrict__ spectra,
                                                  float2* __restrict__ alm) {
-    extern __shared__ float4 sp[];
+    const int lane = threadIdx.x;
+    const int half = lane >> 5;
+    const int r32 = lane & 31;
+    const int m = blockIdx.y;
+    const int b0 = blockIdx.x * 16;
     const int nh = a.n >> 1;
     const int L = a.L;
-    const int m = blockIdx.y;
-    const int b0 = blockIdx.x * TB;
-    for (int e = threadIdx.x; e < TB * nh; e += 64) {
-        const int t = e / nh;
-        const int y = e - t * nh;
-        const int b = b0 + t;
-        sp[e] = b < n_items ? spectra[((size_t)b * L + m) * nh + y] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    __syncthreads();
     const int base = m * L - (m * (m - 1)) / 2 - m;  // pair(m, l) = base + l
-    for (int l = m + threadIdx.x; l < L; l += 64) {
-        const int p = base + l;
-        const bool odd = ((l + m) & 1) != 0;
-        float2 acc[TB];
+    // this lane's A row: item r32 / 2, part r32 % 2
+    const int item = b0 + (r32 >> 1);
+    const bool im = (r32 & 1) != 0;
+    const bool iok = item < n_items;
+    const float4* sp = spectra + ((size_t)(iok ? item : 0) * L + m) * nh;
+    const int ncol = (L - m + 1) / 2;  // degrees l = m, m + 2, ...
+    for (int ct = 0; ct * 32 < ncol; ++ct) {
+        const int le = m + 2 * (ct * 32 + r32);  // this lane's column: even l + m
+        const int lo = le + 1;                   // and odd
+        floatx16 ce, co;
 #pragma unroll
-        for (int t = 0; t < TB; ++t) acc[t] = make_float2(0.f, 0.f);
-        for (int y = 0; y < nh; ++y) {
-            const float wv = wplm[(size_t)y * a.pairs + p];
+        for (int r = 0; r < 16; ++r) ce[r] = co[r] = 0.f;
+        for (int ks = 0; ks < nh / 2; ++ks) {
+            const int y = 2 * ks + half;
+            const float4 v = iok ? sp[y] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float* wy = wplm + (size_t)y * a.pairs + base;
+            const float be = le < L ? wy[le] : 0.f;
+            const float bo = lo < L ? wy[lo] : 0.f;
+            ce = __builtin_amdgcn_mfma_f32_32x32x2f32(im ? v.y : v.x, be, ce, 0, 0, 0);
+            co = __builtin_amdgcn_mfma_f32_32x32x2f32(im ? v.w : v.z, bo, co, 0, 0, 0);
+        }
+        // rows (r / 4) 8 + half 4 + r % 4: registers r, r + 1 (r even) are (re, im) of one item
 #pragma unroll
-            for (int t = 0; t < TB; ++t) {
-                const float4 s = sp[t * nh + y];
-                acc[t].x += wv * (odd ? s.z : s.x);
-                acc[t].y += wv * (odd ? s.w : s.y);
+        for (int r = 0; r < 16; r += 2) {
+            const int row = (r >> 2) * 8 + half * 4 + (r & 3);
+            const int it = b0 + (row >> 1);
+            if (it < n_items) {
+                float2* out = alm + (size_t)it * a.pairs + base;
+                if (le < L) out[le] = make_float2(ce[r], ce[r + 1]);
+                if (lo < L) out[lo] = make_float2(co[r], co[r + 1]);
             }
         }
-#pragma unroll
-        for (int t = 0; t < TB; ++t)
-            if (b0 + t < n_items) alm[(size_t)(b0 + t) * a.pairs + p] = acc[t];
     }
 }
 
@@ -258,9 +272,8 @@ extern "C" int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame
     ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,
                        (const float4*)items, (const float2*)sums, spectra);
     if (int rc = check_launch("zb_sh_coefficients/rings")) return rc;
-    constexpr int TB = 16;
-    ZB_LAUNCH("k_legendre", k_legendre<TB>, dim3((n_items + TB - 1) / TB, L), dim3(64), (size_t)TB * nh * 16, s, a, n_items, band->wplm,
-                       (const float4*)spectra, (float2*)alm);
+    ZB_LAUNCH("k_legendre", k_legendre, dim3((n_items + 15) / 16, L), dim3(64), 0, s, a, n_items, band->wplm, (const float4*)spectra,
+              (float2*)alm);
     if (int rc = check_launch("zb_sh_coefficients/legendre")) return rc;
     ZB_LAUNCH("k_norm", k_norm, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
     return check_launch("zb_sh_coefficients/norm");
