@@ -340,7 +340,8 @@ class Engine:
         else:
             rdens = [self.rden(pi, L) for pi in range(n_ph)]
             corr_ws = [lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k)) for ph in phs]
-        per_item = max(f(1) for f in corr_ws)
+        # bytes per item at scale (the matrix-core path pads items to a multiple of 64)
+        per_item = max(f(4096) for f in corr_ws) // 4096 + 1
         chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
         pairs = band.host.pairs
         alm = torch.empty((chunk, pairs, 2), dtype=torch.float32, device=dev)
