@@ -74,6 +74,21 @@ def flops_per_pattern(L, fold, inversion, iters, n_mask):
     return per, f_spec + f_bfft + f_cfft + f_afft + f_rings + f_leg + f_ref
 
 
+def bytes_per_pattern(L, fold, h, w):
+    """Algorithmic HBM bytes per pattern of the kernels whose data are per pattern: the spectrum
+    after the transforms along b and c (the reference's T is the same size: (n/2 + 1) L nc
+    complex64) written by k_vgemm and read by k_apeak, with the coefficients k_vgemm reads; the
+    pattern, item and record of k_refine."""
+    n = 16
+    while n < 2 * L - 1:
+        n *= 2
+    nc = n // fold
+    nb = n // 2 + 1
+    spectrum = nb * L * nc * 8
+    pairs = L * (L + 1) // 2
+    return {"k_vgemm": spectrum + pairs * 8, "k_apeak": spectrum, "k_refine": 4 * h * w + 16 + 32}
+
+
 def mask_count(h, w):
     r = 0.5 * min(h, w)
     rr, cc = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
@@ -268,15 +283,30 @@ def main():
     if prof:
         for k, (cnt, ms) in prof.items():
             kernels[k] = {"launches": cnt, "ms_total": round(ms, 3)}
-        dom = max(prof, key=lambda k: prof[k][1])
-        cnt, ms = prof[dom]
-        per_launch_ms = ms / cnt
-        patterns_per_launch = args.steps * n / cnt
-        if dom in fpp:
-            achieved = fpp[dom] * patterns_per_launch / (per_launch_ms * 1e-3) / 1e12
-            roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(dom, patterns_per_launch),
-                        "kernel": dom, "launch_ms": round(per_launch_ms, 4), "patterns_per_launch": round(patterns_per_launch, 1)}
+        bpp = bytes_per_pattern(args.bandwidth, fold, H, W)
+
+        def roof(k):
+            cnt, ms = prof[k]
+            per_launch_ms = ms / cnt
+            ppl = args.steps * n / cnt
+            sec = per_launch_ms * 1e-3
+            tf = fpp.get(k, 0.0) * ppl / sec / 1e12
+            gbs = bpp.get(k, 0.0) * ppl / sec / 1e9
+            # the roof that binds: the larger fraction of its peak (FP32 compute 157.3 TFLOP/s,
+            # matrix cores or VALU; HBM 8 TB/s)
+            if gbs / HBM_PEAK_GBS > tf / FP32_PEAK_TFLOPS:
+                r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            else:
+                r = {"bound": "mfma", "achieved": round(tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4)}
+            r.update({"traffic": traffic_from_profiles(k, ppl), "kernel": k, "launch_ms": round(per_launch_ms, 4),
+                      "patterns_per_launch": round(ppl, 1), "tflops": round(tf, 3), "gbs": round(gbs, 1)})
+            return r
+
+        dom = max((k for k in prof if k in fpp), key=lambda k: prof[k][1])
+        roofline = roof(dom)
+        for k in prof:
+            if k in fpp:
+                kernels[k]["roofline"] = {x: roof(k)[x] for x in ("bound", "frac", "tflops", "gbs")}
         total_alg = fpp_total * args.steps * n
         kernels["_all_algorithmic_TFLOPs"] = round(total_alg / el / 1e12, 3)
 

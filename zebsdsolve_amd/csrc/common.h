@@ -78,12 +78,11 @@ __device__ __forceinline__ float3 normalize3(float3 v) {
 __device__ __forceinline__ float2 sphere_to_square(float3 v) {
     float ax = fabsf(v.x), ay = fabsf(v.y);
     float xn = sqrtf(fmaxf(ZB_PI * 0.5f * (ax * ax + ay * ay) / (1.f + fabsf(v.z)), 0.f));
-    float u = xn, w = xn;
-    if (ax >= ay) {
-        w = (4.f * xn / ZB_PI) * atan2f(ay, fmaxf(ax, 1e-30f));
-    } else {
-        u = (4.f * xn / ZB_PI) * atan2f(ax, fmaxf(ay, 1e-30f));
-    }
+    // one arctangent of the smaller over the larger component (the same value either branch of
+    // the reference computes, without a divergent pair of calls)
+    const bool xbig = ax >= ay;
+    const float t = (4.f * xn / ZB_PI) * atan2f(xbig ? ay : ax, fmaxf(xbig ? ax : ay, 1e-30f));
+    const float u = xbig ? xn : t, w = xbig ? t : xn;
     const float sc = 0.79788456080286536f;  // 1 / sqrt(pi / 2)
     return make_float2(u * sc * (v.x < 0.f ? -1.f : 1.f), w * sc * (v.y < 0.f ? -1.f : 1.f));
 }
