@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--correlator", choices=("mfma", "slab"), default="mfma",
                     help="the SO(3) correlation as a GEMM on the matrix cores, or the spectrum + slab transforms")
+    ap.add_argument("--lanes", type=int, default=1, help="streams the chunks are spread over (1: one stream)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -248,10 +249,12 @@ def main():
         e.rden(0, args.bandwidth)
     torch.cuda.synchronize()
 
+    lanes = e.lanes(args.lanes) if args.lanes > 1 else [None]
+
     def step():
         recs = []
-        for (c0, c1), it in zip(chunks, local):
-            r, _ = e.run_batch(pats[c0:c1], it, args.bandwidth)
+        for i, ((c0, c1), it) in enumerate(zip(chunks, local)):
+            r, _ = e.run_batch(pats[c0:c1], it, args.bandwidth, lane=lanes[i % len(lanes)])
             recs.append(r)
         return recs
 
@@ -332,7 +335,7 @@ def main():
         "config": {"workload": f"{n} patterns {H}x{W} f32, Ni (Fm-3m, bundled ebsdsim master), L={args.bandwidth}, "
                                f"{args.iters} Gauss-Newton iterations, chunks U[1000, 8000]",
                    "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks),
-                   "correlator": "mfma" if e.use_mfma(args.bandwidth) else "slab"},
+                   "correlator": "mfma" if e.use_mfma(args.bandwidth) else "slab", "streams": max(args.lanes, 1)},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
                      "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1)},
         "roofline": roofline,

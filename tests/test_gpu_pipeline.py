@@ -240,3 +240,26 @@ def test_refinement_on_other_frames(case, h, w, mask_on):
         assert abs(r[b, 4] - ncc) < 1e-3, (b, r[b, 4], ncc)
         nm = op.nmi(host, side, fr, pnh[b].astype(np.float64), t, r[b, :4].astype(np.float64))
         assert abs(r[b, 5] - nm) < 2e-3, (b, r[b, 5], nm)
+
+
+def test_batches_on_two_streams_match_one_stream(case):
+    """Batches spread over two stream lanes (each with its own buffers) give the records one
+    stream gives: every item's result depends only on its own pattern."""
+    import torch
+    e = case["e"]
+    e.settings.apply(bandwidth=L, refine=True)
+    raw = torch.from_numpy(case["pats"]).cuda()
+    t = e.positions(np.arange(N))
+    items = torch.from_numpy(np.concatenate([t, np.arange(N, dtype=np.float32)[:, None]], 1)).cuda()
+    one, _ = e.run_batch(raw, items, L)
+    torch.cuda.synchronize()
+    lanes = e.lanes(2)
+    half = N // 2
+    it2 = items[half:].clone()
+    it2[:, 3] -= half
+    a, _ = e.run_batch(raw[:half], items[:half].contiguous(), L, lane=lanes[0])
+    b, _ = e.run_batch(raw[half:].contiguous(), it2, L, lane=lanes[1])
+    torch.cuda.synchronize()
+    got = torch.cat([a, b], 1).cpu().numpy()
+    ref = one.cpu().numpy()
+    assert np.array_equal(got[..., :7], ref[..., :7])

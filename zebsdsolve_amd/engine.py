@@ -88,6 +88,26 @@ def mfma_correlation(L, n):
     return n <= 128
 
 
+class Lane:
+    """A HIP stream (default priority) with scratch buffers of its own, so that consecutive
+    batches run concurrently: one batch's GEMM on the matrix cores beside another's refinement
+    on the vector units and gathers. Buffers only grow and are used only on this stream."""
+
+    def __init__(self, dev):
+        self.dev = dev
+        self.stream = torch.cuda.Stream(device=dev)
+        self.bufs = {}
+
+    def buf(self, name, shape, dtype):
+        numel = int(np.prod(shape))
+        b = self.bufs.get(name)
+        if b is None or b.numel() < numel or b.dtype != dtype:
+            with torch.cuda.stream(self.stream):
+                b = torch.empty((max(numel, 1),), dtype=dtype, device=self.dev)
+            self.bufs[name] = b
+        return b[:numel].view(shape)
+
+
 class Phase:
     def __init__(self, m):
         self.master = m
@@ -108,6 +128,7 @@ class Engine:
         # "mfma": the correlation as a GEMM on the matrix cores (bandwidths with n <= 128);
         # "slab": the spectrum + slab transforms (every bandwidth, and the fallback above 128)
         self.correlator = "mfma"
+        self._lanes = []
 
     # ── state ──
 
@@ -295,6 +316,13 @@ class Engine:
         ph.vrden[key] = rden
         return rden
 
+    def lanes(self, k=2):
+        """k streams to spread batches over (run_batch(..., lane=...))."""
+        dev = self._device()
+        while len(self._lanes) < k:
+            self._lanes.append(Lane(dev))
+        return self._lanes[:k]
+
     def use_mfma(self, L):
         return self.correlator == "mfma" and mfma_correlation(L, self.band(L).host.n)
 
@@ -311,10 +339,14 @@ class Engine:
         o.stamp = self.stamp
         return o
 
-    def run_batch(self, raw, items, L=None, workspace=None):
+    def run_batch(self, raw, items, L=None, workspace=None, lane=None):
         """The pipeline on device-resident inputs: raw [n, h, w] (uint8/uint16/float32 tensor) and
         items [n, 4] float32 (t, index into raw). Returns records [n_phases, n, 8] on the device
-        (and the stamp they carry). Nothing is read back."""
+        (and the stamp they carry). Nothing is read back.
+
+        With `lane` (from lanes()), the batch runs on that lane's stream with its buffers, after
+        what the current stream has enqueued so far, and the call returns without joining it:
+        the caller synchronizes before reading the records."""
         lib = nat.load()
         dev = self._device()
         s = self.settings
@@ -325,12 +357,9 @@ class Engine:
         band = self.band(L)
         fr = self.frame()
         mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
-        st = stream_ptr()
-        pats = torch.empty((raw.shape[0], h, w), dtype=torch.float32, device=dev)
-        nat.check(lib.zb_normalize(raw.data_ptr(), kind, raw.shape[0], h, w, mask, pats.data_ptr(), st), "normalize")
         n_ph = len(self.phases)
         opts = self.refine_opts()
-        records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
+        # per-phase tables (built on the current stream the first time)
         phs = [self.phase_band(pi, L) for pi in range(n_ph)]
         mfma = self.use_mfma(L)
         if mfma:
@@ -340,16 +369,34 @@ class Engine:
         else:
             rdens = [self.rden(pi, L) for pi in range(n_ph)]
             corr_ws = [lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k)) for ph in phs]
+        grads = [self.baked(pi)[1][1] for pi in range(n_ph)]
         # bytes per item at scale (the matrix-core path pads items to a multiple of 64)
         per_item = max(f(4096) for f in corr_ws) // 4096 + 1
         chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
         pairs = band.host.pairs
-        alm = torch.empty((chunk, pairs, 2), dtype=torch.float32, device=dev)
-        stats = torch.empty((chunk, 4), dtype=torch.float32, device=dev)
-        peaks = torch.empty((chunk, 8), dtype=torch.float32, device=dev)
         need = max(int(lib.zb_sh_workspace(band.c, chunk)), max(f(chunk) for f in corr_ws))
-        if workspace is None or workspace.numel() < need:
-            workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
+        if lane is None:
+            st = stream_ptr()
+            pats = torch.empty((raw.shape[0], h, w), dtype=torch.float32, device=dev)
+            records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
+            alm = torch.empty((chunk, pairs, 2), dtype=torch.float32, device=dev)
+            stats = torch.empty((chunk, 4), dtype=torch.float32, device=dev)
+            peaks = torch.empty((chunk, 8), dtype=torch.float32, device=dev)
+            if workspace is None or workspace.numel() < need:
+                workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
+        else:
+            lane.stream.wait_stream(torch.cuda.current_stream(dev))
+            st = lane.stream.cuda_stream
+            pats = lane.buf("pats", (raw.shape[0], h, w), torch.float32)
+            alm = lane.buf("alm", (chunk, pairs, 2), torch.float32)
+            stats = lane.buf("stats", (chunk, 4), torch.float32)
+            peaks = lane.buf("peaks", (chunk, 8), torch.float32)
+            workspace = lane.buf("work", (need,), torch.uint8)
+            with torch.cuda.stream(lane.stream):
+                records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
+            raw.record_stream(lane.stream)
+            items.record_stream(lane.stream)
+        nat.check(lib.zb_normalize(raw.data_ptr(), kind, raw.shape[0], h, w, mask, pats.data_ptr(), st), "normalize")
         for c0 in range(0, n, chunk):
             nt = min(chunk, n - c0)
             it = items[c0:c0 + nt]
@@ -362,10 +409,9 @@ class Engine:
                 else:
                     nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
                                                    workspace.numel(), peaks.data_ptr(), st), "correlate")
-                _, (_, grad, _) = self.baked(pi)
                 rec = records[pi]
-                nat.check(lib.zb_refine(grad.data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(), 8,
-                                        nt, opts, rec.data_ptr(), c0, st), "refine")
+                nat.check(lib.zb_refine(grads[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(),
+                                        8, nt, opts, rec.data_ptr(), c0, st), "refine")
         return records, opts.stamp
 
     def _raw_for(self, cells):
