@@ -122,10 +122,11 @@ def test_vcorr_argument_checks():
     assert rc == -2 and "workspace" in nat.last_error()
 
 
-@pytest.mark.parametrize("L,N", [(32, 300), (64, 200)])
+@pytest.mark.parametrize("L,N", [(32, 300), (64, 200), (64, 1500)])
 def test_vcorr_matches_slab_path_over_many_tiles(L, N):
-    """Several item tiles and a ragged last one (N not a multiple of 64): random unit-norm
-    coefficients through both correlators."""
+    """Several item tiles and a ragged last one (N not a multiple of 128), with many blocks in
+    flight at once: random unit-norm coefficients through both correlators, and the matrix-core
+    one run twice gives the same bits."""
     import torch
     from zebsdsolve_amd import _native as nat, engine, sym
     from zebsdsolve_amd.geometry import Geometry
@@ -153,6 +154,10 @@ def test_vcorr_matches_slab_path_over_many_tiles(L, N):
     nat.check(lib.zb_so3_correlate(band.c, ph.c, g.data_ptr(), N, e.rden(0, L).data_ptr(), N, ws2.data_ptr(), ws2.numel(), p2.data_ptr(),
                                    stream_ptr()), "so3")
     a, b = p1.cpu().numpy(), p2.cpu().numpy()
+    p3 = torch.empty_like(p1)
+    nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), g.data_ptr(), N, e.vrden(0, L).data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                     p3.data_ptr(), stream_ptr()), "vcorr again")
+    assert np.array_equal(a, p3.cpu().numpy())
     # the best score of every item to float32 rounding of the two summation orders
     assert np.all(np.abs(a[:, 4] - b[:, 4]) <= 2e-5 * np.abs(b[:, 4])), np.abs(a[:, 4] - b[:, 4]).max()
     # the same orientation, except where two cells score within rounding of each other

@@ -59,8 +59,9 @@ __device__ __forceinline__ void buf_st_f32(float x, __amdgpu_buffer_rsrc_t r, un
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, 0);
 }
 
-// items per V row are padded to a multiple of this (the largest tile)
-constexpr int kItemPad = 64;
+// items per V row are padded to a multiple of this (a k_vgemm block's items are 4 waves x 32),
+// plus a skew of 32
+constexpr int kItemPad = 128;
 
 struct VArg {
     int L, n, log2n, nc, log2nc, nb, slots, cols, l_step, pairs, tiles, n_items, ni_pad;
@@ -74,143 +75,148 @@ __device__ __forceinline__ int xcd_block(int id, int G) {
     return x * q + min(x, r) + (id >> 3);
 }
 
-// One wave's GEMM for orders m' whose first live degree slot is at least S0: every slot from S0
+// Waves per block: each computes its own tile of 64 items against the same table rows, which
+// the block stages once per j through LDS.
+constexpr int kWaves = 4;
+
+// One block's GEMM for orders m' whose first live degree slot is at least S0: every slot from S0
 // to S - 1 is multiplied (the items' coefficients are zero below their first slot and past the
-// last one, and so is the table below l = m'), so no branch stands between a table load and its
-// MFMA. The table rows of j + 1 are loaded while j's products run.
+// last one, and so is the table below l = m'), so no branch stands between the table and the
+// MFMA. The table rows of j + 1 go from HBM/L2 to registers while j's products run, and from
+// there to the other LDS buffer after them.
 template <int MB, int NBLK, int S, int S0>
 __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int cg, const float* __restrict__ tab, const float (&areg)[MB][S],
-                                          float* __restrict__ v) {
+                                          float* __restrict__ v, floatx4* lds0, floatx4* lds1) {
     constexpr int NQ = (S - S0) / 4;                     // groups of four slots from S0
-    const int lane = threadIdx.x;
+    constexpr int F4 = NQ * NBLK * 64;                   // 16-byte pieces of one j's rows
+    constexpr int PER = (F4 + 64 * kWaves - 1) / (64 * kWaves);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int r32 = lane & 31;
     const int half = lane >> 5;
     const int ncg = a.cols / (NBLK * 32);
-    // the tiled table: per (j, m', cg) [S/4][NBLK][64 lanes][4 slots], 16 bytes per lane
+    // the tiled table: per (j, m', cg) [S/4][NBLK][64 lanes][4 slots]; this block's rows start
+    // at slot group S0 / 4 and are F4 consecutive 16-byte pieces for each j
     const size_t blk = (size_t)S * NBLK * 64;
-    const float* tm = tab + ((size_t)mp * ncg + cg) * blk;
+    const size_t first = ((size_t)mp * ncg + cg) * blk + (size_t)(S0 / 4) * NBLK * 64 * 4;
     const size_t total = (size_t)a.nb * a.L * ncg * blk;
-    const auto trs = buf(tm, (long long)(total - ((size_t)mp * ncg + cg) * blk) * 4);
+    const auto trs = buf(tab + first, (long long)(total - first) * 4);
     const unsigned jstride = (unsigned)((size_t)a.L * ncg * blk * 4);
-    const unsigned tl = (unsigned)lane * 16u;
-    // the table rows are the A operand (rows = V columns), the coefficients the B operand
-    // (columns = items): a lane's four consecutive accumulators are four adjacent V columns of
-    // one item, stored as one 16-byte write
-    floatx4 bc[NQ][NBLK], bn[NQ][NBLK];
-    auto load = [&](floatx4 (&b)[NQ][NBLK], int j) {
-        const unsigned sj = (unsigned)j * jstride;
+    floatx4 pre[PER];
+    // every thread loads (a clamped piece when the rows are fewer than the threads) so that
+    // the waits count the same loads on every path
+    auto stage_load = [&](int j) {
 #pragma unroll
-        for (int k = 0; k < NQ; ++k)
-#pragma unroll
-            for (int nb = 0; nb < NBLK; ++nb)
-                b[k][nb] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(
-                                                           trs, (int)tl, (int)(sj + (unsigned)(((S0 / 4 + k) * NBLK + nb) * 1024)), 0));
+        for (int p = 0; p < PER; ++p) {
+            const int idx = min(tid + 64 * kWaves * p, F4 - 1);
+            pre[p] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(trs, idx * 16, (int)((unsigned)j * jstride), 0));
+        }
     };
-    load(bn, 0);
+    // pieces past the rows go to a spare slot after them (no predicate, so the loads above
+    // stay where they are issued)
+    auto stage_store = [&](floatx4* dst) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k)
-#pragma unroll
-        for (int nb = 0; nb < NBLK; ++nb) bc[k][nb] = bn[k][nb];
+        for (int p = 0; p < PER; ++p) {
+            const int idx = tid + 64 * kWaves * p;
+            dst[idx < F4 ? idx : F4] = pre[p];
+        }
+    };
     // V row of one (j, m'): ni_pad items x cols floats (2 nc, or 32 when nc < 16: the table's
     // zero columns are stored too, so no store needs a predicate)
     const int w = a.cols;
     const int c0 = cg * NBLK * 32;
     const size_t vrow = (size_t)a.ni_pad * w;
-    // per-lane byte offsets inside a row, the same for every j
     unsigned loff[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + r32) * w + c0 + half * 4);
-    // one j: prefetch the next j's rows, the products, the copy of the prefetched rows, the
-    // stores. Two accumulator sets alternate between consecutive j, so the products of j + 1
+    // one sum per output tile: the other waves on the SIMD fill the wait of a chain's next
+    // product for the previous one
+    typedef floatx16 Acc[MB][NBLK][1];
+    // one j: prefetch the next j's rows, the products (the rows are the A operand, rows = V
+    // columns; the coefficients the B operand, columns = items), the stores, the staging of the
+    // next rows. Two accumulator sets alternate between consecutive j, so the products of j + 1
     // never wait for the stores of j to read their registers.
-    // KS partial sums per output tile, so that at least four independent MFMA chains are in
-    // flight (a chain's next product waits for the previous one)
-    constexpr int KS = (MB * NBLK >= 4) ? 1 : 4 / (MB * NBLK);
-    typedef floatx16 Acc[MB][NBLK][KS];
-    auto step = [&](int j, Acc& acc, Acc& other) {
-        // the last j reloads its own rows: no branch, so the waits count exactly
-        load(bn, min(j + 1, a.nb - 1));
-        // keep the loads ahead of the products (the scheduler would otherwise sink them to
-        // their use and expose their latency)
+    auto step = [&](int j, Acc& acc, Acc& other, const floatx4* cur, floatx4* nxt) {
+        // the last j stages its own rows again: no branch
+        stage_load(min(j + 1, a.nb - 1));
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
             for (int nb = 0; nb < NBLK; ++nb)
 #pragma unroll
-                for (int p = 0; p < KS; ++p)
+                for (int r = 0; r < 16; ++r) acc[mb][nb][0][r] = 0.f;
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) acc[mb][nb][p][r] = 0.f;
+        for (int k = 0; k < NQ; ++k) {
+            floatx4 fa[NBLK];
 #pragma unroll
-        for (int k = 0; k < NQ; ++k)
+            for (int nb = 0; nb < NBLK; ++nb) fa[nb] = cur[(k * NBLK + nb) * 64 + lane];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
                     for (int nb = 0; nb < NBLK; ++nb) {
-                        floatx16& c = acc[mb][nb][(4 * k + i) % KS];
-                        c = __builtin_amdgcn_mfma_f32_32x32x2f32(bc[k][nb][i], areg[mb][S0 + 4 * k + i], c, 0, 0, 0);
+                        floatx16& c = acc[mb][nb][0];
+                        c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[nb][i], areg[mb][S0 + 4 * k + i], c, 0, 0, 0);
                     }
+        }
         // the other set (stored one step ago) stays live across these products, so the two
         // sets get their own registers
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
             for (int nb = 0; nb < NBLK; ++nb)
-#pragma unroll
-                for (int p = 0; p < KS; ++p) asm volatile("" ::"a"(other[mb][nb][p]));
+                asm volatile("" ::"a"(other[mb][nb][0]));
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int k = 0; k < NQ; ++k)
-#pragma unroll
-            for (int nb = 0; nb < NBLK; ++nb) bc[k][nb] = bn[k][nb];
         // C[V column][item]: the lane holds item r32, V columns (r / 4) 8 + half 4 + r % 4
         const auto vrs = buf(v + ((size_t)j * a.L + mp) * vrow, (long long)vrow * 4);
+        // the stores read the accumulators themselves: their registers are written again only
+        // by the products two steps on
 #pragma unroll
         for (int nb = 0; nb < NBLK; ++nb)
 #pragma unroll
             for (int g = 0; g < 4; ++g)
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
-                    floatx4 q4 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int p = 0; p < KS; ++p) {
-                        const floatx16& x = acc[mb][nb][p];
-                        q4 += floatx4{x[4 * g], x[4 * g + 1], x[4 * g + 2], x[4 * g + 3]};
-                    }
+                    const floatx16& x = acc[mb][nb][0];
+                    const floatx4 q4 = {x[4 * g], x[4 * g + 1], x[4 * g + 2], x[4 * g + 3]};
                     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, q4), vrs, (int)loff[mb], (int)((nb * 32 + g * 8) * 4), 0);
                 }
+        stage_store(nxt);
+        __syncthreads();
     };
+    stage_load(0);
+    stage_store(lds0);
+    __syncthreads();
     Acc acc0, acc1;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int nb = 0; nb < NBLK; ++nb)
 #pragma unroll
-            for (int p = 0; p < KS; ++p)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) acc1[mb][nb][p][r] = 0.f;
+            for (int r = 0; r < 16; ++r) acc1[mb][nb][0][r] = 0.f;
     // an odd count repeats the last j (the same values to the same place)
     for (int j = 0; j < a.nb; j += 2) {
-        step(j, acc0, acc1);
-        step(min(j + 1, a.nb - 1), acc1, acc0);
+        step(j, acc0, acc1, lds0, lds1);
+        step(min(j + 1, a.nb - 1), acc1, acc0, lds1, lds0);
     }
 }
 
 // The bucket of four slots the first live slot falls in picks the unrolled variant.
 template <int MB, int NBLK, int S, int S0 = 0>
 __device__ __forceinline__ void vgemm_bucket(int bucket, const VArg& a, int mp, int tile, int cg, const float* __restrict__ tab,
-                                             const float (&areg)[MB][S], float* __restrict__ v) {
+                                             const float (&areg)[MB][S], float* __restrict__ v, floatx4* lds0, floatx4* lds1) {
     if constexpr (S0 < S) {
-        if (bucket == S0 / 4) vgemm_run<MB, NBLK, S, S0>(a, mp, tile, cg, tab, areg, v);
-        else vgemm_bucket<MB, NBLK, S, S0 + 4>(bucket, a, mp, tile, cg, tab, areg, v);
+        if (bucket == S0 / 4) vgemm_run<MB, NBLK, S, S0>(a, mp, tile, cg, tab, areg, v, lds0, lds1);
+        else vgemm_bucket<MB, NBLK, S, S0 + 4>(bucket, a, mp, tile, cg, tab, areg, v, lds0, lds1);
     }
 }
 
 // V: [j][m'][ni_pad][cols] floats, complex pairs (re, im) for c < nc.
 template <int MB, int NBLK, int S>
-__global__ void __launch_bounds__(64) k_vgemm(VArg a, const float* __restrict__ tab, const float2* __restrict__ coef, float* __restrict__ v) {
+__global__ void __launch_bounds__(64 * kWaves) k_vgemm(VArg a, const float* __restrict__ tab, const float2* __restrict__ coef, float* __restrict__ v) {
+    extern __shared__ floatx4 vlds[];
     // blocks are dealt to the 8 XCDs round-robin (block id mod 8). The work of an m' falls with
     // m' (fewer live slots), so when L is a multiple of 8 each XCD takes the m' of a snake
     // order (x, 15 - x, 16 + x, 31 - x, ...) and the XCDs get equal work; within an XCD the
@@ -229,14 +235,14 @@ __global__ void __launch_bounds__(64) k_vgemm(VArg a, const float* __restrict__ 
         rem = logical - mp * per_m;
     }
     const int cg = rem / a.tiles;
-    const int tile = rem - cg * a.tiles;
-    const int lane = threadIdx.x;
+    const int tile = (rem - cg * a.tiles) * kWaves + (threadIdx.x >> 6);  // this wave's tile of 32 MB items
+    const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int r32 = lane & 31;
     const int s0 = (mp + a.l_step - 1) / a.l_step;
     const int pbase = mp * a.L - (mp * (mp - 1)) / 2 - mp;  // pair(mp, l) = pbase + l
-    // A: rows = items, k = (re, im) of one degree slot; zero outside [s0, slots) and past the
-    // last item
+    // B: rows k = (re, im) of one degree slot, columns = items; zero outside [s0, slots) and
+    // past the last item
     float areg[MB][S];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
@@ -252,7 +258,8 @@ __global__ void __launch_bounds__(64) k_vgemm(VArg a, const float* __restrict__ 
             areg[mb][s] = x;
         }
     }
-    vgemm_bucket<MB, NBLK, S>(min(s0, S - 1) >> 2, a, mp, tile, cg, tab, areg, v);
+    constexpr int F4MAX = (S / 4) * NBLK * 64 + 1;  // one j's rows and a spare slot
+    vgemm_bucket<MB, NBLK, S>(min(s0, S - 1) >> 2, a, mp, tile, cg, tab, areg, v, vlds, vlds + F4MAX);
 }
 
 __device__ __forceinline__ unsigned ord_bits(float v) {
@@ -308,6 +315,8 @@ __global__ void __launch_bounds__(64) k_apeak(VArg a, const float2* __restrict__
     const long slab0 = (long)blockIdx.x * per_block;
     const int j = (int)(slab0 / a.ni_pad);
     const int b = (int)(slab0 - (long)j * a.ni_pad) + sl;
+    // a block wholly in the padding has nothing to do
+    if ((int)(slab0 - (long)j * a.ni_pad) >= a.n_items) return;
     const bool live = b < a.n_items;
     const int wc = a.cols / 2;                              // complex pairs per item row
     const unsigned rs = (unsigned)(a.ni_pad * wc) * 8u;    // bytes per m' row
@@ -502,7 +511,9 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
     a.l_step = t->l_step;
     a.pairs = t->L * (t->L + 1) / 2;
     a.n_items = n_items;
-    a.ni_pad = (n_items + kItemPad - 1) / kItemPad * kItemPad;
+    // the items of a row, padded to whole blocks, and 32 more so that consecutive rows (one per
+    // (j, m')) do not sit a power of two apart in HBM
+    a.ni_pad = (n_items + kItemPad - 1) / kItemPad * kItemPad + 32;
     a.tiles = 1;
     return ZB_OK;
 }
@@ -527,16 +538,16 @@ extern "C" int64_t zb_vcorr_workspace(const zb_vtable* t, int32_t n_items, int32
 
 template <int MB, int NBLK, int S>
 static void launch_vgemm_t(const VArg& a, const float* q, const float* alm, float* v, hipStream_t s) {
-    const int tiles = (a.n_items + 32 * MB - 1) / (32 * MB);
+    // a block takes kWaves tiles of 32 MB items (ni_pad is a multiple of that)
+    const int tiles = (a.n_items + kWaves * 32 * MB - 1) / (kWaves * 32 * MB);
     VArg b = a;
     b.tiles = tiles;
     const int ncg = a.cols / (NBLK * 32);
-    ZB_LAUNCH("k_vgemm", (k_vgemm<MB, NBLK, S>), dim3((unsigned)(tiles * a.L * ncg)), dim3(64), 0, s, b, q, (const float2*)alm, v);
+    const size_t lds = 2 * ((size_t)(S / 4) * NBLK * 64 + 1) * 16;  // two buffers of one j's rows (and a spare slot)
+    ZB_LAUNCH("k_vgemm", (k_vgemm<MB, NBLK, S>), dim3((unsigned)(tiles * a.L * ncg)), dim3(64 * kWaves), lds, s, b, q, (const float2*)alm, v);
 }
 
-// Register budget per lane (256 architectural VGPRs): coefficients MB S, table rows twice
-// (current and next j) 2 NBLK (S - S0); the accumulators MB NBLK 16 go to the accumulation
-// registers. The choice of (S, NBLK) fixes the table's tiling (zb_vtable_shape).
+// The choice of (S, NBLK) fixes the table's tiling (zb_vtable_shape).
 static void vgemm_shape(int nc, int slots, int& cols, int& S, int& nblk) {
     cols = 32 * ((2 * nc + 31) / 32);
     S = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : 64;
@@ -546,13 +557,15 @@ static void vgemm_shape(int nc, int slots, int& cols, int& S, int& nblk) {
 static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* v, hipStream_t s) {
     int cols, S, nblk;
     vgemm_shape(a.nc, a.slots, cols, S, nblk);
-#define ZB_VG(MB, NB, SS) launch_vgemm_t<MB, NB, SS>(a, q, alm, v, s)
+    // one tile of 32 items per wave: the coefficients (S registers), two sets of accumulators
+    // (2 NBLK 16) and the staged table rows leave room for two or three waves per SIMD
+#define ZB_VG(NB, SS) launch_vgemm_t<1, NB, SS>(a, q, alm, v, s)
     if (S == 64) {
-        ZB_VG(1, 1, 64);
+        ZB_VG(1, 64);
     } else if (nblk == 1) {
-        if (S == 8) ZB_VG(2, 1, 8); else if (S == 16) ZB_VG(2, 1, 16); else ZB_VG(2, 1, 32);
+        if (S == 8) ZB_VG(1, 8); else if (S == 16) ZB_VG(1, 16); else ZB_VG(1, 32);
     } else {
-        if (S == 8) ZB_VG(2, 2, 8); else if (S == 16) ZB_VG(2, 2, 16); else ZB_VG(2, 2, 32);
+        if (S == 8) ZB_VG(2, 8); else if (S == 16) ZB_VG(2, 16); else ZB_VG(2, 32);
     }
 #undef ZB_VG
     return check_launch("k_vgemm");
