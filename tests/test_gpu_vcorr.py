@@ -12,7 +12,8 @@ from tests.conftest import GAN, NI, gpu_available
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
 
-CASES = [("Ni", NI, 32, 10), ("GaN", GAN, 32, 10), ("Ni", NI, 64, 4)]
+# the bench's bandwidth, and the others of the two-phase workload at n = 128 (L = 48, 64)
+CASES = [("Ni", NI, 32, 10), ("GaN", GAN, 32, 10), ("Ni", NI, 64, 4), ("Ni", NI, 48, 4), ("GaN", GAN, 48, 4), ("GaN", GAN, 64, 3)]
 
 
 def _setup(path, L, N):
