@@ -68,8 +68,9 @@ __device__ __forceinline__ float4 qexp(float3 p) {
     return make_float4(cosf(0.5f * th), k * p.x, k * p.y, k * p.z);
 }
 
+// (the hardware reciprocal square root, 1 ulp)
 __device__ __forceinline__ float3 normalize3(float3 v) {
-    float r = 1.f / sqrtf(dot3(v, v));
+    float r = __builtin_amdgcn_rsqf(dot3(v, v));
     return make_float3(v.x * r, v.y * r, v.z * r);
 }
 
@@ -77,7 +78,7 @@ __device__ __forceinline__ float3 normalize3(float3 v) {
 // taken as (x^2 + y^2) / (1 + |z|), which keeps its digits near the pole (common.wgsl:44-57).
 __device__ __forceinline__ float2 sphere_to_square(float3 v) {
     float ax = fabsf(v.x), ay = fabsf(v.y);
-    float xn = sqrtf(fmaxf(ZB_PI * 0.5f * (ax * ax + ay * ay) / (1.f + fabsf(v.z)), 0.f));
+    float xn = sqrtf(fmaxf(ZB_PI * 0.5f * (ax * ax + ay * ay) * __builtin_amdgcn_rcpf(1.f + fabsf(v.z)), 0.f));
     // one arctangent of the smaller over the larger component (the same value either branch of
     // the reference computes, without a divergent pair of calls)
     const bool xbig = ax >= ay;

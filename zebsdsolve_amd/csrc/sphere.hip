@@ -35,7 +35,8 @@ __device__ __forceinline__ float2 sample_point(const SphArg& a, const float* __r
     const float nd = a.c2[0] * d.x + a.c2[1] * d.y + a.c2[2] * d.z;
     float val = 0.f, w = 0.f;
     if (nd > 1e-4f) {
-        const float s = (a.c2[0] * it.x + a.c2[1] * it.y + a.c2[2] * it.z) / nd;
+        // (the hardware reciprocal, 1 ulp: the ray parameter of a sample, not a decision)
+        const float s = (a.c2[0] * it.x + a.c2[1] * it.y + a.c2[2] * it.z) * __builtin_amdgcn_rcpf(nd);
         const float hx = s * d.x - it.x, hy = s * d.y - it.y, hz = s * d.z - it.z;
         const float row = a.c0[0] * hx + a.c0[1] * hy + a.c0[2] * hz;
         const float col = a.c1[0] * hx + a.c1[1] * hy + a.c1[2] * hz;
