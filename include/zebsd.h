@@ -173,14 +173,16 @@ int32_t zb_vcorr_denominators(const zb_vtable* t_f, const zb_vtable* t_f2, const
 
 /* Gauss-Newton refinement of each item's orientation on the detector, then its NMI, into one
  * record per item: records[(out0 + b) * 8 ..] = best q (w, x, y, z), NCC, NMI, last improving
- * step (radians), stamp. mgrad: [2][side][side][4] f32 (master gradient on the sphere, master
- * value in .w); items: [n][4] (t, pattern index); seeds: orientation of item b at
+ * step (radians), stamp. master: [2][side][side] f32, the (smoothed) master whose gradient
+ * field zb_master_gradient made; mgrad_h: that field in half precision (zb_master_pack), which
+ * the Gauss-Newton passes read; the recorded NCC and NMI are those of the float32 master at the
+ * best orientation. items: [n][4] (t, pattern index); seeds: orientation of item b at
  * seeds[b * seed_stride .. + 4]; pats: normalized patterns [*][h][w].
  * Replaces Detector.seed + Detector.refine (src/detector.zig:178-212: detector.wgsl seed,
  * refine_init, sums_rot, refine_rot_step, nmi, collect), fused into one launch. */
-int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* items,
-                  const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* opts, float* records,
-                  int32_t out0, void* stream);
+int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, const zb_frame* frame, const float* pats,
+                  const float* items, const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* opts,
+                  float* records, int32_t out0, void* stream);
 
 /* Flag of zb_sums: zero the sums of the shift directions (orientation refinement's view). */
 #define ZB_SUMS_ROTATION_ONLY 2
@@ -213,6 +215,12 @@ int32_t zb_master_blur(const float* src, float* dst, int32_t side, float sigma_r
  * with its value in .w: grad [2][side][side][4].
  * Replaces Prep.gradient (src/prep.zig:253-260; mastersph.wgsl:124-138). */
 int32_t zb_master_gradient(const float* sheets, int32_t side, float* grad, void* stream);
+
+/* The gradient field in half precision, each value rounded to nearest even: grad_h
+ * [2][side][side][4] f16 (8 bytes a texel), for zb_refine's Gauss-Newton passes. The reference
+ * keeps only the float32 field (mastersph.wgsl:124-138); this copy halves the refinement's
+ * vector-memory instructions. */
+int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* grad_h, void* stream);
 
 /* Kernel timing (events around every launch while on). The reference times its dispatches the
  * same way (Gpu.setProfile / profileReport, src/gpu/gpu.zig:287-313); its C ABI has no entry

@@ -21,7 +21,7 @@ def declared():
 def test_header_declares_the_entry_points():
     names = declared()
     for n in ("zb_normalize", "zb_sh_coefficients", "zb_so3_correlate", "zb_so3_denominators", "zb_refine", "zb_project",
-              "zb_master_blur", "zb_master_gradient", "zb_last_error", "zb_version"):
+              "zb_master_blur", "zb_master_gradient", "zb_master_pack", "zb_last_error", "zb_version"):
         assert n in names
 
 
@@ -54,7 +54,7 @@ def test_bad_arguments_fail_with_a_message_and_launch_nothing():
     rc = lib.zb_vcorr_correlate(ctypes.byref(vt), None, None, 4, None, 4, None, 0, None, None)
     assert rc == -1 and "zb_vcorr_correlate" in nat.last_error()
     opts = nat.RefineOpts(5, 0.035, 1.0, 0.02, 1e4, 64, 1)  # 64 NMI bins: too many
-    rc = lib.zb_refine(None, 501, ctypes.byref(nat.Frame()), None, None, None, 8, 4, ctypes.byref(opts), None, 0, None)
+    rc = lib.zb_refine(None, None, 501, ctypes.byref(nat.Frame()), None, None, None, 8, 4, ctypes.byref(opts), None, 0, None)
     assert rc == -1
 
 

@@ -87,6 +87,15 @@ def test_master_blur_and_gradient(case):
     assert np.abs(gg[:, 3] - g[:, 3]).max() == 0.0
 
 
+def test_master_pack_rounds_the_field_to_half(case):
+    """zb_master_pack: every value of the gradient field rounded to the nearest half (ties to
+    even), bit for bit as numpy's float16 conversion."""
+    e = case["e"]
+    gh = e.grad_half(0).cpu().numpy().view(np.uint16)
+    ref = case["grad"].reshape(-1).astype(np.float16).view(np.uint16)
+    assert gh.shape == ref.shape and np.array_equal(gh, ref)
+
+
 def _gpu_coefficients(case, flags=3, window=False):
     import torch
     from zebsdsolve_amd import _native as nat
@@ -203,7 +212,7 @@ def test_refinement_on_other_frames(case, h, w, mask_on):
     e.add_phase(NI)
     e.settings.apply(mask=mask_on)
     e.set_patterns(np.zeros((1, n, h, w), np.float32))
-    _, (_, grad_d, host) = e.baked(0)
+    _, (sheets_d, grad_d, host) = e.baked(0)
     side = e.phases[0].master.side
     R = op.bruker_R(70.0, 0.0)
     fr = op.frame(R, h, w, mask_on=mask_on)
@@ -226,7 +235,7 @@ def test_refinement_on_other_frames(case, h, w, mask_on):
     sd = torch.from_numpy(seeds.astype(np.float32)).cuda()
     opts = nat.RefineOpts(4, float(np.radians(2.0)), 1.0, 0.02, 1e4, 16, 9)
     rec = torch.empty((n, 8), dtype=torch.float32, device="cuda")
-    nat.check(lib.zb_refine(grad_d.data_ptr(), side, e.frame(), pn.data_ptr(), items.data_ptr(), sd.data_ptr(), 4, n, opts, rec.data_ptr(), 0, st),
+    nat.check(lib.zb_refine(sheets_d.data_ptr(), e.grad_half(0).data_ptr(), side, e.frame(), pn.data_ptr(), items.data_ptr(), sd.data_ptr(), 4, n, opts, rec.data_ptr(), 0, st),
               "refine")
     r = rec.cpu().numpy()
     assert np.all(r[:, 7] == 9.0)

@@ -50,11 +50,12 @@ _SIGS = {
     "zb_vcorr_workspace": ([ctypes.POINTER(VTable), i32, i32], i64),
     "zb_vcorr_correlate": ([ctypes.POINTER(VTable), vp, vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_vcorr_denominators": ([ctypes.POINTER(VTable), ctypes.POINTER(VTable), vp, vp, i32, vp, i64, vp, vp], i32),
-    "zb_refine": ([vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
+    "zb_refine": ([vp, vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
     "zb_sums": ([vp, i32, ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),
     "zb_project": ([vp, i32, ctypes.POINTER(Frame), vp, i32, vp, vp], i32),
     "zb_master_blur": ([vp, vp, i32, ctypes.c_float, vp], i32),
     "zb_master_gradient": ([vp, i32, vp, vp], i32),
+    "zb_master_pack": ([vp, i32, vp, vp], i32),
     "zb_profile": ([i32], i32),
     "zb_profile_report": ([ctypes.c_char_p, i32], i32),
 }

@@ -74,15 +74,33 @@ __device__ __forceinline__ float3 normalize3(float3 v) {
     return make_float3(v.x * r, v.y * r, v.z * r);
 }
 
+// arctan(r) for r in [0, 1] as r P(r^2), P of degree 7: coefficients fitted here to arctan on
+// [0, 1] by weighted least squares reweighted toward equal ripple; in float32 Horner evaluation
+// the error is at most 1.2e-7 (2 ulp of pi / 4).
+__device__ __forceinline__ float atan01(float r) {
+    const float x = r * r;
+    float p = -0.0040545594f;
+    p = fmaf(p, x, 0.021862937f);
+    p = fmaf(p, x, -0.055912312f);
+    p = fmaf(p, x, 0.096421979f);
+    p = fmaf(p, x, -0.13908631f);
+    p = fmaf(p, x, 0.19946566f);
+    p = fmaf(p, x, -0.33329862f);
+    p = fmaf(p, x, 0.99999934f);
+    return p * r;
+}
+
 // Unit vector -> equal-area square coordinates in [-1, 1]^2 of its own hemisphere. 1 - |z| is
 // taken as (x^2 + y^2) / (1 + |z|), which keeps its digits near the pole (common.wgsl:44-57).
+// The square root and reciprocals are the hardware's (1 ulp).
 __device__ __forceinline__ float2 sphere_to_square(float3 v) {
     float ax = fabsf(v.x), ay = fabsf(v.y);
-    float xn = sqrtf(fmaxf(ZB_PI * 0.5f * (ax * ax + ay * ay) * __builtin_amdgcn_rcpf(1.f + fabsf(v.z)), 0.f));
+    float xn = __builtin_amdgcn_sqrtf(fmaxf(ZB_PI * 0.5f * (ax * ax + ay * ay) * __builtin_amdgcn_rcpf(1.f + fabsf(v.z)), 0.f));
     // one arctangent of the smaller over the larger component (the same value either branch of
     // the reference computes, without a divergent pair of calls)
     const bool xbig = ax >= ay;
-    const float t = (4.f * xn / ZB_PI) * atan2f(xbig ? ay : ax, fmaxf(xbig ? ax : ay, 1e-30f));
+    const float t = (1.2732395447351627f * xn) * atan01(  // 4 / pi
+        (xbig ? ay : ax) * __builtin_amdgcn_rcpf(fmaxf(xbig ? ax : ay, 1e-30f)));
     const float u = xbig ? xn : t, w = xbig ? t : xn;
     const float sc = 0.79788456080286536f;  // 1 / sqrt(pi / 2)
     return make_float2(u * sc * (v.x < 0.f ? -1.f : 1.f), w * sc * (v.y < 0.f ? -1.f : 1.f));
@@ -127,19 +145,122 @@ __device__ __forceinline__ float sheet_bilinear<float>(const float* __restrict__
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a + (b - a) * t; }
 
-template <>
-__device__ __forceinline__ float4 sheet_bilinear<float4>(const float4* __restrict__ s, int side, float3 v) {
+// The same lookup in two halves, so that a lane can have the loads of several points in flight
+// before it blends any: sheet_taps computes the texel and issues the four loads, taps_blend
+// interpolates (the arithmetic of sheet_bilinear, operation for operation).
+template <typename T>
+struct Taps {
+    T a, b, c, d;
+    float tj, ti;
+};
+
+// The texel (offset of its top-left corner over both sheets) and the fractions for direction v.
+__device__ __forceinline__ int sheet_texel(int side, float3 v, float& tj, float& ti) {
     float2 uv = sphere_to_square(v);
     float last = (float)(side - 1);
     float fj = fminf(fmaxf((uv.x + 1.f) * 0.5f * last, 0.f), last);
     float fi = fminf(fmaxf((uv.y + 1.f) * 0.5f * last, 0.f), last);
     int c0 = min((int)fj, side - 2), r0 = min((int)fi, side - 2);
-    int o = r0 * side + c0 + (v.z < 0.f ? side * side : 0);
-    float tj = fj - (float)c0, ti = fi - (float)r0;
-    float4 a = s[o], b = s[o + 1], c = s[o + side], d = s[o + side + 1];
+    tj = fj - (float)c0;
+    ti = fi - (float)r0;
+    return r0 * side + c0 + (v.z < 0.f ? side * side : 0);
+}
+
+template <typename T>
+__device__ __forceinline__ void sheet_taps(const T* __restrict__ s, int side, float3 v, Taps<T>& t) {
+    const int o = sheet_texel(side, v, t.tj, t.ti);
+    t.a = s[o];
+    t.b = s[o + 1];
+    t.c = s[o + side];
+    t.d = s[o + side + 1];
+}
+
+// Only the value channel (w) of a (grad M, M) sheet: a quarter of the bytes.
+__device__ __forceinline__ void sheet_taps_w(const float4* __restrict__ s, int side, float3 v, Taps<float>& t) {
+    const int o = sheet_texel(side, v, t.tj, t.ti);
+    t.a = s[o].w;
+    t.b = s[o + 1].w;
+    t.c = s[o + side].w;
+    t.d = s[o + side + 1].w;
+}
+
+// ── half-precision texels: (grad M, M) as four halves, 8 bytes ──
+typedef _Float16 zb_half2 __attribute__((ext_vector_type(2)));
+typedef unsigned zb_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack_half2(float a, float b) {
+    const zb_half2 h = {(_Float16)a, (_Float16)b};
+    return __builtin_bit_cast(unsigned, h);
+}
+
+__device__ __forceinline__ float2 unpack_half2(unsigned u) {
+    const zb_half2 h = __builtin_bit_cast(zb_half2, u);
+    return make_float2((float)h.x, (float)h.y);
+}
+
+// The four texels of a lookup in the half-precision field: each row's pair in one 16-byte load
+// (texels o, o + 1, then o + side, o + side + 1), through a buffer resource over the field.
+struct TapsH {
+    zb_u32x4 top, bot;
+    float tj, ti;
+};
+
+__device__ __forceinline__ void sheet_taps_h(__amdgpu_buffer_rsrc_t field, int side, float3 v, TapsH& t) {
+    const int o = sheet_texel(side, v, t.tj, t.ti);
+    t.top = __builtin_amdgcn_raw_buffer_load_b128(field, o * 8, 0, 0);
+    t.bot = __builtin_amdgcn_raw_buffer_load_b128(field, (o + side) * 8, 0, 0);
+}
+
+// The four texels of a lookup in float32 sheets: each row's pair in one 8-byte load.
+struct TapsM {
+    float2 top, bot;
+    float tj, ti;
+};
+
+__device__ __forceinline__ void sheet_taps_m(__amdgpu_buffer_rsrc_t sheets, int side, float3 v, TapsM& t) {
+    const int o = sheet_texel(side, v, t.tj, t.ti);
+    t.top = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sheets, o * 4, 0, 0));
+    t.bot = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sheets, (o + side) * 4, 0, 0));
+}
+
+__device__ __forceinline__ float taps_blend(const TapsM& t) {
+    float top = lerpf(t.top.x, t.top.y, t.tj), bot = lerpf(t.bot.x, t.bot.y, t.tj);
+    return lerpf(top, bot, t.ti);
+}
+
+__device__ __forceinline__ float taps_blend(const Taps<float>& t) {
+    float top = lerpf(t.a, t.b, t.tj), bot = lerpf(t.c, t.d, t.tj);
+    return lerpf(top, bot, t.ti);
+}
+
+__device__ __forceinline__ float4 taps_blend(const Taps<float4>& t) {
+    const float4 a = t.a, b = t.b, c = t.c, d = t.d;
+    const float tj = t.tj, ti = t.ti;
     float4 top = make_float4(lerpf(a.x, b.x, tj), lerpf(a.y, b.y, tj), lerpf(a.z, b.z, tj), lerpf(a.w, b.w, tj));
     float4 bot = make_float4(lerpf(c.x, d.x, tj), lerpf(c.y, d.y, tj), lerpf(c.z, d.z, tj), lerpf(c.w, d.w, tj));
     return make_float4(lerpf(top.x, bot.x, ti), lerpf(top.y, bot.y, ti), lerpf(top.z, bot.z, ti), lerpf(top.w, bot.w, ti));
+}
+
+__device__ __forceinline__ float4 taps_blend(const TapsH& t) {
+    Taps<float4> f;
+    float2 lo = unpack_half2(t.top.x), hi = unpack_half2(t.top.y);
+    f.a = make_float4(lo.x, lo.y, hi.x, hi.y);
+    lo = unpack_half2(t.top.z), hi = unpack_half2(t.top.w);
+    f.b = make_float4(lo.x, lo.y, hi.x, hi.y);
+    lo = unpack_half2(t.bot.x), hi = unpack_half2(t.bot.y);
+    f.c = make_float4(lo.x, lo.y, hi.x, hi.y);
+    lo = unpack_half2(t.bot.z), hi = unpack_half2(t.bot.w);
+    f.d = make_float4(lo.x, lo.y, hi.x, hi.y);
+    f.tj = t.tj;
+    f.ti = t.ti;
+    return taps_blend(f);
+}
+
+template <>
+__device__ __forceinline__ float4 sheet_bilinear<float4>(const float4* __restrict__ s, int side, float3 v) {
+    Taps<float4> t;
+    sheet_taps<float4>(s, side, v, t);
+    return taps_blend(t);
 }
 
 // ── complex helpers ──

@@ -194,3 +194,23 @@ extern "C" int32_t zb_master_gradient(const float* sheets, int32_t side, float* 
     ZB_LAUNCH("k_gradient", k_gradient, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, sheets, side, h, (float4*)grad);
     return check_launch("zb_master_gradient");
 }
+
+// The gradient field in half precision (each float rounded to nearest even): 8 bytes a texel, so
+// that a texel and its right-hand neighbour come in one 16-byte load.
+__global__ void __launch_bounds__(256) k_pack_half(const float4* __restrict__ grad, int total, uint2* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const float4 g = grad[i];
+    out[i] = make_uint2(pack_half2(g.x, g.y), pack_half2(g.z, g.w));
+}
+
+extern "C" int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* grad_h, void* stream) {
+    if (!grad || !grad_h || side < 2) {
+        set_error("zb_master_pack: bad arguments");
+        return ZB_EINVAL;
+    }
+    const int total = 2 * side * side;
+    ZB_LAUNCH("k_pack_half", k_pack_half, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, (const float4*)grad, total,
+              (uint2*)grad_h);
+    return check_launch("zb_master_pack");
+}

@@ -71,31 +71,51 @@ __device__ int build_list(const RefArg& a, unsigned short* list, int lane) {
     return total;
 }
 
-__device__ Sums rot_sums(const RefArg& a, const float4* __restrict__ mg, const float* __restrict__ pat, const unsigned short* list, int n_px,
+// The ray of list pixel `pix` (row-major index) from the item's origin t, in the sample frame.
+__device__ __forceinline__ float3 pixel_ray(const RefArg& a, int pix, float inv_w, float3 t) {
+    const int row = (int)(((float)pix + 0.5f) * inv_w);
+    const float fr = (float)row, fc = (float)(pix - row * a.w);
+    return make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
+}
+
+// Pixels per lane whose gathers are in flight together: a lane issues the loads of kBatch pixels
+// (3 each) before it blends any.
+constexpr int kBatch = 4;
+
+__device__ Sums rot_sums(const RefArg& a, __amdgpu_buffer_rsrc_t mgh, const float* __restrict__ pat, const unsigned short* list, int n_px,
                          float inv_w, float4 q, float3 t, int lane) {
     Sums s;
     s.n = s.m = s.mm = s.me = 0.f;
     s.sd = s.sv = s.se = s.hd = s.ho = make_float3(0.f, 0.f, 0.f);
-    // four pixels' gathers in flight per lane (the lookups of the 8 MB gradient field miss L2)
-#pragma unroll 4
-    for (int k = lane; k < n_px; k += 64) {
-        const int pix = list[k];
-        const int row = (int)(((float)pix + 0.5f) * inv_w);
-        const float fr = (float)row, fc = (float)(pix - row * a.w);
-        float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
-        const float3 c = qrot(q, normalize3(u));
-        const float4 m = sheet_bilinear<float4>(mg, a.side, c);
-        const float3 d = cross3(c, make_float3(m.x, m.y, m.z));
-        const float e = pat[pix];
-        s.n += 1.f;
-        s.m += m.w;
-        s.mm += m.w * m.w;
-        s.me += m.w * e;
-        s.sd = f3add(s.sd, d);
-        s.sv = f3add(s.sv, f3s(d, m.w));
-        s.se = f3add(s.se, f3s(d, e));
-        s.hd = f3add(s.hd, make_float3(d.x * d.x, d.y * d.y, d.z * d.z));
-        s.ho = f3add(s.ho, make_float3(d.x * d.y, d.x * d.z, d.y * d.z));
+    for (int k0 = lane; k0 < n_px; k0 += 64 * kBatch) {
+        TapsH tp[kBatch];
+        float3 c[kBatch];
+        float e[kBatch];
+#pragma unroll
+        for (int p = 0; p < kBatch; ++p) {
+            // lanes past the end look up the last pixel again and add nothing
+            const int pix = list[min(k0 + 64 * p, n_px - 1)];
+            c[p] = qrot(q, normalize3(pixel_ray(a, pix, inv_w, t)));
+            sheet_taps_h(mgh, a.side, c[p], tp[p]);
+            e[p] = pat[pix];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int p = 0; p < kBatch; ++p) {
+            if (k0 + 64 * p < n_px) {
+                const float4 m = taps_blend(tp[p]);
+                const float3 d = cross3(c[p], make_float3(m.x, m.y, m.z));
+                s.n += 1.f;
+                s.m += m.w;
+                s.mm += m.w * m.w;
+                s.me += m.w * e[p];
+                s.sd = f3add(s.sd, d);
+                s.sv = f3add(s.sv, f3s(d, m.w));
+                s.se = f3add(s.se, f3s(d, e[p]));
+                s.hd = f3add(s.hd, make_float3(d.x * d.x, d.y * d.y, d.z * d.z));
+                s.ho = f3add(s.ho, make_float3(d.x * d.y, d.x * d.z, d.y * d.z));
+            }
+        }
     }
     s.n = wsum(s.n);
     s.m = wsum(s.m);
@@ -107,6 +127,24 @@ __device__ Sums rot_sums(const RefArg& a, const float4* __restrict__ mg, const f
     s.hd = wsum3(s.hd);
     s.ho = wsum3(s.ho);
     return s;
+}
+
+// The master's values at orientation q for the list pixels k0 + 64 p, p < kBatch (lanes past the
+// end get the last pixel's), with the pattern's values there: all gathers issued before any
+// blend. msh: a buffer resource over the float32 sheets.
+__device__ __forceinline__ void sims_batch(const RefArg& a, __amdgpu_buffer_rsrc_t msh, const float* __restrict__ pat,
+                                           const unsigned short* list, int n_px, float inv_w, float4 q, float3 t, int k0,
+                                           float* sim, float* e) {
+    TapsM tp[kBatch];
+#pragma unroll
+    for (int p = 0; p < kBatch; ++p) {
+        const int pix = list[min(k0 + 64 * p, n_px - 1)];
+        sheet_taps_m(msh, a.side, qrot(q, normalize3(pixel_ray(a, pix, inv_w, t))), tp[p]);
+        e[p] = pat[pix];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int p = 0; p < kBatch; ++p) sim[p] = taps_blend(tp[p]);
 }
 
 __device__ __forceinline__ float det3(float3 c0, float3 c1, float3 c2) {
@@ -123,19 +161,24 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
     return make_float3(det3(g, m1, m2) / det, det3(m0, g, m2) / det, det3(m0, m1, g) / det);
 }
 
+// The Gauss-Newton passes read the half-precision gradient field (mgh: one 16-byte load per
+// texel row pair, half the vector-memory instructions of float32 texels); the recorded NCC and
+// the NMI are those of the float32 master (ms, the values the field holds in .w) at the best
+// orientation, from the NMI's first pass (one 8-byte load per texel row pair).
 // MAXP > 0: each lane keeps the master values of its (at most MAXP) pixels at the best
 // orientation from the first NMI pass for the second; MAXP = 0: the second pass looks them up
 // again.
 template <int MAXP>
-__global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restrict__ mg, const float* __restrict__ pats,
+__global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict__ ms, const uint2* __restrict__ mgh, const float* __restrict__ pats,
                                                const float4* __restrict__ items, const float* __restrict__ seeds, int n_items,
                                                float* __restrict__ records) {
+    static_assert(MAXP % kBatch == 0, "MAXP: whole batches");
     extern __shared__ unsigned lds[];
-    unsigned* hist = lds;                                          // [32][32]
-    unsigned short* list = (unsigned short*)(lds + 32 * 32);       // the masked pixels
+    unsigned* hist = lds;                                               // [nmi_bins][nmi_bins]
+    unsigned short* list = (unsigned short*)(lds + a.nmi_bins * a.nmi_bins);  // the masked pixels
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
-    for (int i = lane; i < 32 * 32; i += 64) hist[i] = 0u;
+    for (int i = lane; i < a.nmi_bins * a.nmi_bins; i += 64) hist[i] = 0u;
     const int n_px = build_list(a, list, lane);
     const float inv_w = 1.f / (float)a.w;
     const float4 it = items[b];
@@ -151,9 +194,11 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
     float3 gbest = make_float3(0.f, 0.f, 0.f);
     float hbest[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float last_step = 0.f, promised = 0.f, step_size = 0.f, accepted = 0.f;
+    const __amdgpu_buffer_rsrc_t fh = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint2*>(mgh), (short)0, 2 * a.side * a.side * 8, 0x00020000);
+    const __amdgpu_buffer_rsrc_t fm = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(ms), (short)0, 2 * a.side * a.side * 4, 0x00020000);
 
     for (int itn = 0; itn <= a.iters; ++itn) {
-        const Sums s = rot_sums(a, mg, pat, list, n_px, inv_w, cur, t, lane);
+        const Sums s = rot_sums(a, fh, pat, list, n_px, inv_w, cur, t, lane);
         // NCC, its gradient and Gauss-Newton matrix over the rotation (detector.wgsl:561-583)
         const float n = fmaxf(s.n, 1.f);
         const float mean = s.m / n;
@@ -212,39 +257,37 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
     // normalized mutual information of the pattern and the master at the best orientation,
     // each binned between its own extremes under the mask (detector.wgsl:739-832)
     const int nb = a.nmi_bins;
-    auto sim_at = [&](int pix) {
-        const int row = (int)(((float)pix + 0.5f) * inv_w);
-        const float fr = (float)row, fc = (float)(pix - row * a.w);
-        float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
-        return sheet_bilinear<float4>(mg, a.side, qrot(qbest, normalize3(u))).w;
-    };
     float emin = 3.4e38f, emax = -3.4e38f, smin = 3.4e38f, smax = -3.4e38f;
+    float f_n = 0.f, f_m = 0.f, f_mm = 0.f, f_me = 0.f;  // the NCC's sums there (float32 master)
+    auto extremes = [&](float e, float sim) {
+        emin = fminf(emin, e);
+        emax = fmaxf(emax, e);
+        smin = fminf(smin, sim);
+        smax = fmaxf(smax, sim);
+        f_n += 1.f;
+        f_m += sim;
+        f_mm += sim * sim;
+        f_me += sim * e;
+    };
     float simv[MAXP > 0 ? MAXP : 1];
     if constexpr (MAXP > 0) {
 #pragma unroll
-        for (int i = 0; i < MAXP; ++i) {
-            const int k = lane + 64 * i;
-            simv[i] = 0.f;
-            if (k < n_px) {
-                const int pix = list[k];
-                const float e = pat[pix];
-                const float sim = sim_at(pix);
-                simv[i] = sim;
-                emin = fminf(emin, e);
-                emax = fmaxf(emax, e);
-                smin = fminf(smin, sim);
-                smax = fmaxf(smax, sim);
+        for (int i0 = 0; i0 < MAXP; i0 += kBatch) {
+            float sim[kBatch] = {}, e[kBatch] = {};
+            if (64 * i0 < n_px) sims_batch(a, fm, pat, list, n_px, inv_w, qbest, t, lane + 64 * i0, sim, e);
+#pragma unroll
+            for (int p = 0; p < kBatch; ++p) {
+                simv[i0 + p] = sim[p];
+                if (lane + 64 * (i0 + p) < n_px) extremes(e[p], sim[p]);
             }
         }
     } else {
-        for (int k = lane; k < n_px; k += 64) {
-            const int pix = list[k];
-            const float e = pat[pix];
-            const float sim = sim_at(pix);
-            emin = fminf(emin, e);
-            emax = fmaxf(emax, e);
-            smin = fminf(smin, sim);
-            smax = fmaxf(smax, sim);
+        for (int k0 = lane; k0 < n_px; k0 += 64 * kBatch) {
+            float sim[kBatch], e[kBatch];
+            sims_batch(a, fm, pat, list, n_px, inv_w, qbest, t, k0, sim, e);
+#pragma unroll
+            for (int p = 0; p < kBatch; ++p)
+                if (k0 + 64 * p < n_px) extremes(e[p], sim[p]);
         }
     }
     for (int o = 1; o < 64; o <<= 1) {
@@ -252,6 +295,16 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
         emax = fmaxf(emax, __shfl_xor(emax, o, 64));
         smin = fminf(smin, __shfl_xor(smin, o, 64));
         smax = fmaxf(smax, __shfl_xor(smax, o, 64));
+    }
+    // the recorded NCC: that of the float32 master at the best orientation (the rule of the
+    // Gauss-Newton passes)
+    float ncc_best = -2.f;
+    {
+        const float n = fmaxf(wsum(f_n), 1.f);
+        const float mean = wsum(f_m) / n;
+        const float var_n = wsum(f_mm) - n * mean * mean;
+        const float me = wsum(f_me);
+        if (var_n > 1e-12f) ncc_best = me * (1.f / sqrtf(var_n));
     }
     const float ke = (float)nb / fmaxf(emax - emin, 1e-20f);
     const float ks = (float)nb / fmaxf(smax - smin, 1e-20f);
@@ -262,14 +315,23 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
     };
     if constexpr (MAXP > 0) {
 #pragma unroll
-        for (int i = 0; i < MAXP; ++i) {
-            const int k = lane + 64 * i;
-            if (k < n_px) bin(pat[list[k]], simv[i]);
+        for (int i0 = 0; i0 < MAXP; i0 += kBatch) {
+            if (64 * i0 < n_px) {
+                float e[kBatch];
+#pragma unroll
+                for (int p = 0; p < kBatch; ++p) e[p] = pat[list[min(lane + 64 * (i0 + p), n_px - 1)]];
+#pragma unroll
+                for (int p = 0; p < kBatch; ++p)
+                    if (lane + 64 * (i0 + p) < n_px) bin(e[p], simv[i0 + p]);
+            }
         }
     } else {
-        for (int k = lane; k < n_px; k += 64) {
-            const int pix = list[k];
-            bin(pat[pix], sim_at(pix));
+        for (int k0 = lane; k0 < n_px; k0 += 64 * kBatch) {
+            float sim[kBatch], e[kBatch];
+            sims_batch(a, fm, pat, list, n_px, inv_w, qbest, t, k0, sim, e);
+#pragma unroll
+            for (int p = 0; p < kBatch; ++p)
+                if (k0 + 64 * p < n_px) bin(e[p], sim[p]);
         }
     }
     __syncthreads();
@@ -301,7 +363,7 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float4* __restric
         r[1] = qbest.y;
         r[2] = qbest.z;
         r[3] = qbest.w;
-        r[4] = best;
+        r[4] = ncc_best;
         r[5] = nmi;
         r[6] = last_step;
         r[7] = a.stamp;
@@ -425,15 +487,16 @@ extern "C" int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* fra
     return check_launch("zb_sums");
 }
 
-extern "C" int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* items, const float* seeds,
-                             int32_t seed_stride, int32_t n_items, const zb_refine_opts* o, float* records, int32_t out0, void* stream) {
+extern "C" int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, const zb_frame* frame, const float* pats, const float* items,
+                             const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* o, float* records, int32_t out0,
+                             void* stream) {
     if (!frame || !o || side < 2 || n_items < 0 || frame->pat_h < 2 || frame->pat_w < 2 || seed_stride < 4 || out0 < 0 || o->iters < 0 ||
         o->nmi_bins < 2 || o->nmi_bins > 32) {
         set_error("zb_refine: bad arguments");
         return ZB_EINVAL;
     }
     if (n_items == 0) return ZB_OK;
-    if (!mgrad || !pats || !items || !seeds || !records) {
+    if (!master || !mgrad_h || !pats || !items || !seeds || !records) {
         set_error("zb_refine: null pointer");
         return ZB_EINVAL;
     }
@@ -451,16 +514,17 @@ extern "C" int32_t zb_refine(const float* mgrad, int32_t side, const zb_frame* f
     a.scale_min = o->scale_min;
     a.scale_max = o->scale_max;
     a.stamp = (float)o->stamp;
-    const size_t lds = 32 * 32 * 4 + list_bytes;
+    // the joint histogram, then the list (the histogram's size keeps the list 4-byte aligned)
+    const size_t lds = (size_t)o->nmi_bins * o->nmi_bins * 4 + list_bytes;
     if (lds > 64 * 1024) {
-        set_error("zb_refine: %d masked pixels: at most %d", n_px, (64 * 1024 - 4096) / 2);
+        set_error("zb_refine: %d masked pixels: at most %d", n_px, (int)(64 * 1024 - o->nmi_bins * o->nmi_bins * 4) / 2);
         return ZB_EINVAL;
     }
     if (n_px <= 64 * 48)
-        ZB_LAUNCH("k_refine", (k_refine<48>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items,
-                  seeds, n_items, records);
+        ZB_LAUNCH("k_refine", (k_refine<48>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
+                  (const float4*)items, seeds, n_items, records);
     else
-        ZB_LAUNCH("k_refine", (k_refine<0>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, (const float4*)items,
-                  seeds, n_items, records);
+        ZB_LAUNCH("k_refine", (k_refine<0>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
+                  (const float4*)items, seeds, n_items, records);
     return check_launch("zb_refine");
 }

@@ -112,6 +112,7 @@ class Phase:
     def __init__(self, m):
         self.master = m
         self.baked = {}     # bake key -> (sheets_dev [2, s, s], grad_dev [2, s, s, 4], sheets_host)
+        self.baked_h = {}   # bake key -> grad_dev in half precision [2, s, s, 4] (zb_master_pack)
         self.bands = {}     # (L, bake key) -> DevicePhase
 
 
@@ -245,6 +246,16 @@ class Engine:
         p.baked[key] = (cur, grad, np.asarray(host, np.float32).reshape(2, side, side))
         return key, p.baked[key]
 
+    def grad_half(self, pi):
+        """The baked gradient field in half precision, which zb_refine's Gauss-Newton passes read."""
+        p = self.phases[pi]
+        key, (_, grad, _) = self.baked(pi)
+        if key not in p.baked_h:
+            gh = torch.empty((2 * p.master.side * p.master.side * 4,), dtype=torch.float16, device=grad.device)
+            nat.check(nat.load().zb_master_pack(grad.data_ptr(), p.master.side, gh.data_ptr(), stream_ptr()), "master pack")
+            p.baked_h[key] = gh
+        return p.baked_h[key]
+
     def band(self, L):
         if L not in self.bands:
             self.bands[L] = DeviceBand(L, self._device())
@@ -369,7 +380,8 @@ class Engine:
         else:
             rdens = [self.rden(pi, L) for pi in range(n_ph)]
             corr_ws = [lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k)) for ph in phs]
-        grads = [self.baked(pi)[1][1] for pi in range(n_ph)]
+        sheets = [self.baked(pi)[1][0] for pi in range(n_ph)]
+        grads_h = [self.grad_half(pi) for pi in range(n_ph)]
         # bytes per item at scale (the matrix-core path pads items to a multiple of 64)
         per_item = max(f(4096) for f in corr_ws) // 4096 + 1
         chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
@@ -410,7 +422,7 @@ class Engine:
                     nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
                                                    workspace.numel(), peaks.data_ptr(), st), "correlate")
                 rec = records[pi]
-                nat.check(lib.zb_refine(grads[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(),
+                nat.check(lib.zb_refine(sheets[pi].data_ptr(), grads_h[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(),
                                         8, nt, opts, rec.data_ptr(), c0, st), "refine")
         return records, opts.stamp
 
@@ -460,8 +472,8 @@ class Engine:
         o = self.refine_opts()
         o.iters = 0
         rec = torch.empty((n, RECORD), dtype=torch.float32, device=dev)
-        _, (_, grad, _) = self.baked(pi)
-        nat.check(lib.zb_refine(grad.data_ptr(), self.phases[pi].master.side, self.frame(), pats.data_ptr(), items.data_ptr(),
+        _, (sheets, _, _) = self.baked(pi)
+        nat.check(lib.zb_refine(sheets.data_ptr(), self.grad_half(pi).data_ptr(), self.phases[pi].master.side, self.frame(), pats.data_ptr(), items.data_ptr(),
                                 seeds.data_ptr(), 4, n, o, rec.data_ptr(), 0, st), "nmi")
         return rec[:, 5].cpu().numpy()
 
