@@ -303,21 +303,46 @@ __device__ __forceinline__ float2 zterm(float2 x, float2 yc, int k) {
     return make_float2(x.x + yc.x - t.y, x.y + yc.y + t.x);
 }
 
-template <int H, int LH, int MODE>
-__global__ void __launch_bounds__(64) k_apeak(VArg a, const float2* __restrict__ v, const float2* __restrict__ rden, int rden_div,
-                                              unsigned long long* __restrict__ best, float2* __restrict__ vol) {
-    const int lane = threadIdx.x;
+// ONEVOL: every block's items share one denominator volume (one volume for the call, or volumes
+// of whole blocks), whose columns for the block's j then go through LDS.
+template <int H, int LH, int MODE, bool ONEVOL>
+__global__ void __launch_bounds__(256) k_apeak(VArg a, const float2* __restrict__ v, const float* __restrict__ rden, int rden_div,
+                                               unsigned long long* __restrict__ best, float2* __restrict__ vol) {
+    // MODE 0 with one volume: the denominators of the block's j, float4 slot (a4 nc + c) holding
+    // column c's values 4 a4 .. 4 a4 + 3 (adjacent columns in adjacent slots for the reads)
+    extern __shared__ float4 rlds[];
+    constexpr int N = 2 * H;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int sl = lane >> a.log2nc;
     const int c = lane & (a.nc - 1);
     // slabs (j, item b) over the padded items, items innermost: a block's slabs share j (ni_pad
-    // is a multiple of the slabs per block) and read adjacent items' rows
-    const int per_block = 64 >> a.log2nc;
+    // is a multiple of the slabs per block) and its waves read adjacent items' rows
+    const int per_wave = 64 >> a.log2nc;
+    const int nw = (int)(blockDim.x >> 6);
+    const int per_block = per_wave * nw;
     const long slab0 = (long)blockIdx.x * per_block;
     const int j = (int)(slab0 / a.ni_pad);
-    const int b = (int)(slab0 - (long)j * a.ni_pad) + sl;
+    const int b0 = (int)(slab0 - (long)j * a.ni_pad);
     // a block wholly in the padding has nothing to do
-    if ((int)(slab0 - (long)j * a.ni_pad) >= a.n_items) return;
+    if (b0 >= a.n_items) return;
+    const int b = b0 + (tid >> 6) * per_wave + sl;
     const bool live = b < a.n_items;
+    constexpr bool one_vol = MODE == 0 && ONEVOL;
+    if constexpr (one_vol) {
+        // straight from memory into LDS (no registers), 64 slots an instruction, issued before the
+        // spectrum's loads
+        const float4* src = reinterpret_cast<const float4*>(rden + ((size_t)(b0 / rden_div) * a.nb + j) * a.nc * N);
+        const int slots = (a.nc * N) >> 2;
+        for (int k = tid >> 6; k * 64 < slots; k += nw) {
+            const int sidx = k * 64 + lane;
+            const int a4 = sidx >> a.log2nc, cc = sidx & (a.nc - 1);
+            if (sidx < slots)
+                __builtin_amdgcn_global_load_lds((const void*)(src + cc * (N / 4) + a4), (void __attribute__((address_space(3)))*)(rlds + k * 64),
+                                                 16, 0, 0);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     const int wc = a.cols / 2;                              // complex pairs per item row
     const unsigned rs = (unsigned)(a.ni_pad * wc) * 8u;    // bytes per m' row
     const auto vrs = buf(v + (size_t)j * a.L * a.ni_pad * wc, (long long)a.L * rs);
@@ -328,12 +353,16 @@ __global__ void __launch_bounds__(64) k_apeak(VArg a, const float2* __restrict__
     // the real transform along a (C = Re V_0 + 2 Re sum_{m' > 0} V_m' e^{-i m' a}) from a complex
     // one of half its length, built in place pairwise (q, h - q); X_0 is taken real
     constexpr int KS = 64 / H;  // w_q = exp(-2 pi i q / n) = exp(-2 pi i (q KS) / 128)
+    // For the pair (q, h - q) with s = X_q + conj X_{h-q}, d = X_q - conj X_{h-q}, t = w_q d:
+    // Z_q = s + i t and, as w_{h-q} = -conj w_q, Z_{h-q} = conj s + i conj t (one product a pair).
     z[0] = make_float2(z[0].x, z[0].x);
 #pragma unroll
     for (int q = 1; q < H / 2; ++q) {
         const float2 x = z[q], y = z[H - q];
-        z[q] = zterm(x, make_float2(y.x, -y.y), q * KS);
-        z[H - q] = zterm(y, make_float2(x.x, -x.y), (H - q) * KS);
+        const float2 sm = make_float2(x.x + y.x, x.y - y.y);
+        const float2 t = twmul(make_float2(x.x - y.x, x.y + y.y), q * KS);
+        z[q] = make_float2(sm.x - t.y, sm.y + t.x);
+        z[H - q] = make_float2(sm.x + t.y, t.x - sm.y);
     }
     z[H / 2] = zterm(z[H / 2], make_float2(z[H / 2].x, -z[H / 2].y), (H / 2) * KS);
     // radix-2 decimation in frequency: natural in, bit-reversed out
@@ -357,14 +386,28 @@ __global__ void __launch_bounds__(64) k_apeak(VArg a, const float2* __restrict__
         for (int p = 0; p < H; ++p) out[brev_c<LH>(p)] = z[p];
         return;
     }
-    // padding lanes (b >= n_items) read item 0's volume and are masked below
-    const float2* rd = rden + (((size_t)((live ? b : 0) / rden_div) * a.nb + j) * a.nc + c) * H;
     float m = -3.4e38f;
+    if constexpr (one_vol) {
+        // the loads into LDS were issued first: waiting for them waits for no spectrum load
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const float* rd = reinterpret_cast<const float*>(rlds) + c * 4;
 #pragma unroll
-    for (int p = 0; p < H; ++p) {
-        const float2 r = rd[brev_c<LH>(p)];
-        z[p] = make_float2(z[p].x * r.x, z[p].y * r.y);
-        m = fmaxf(m, fmaxf(z[p].x, z[p].y));
+        for (int p = 0; p < H; ++p) {
+            const int f2 = 2 * brev_c<LH>(p);  // values f2, f2 + 1: slot f2 / 4, offset f2 % 4
+            const float2 r = *reinterpret_cast<const float2*>(rd + (f2 >> 2) * a.nc * 4 + (f2 & 3));
+            z[p] = make_float2(z[p].x * r.x, z[p].y * r.y);
+            m = fmaxf(m, fmaxf(z[p].x, z[p].y));
+        }
+    } else {
+        // padding lanes (b >= n_items) read item 0's volume and are masked below
+        const float2* rd = reinterpret_cast<const float2*>(rden) + (((size_t)((live ? b : 0) / rden_div) * a.nb + j) * a.nc + c) * H;
+#pragma unroll
+        for (int p = 0; p < H; ++p) {
+            const float2 r = rd[brev_c<LH>(p)];
+            z[p] = make_float2(z[p].x * r.x, z[p].y * r.y);
+            m = fmaxf(m, fmaxf(z[p].x, z[p].y));
+        }
     }
     if (!live) m = -3.4e38f;
     float smax = m;
@@ -586,12 +629,26 @@ extern "C" int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int
 
 template <int MODE>
 static int launch_apeak(const VArg& a, const float* v, const float* rden, int rden_div, unsigned long long* best, float* vol, hipStream_t s) {
-    const int per_block = 64 >> a.log2nc;
-    const long slabs = (long)a.ni_pad * a.nb;  // ni_pad is a multiple of per_block
+    const int per_wave = 64 >> a.log2nc;
+    // up to four waves a block, at most 32 slabs (ni_pad is a multiple of 32)
+    const int waves = per_wave >= 32 ? 1 : (32 / per_wave < 4 ? 32 / per_wave : 4);
+    const int per_block = per_wave * waves;
+    const long slabs = (long)a.ni_pad * a.nb;
     const dim3 grid((unsigned)(slabs / per_block));
     const int H = a.n / 2;
-#define ZB_AP(HH, LL) \
-    ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE>), grid, dim3(64), 0, s, a, (const float2*)v, (const float2*)rden, rden_div, best, (float2*)vol)
+    // blocks start at multiples of per_block, so volumes of a multiple of per_block items never
+    // split a block
+    const bool onevol = MODE == 0 && (rden_div >= a.n_items || rden_div % per_block == 0);
+    const size_t lds = onevol ? (size_t)a.nc * a.n * sizeof(float) : 0;
+#define ZB_AP(HH, LL)                                                                                                              \
+    do {                                                                                                                           \
+        if (onevol)                                                                                                                \
+            ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE, true>), grid, dim3(64 * waves), lds, s, a, (const float2*)v, rden, rden_div, best, \
+                      (float2*)vol);                                                                                               \
+        else                                                                                                                       \
+            ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE, false>), grid, dim3(64 * waves), lds, s, a, (const float2*)v, rden, rden_div,     \
+                      best, (float2*)vol);                                                                                         \
+    } while (0)
     if (H == 8) ZB_AP(8, 3); else if (H == 16) ZB_AP(16, 4); else if (H == 32) ZB_AP(32, 5); else ZB_AP(64, 6);
 #undef ZB_AP
     return check_launch("k_apeak");
