@@ -11,21 +11,23 @@
 // left per pattern is a complex GEMM per (j, m') with the item's coefficients (K = its degrees l
 // >= m'), and the real transform along a.
 //
-//   k_vgemm   one wave per (m', column group, tile of 32 MB items), looping over every j: the
-//             items' coefficients for m' are the MFMA B operand (columns = items, k = (re,
-//             im) of one degree), held in registers across all j; the table rows are the A
-//             operand (rows = V columns, (re, im) interleaved, so one real MFMA per degree
-//             gives V as complex pairs), loaded one j ahead. v_mfma_f32_32x32x2_f32 (float32
-//             in, float32 accumulate). V goes to HBM as complex pairs [j][m'][item][c]: a lane
-//             holds four adjacent columns of one item per accumulator group (one 16-byte
-//             store), from a scalar base plus a per-lane offset that does not change with j.
-//   k_apeak   one lane per (j, item, c) column: the column's L orders into the half-length
-//             complex input of the real transform along a, a radix-2 FFT of h = n/2 points in
-//             registers (twiddles are compile-time constants, the trivial ones skipped), times
-//             the reciprocal denominator, the column's maximum, the slab's (xor shuffles over its
-//             nc lanes); only a slab that reaches the item's best so far looks up its lowest
-//             index of that value and offers (value, index) with one 64-bit atomicMax. Or the
-//             volume itself (denominators).
+//   k_vgemm   four waves per (m', column group, 128 items), each its own 32 items, looping over
+//             every j: the items' coefficients for m' are the MFMA A operand (rows = items, k =
+//             (re, im) of one degree), held in registers across all j; the table rows the B
+//             operand (columns = V columns, (re, im) interleaved, so one real MFMA per degree
+//             gives V as complex pairs), staged once per block through LDS one j ahead.
+//             v_mfma_f32_32x32x2_f32 (float32 in, float32 accumulate). V goes to HBM as complex
+//             pairs [j][m'][item][c]: per accumulator register the wave writes two items' whole
+//             runs of 32 columns (128 bytes, a cache line each), from a scalar base plus a
+//             per-lane offset that does not change with j.
+//   k_apeak   one lane per (j, item, c) column, four waves a block on one j: the column's L
+//             orders into the half-length complex input of the real transform along a, a
+//             radix-2 FFT of h = n/2 points in registers (twiddles are compile-time constants,
+//             the trivial ones skipped), times the reciprocal denominator (the block's columns,
+//             loaded straight into LDS ahead of the spectrum), the column's maximum, the slab's
+//             (xor shuffles over its nc lanes); only a slab that reaches the item's best so far
+//             looks up its lowest index of that value and offers (value, index) with one 64-bit
+//             atomicMax. Or the volume itself (denominators).
 //   k_vfinish the peak's six neighbours summed directly from V (already along c), parabolic
 //             offsets and the orientation, as so3.hip's k_finish.
 //
@@ -128,12 +130,12 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
     const size_t vrow = (size_t)a.ni_pad * w;
     unsigned loff[MB];
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + r32) * w + c0 + half * 4);
+    for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + half * 4) * w + c0 + r32);
     // one sum per output tile: the other waves on the SIMD fill the wait of a chain's next
     // product for the previous one
     typedef floatx16 Acc[MB][NBLK][1];
-    // one j: prefetch the next j's rows, the products (the rows are the A operand, rows = V
-    // columns; the coefficients the B operand, columns = items), the stores, the staging of the
+    // one j: prefetch the next j's rows, the products (the coefficients are the A operand, rows =
+    // items; the table rows the B operand, columns = V columns), the stores, the staging of the
     // next rows. Two accumulator sets alternate between consecutive j, so the products of j + 1
     // never wait for the stores of j to read their registers.
     auto step = [&](int j, Acc& acc, Acc& other, const floatx4* cur, floatx4* nxt) {
@@ -158,7 +160,7 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
 #pragma unroll
                     for (int nb = 0; nb < NBLK; ++nb) {
                         floatx16& c = acc[mb][nb][0];
-                        c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[nb][i], areg[mb][S0 + 4 * k + i], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x2f32(areg[mb][S0 + 4 * k + i], fa[nb][i], c, 0, 0, 0);
                     }
         }
         // the other set (stored one step ago) stays live across these products, so the two
@@ -169,19 +171,19 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
             for (int nb = 0; nb < NBLK; ++nb)
                 asm volatile("" ::"a"(other[mb][nb][0]));
         __builtin_amdgcn_sched_barrier(0);
-        // C[V column][item]: the lane holds item r32, V columns (r / 4) 8 + half 4 + r % 4
+        // C[item][V column]: the lane holds V column r32 of items (r / 4) 8 + half 4 + r % 4, so
+        // each store writes two items' whole 128-byte runs of 32 columns (whole cache lines)
         const auto vrs = buf(v + ((size_t)j * a.L + mp) * vrow, (long long)vrow * 4);
         // the stores read the accumulators themselves: their registers are written again only
         // by the products two steps on
 #pragma unroll
         for (int nb = 0; nb < NBLK; ++nb)
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int r = 0; r < 16; ++r)
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
                     const floatx16& x = acc[mb][nb][0];
-                    const floatx4 q4 = {x[4 * g], x[4 * g + 1], x[4 * g + 2], x[4 * g + 3]};
-                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, q4), vrs, (int)loff[mb], (int)((nb * 32 + g * 8) * 4), 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x[r]), vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w + nb * 32) * 4), 0);
                 }
         stage_store(nxt);
         __syncthreads();
