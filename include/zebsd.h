@@ -52,7 +52,8 @@ typedef struct zb_frame {
 typedef struct zb_band {
     int32_t L;
     int32_t n;
-    const float* ring;    /* [n][2]: cos(theta), quadrature weight */
+    const float* ring;    /* [n][2]: cos(theta), quadrature weight; Gauss-Legendre, so ring n - 1 - y mirrors ring y
+                             (cos theta negated, the same weight), which the sampling relies on */
     const float* wplm;    /* [n/2][L(L+1)/2]: w_y P_l^m(cos theta_y) 2 pi / (2n) */
     const float* dq;      /* [L][L][L]: d^l_{k,m}(pi/2) at (m L + l) L + k */
     const float* twiddle; /* [1024][2]: exp(-2 pi i t / 1024) */

@@ -25,21 +25,43 @@ struct SphArg {
     int L, n, log2nphi, pairs, flags;
 };
 
-// Grid point (ring y, point j) of an item: (pattern value, window) (sphere.wgsl:67-101).
-__device__ __forceinline__ float2 sample_point(const SphArg& a, const float* __restrict__ ring, const float2* __restrict__ tw,
-                                               const float* __restrict__ pats, float4 it, int y, int j) {
-    const float ct = ring[2 * y];
-    const float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
-    const float2 e = tw[j << (10 - a.log2nphi)];  // exp(-i phi_j)
-    const float3 d = make_float3(st * e.x, -st * e.y, ct);
-    const float nd = a.c2[0] * d.x + a.c2[1] * d.y + a.c2[2] * d.z;
+// The sample of an item along grid direction d (sphere.wgsl:67-101): the ray meets the detector
+// plane at h = (c2.t / c2.d) d - t, so row = c0.h and col = c1.h. Given nd = c2.d, p0 = c0.d and
+// p1 = c1.d, and the item's t0 = c0.t, t1 = c1.t, t2 = c2.t: row = s p0 - t0, col = s p1 - t1 with
+// s = t2 / nd. Returns (pattern value, window): the bilinear value where the ray lands on the
+// detector and inside the mask, else (0, 0).
+struct ItemGeom {
+    float t0, t1, t2;
+    const float* pat;
+};
+
+__device__ __forceinline__ ItemGeom item_geom(const SphArg& a, const float* __restrict__ pats, float4 it) {
+    ItemGeom g;
+    g.t0 = a.c0[0] * it.x + a.c0[1] * it.y + a.c0[2] * it.z;
+    g.t1 = a.c1[0] * it.x + a.c1[1] * it.y + a.c1[2] * it.z;
+    g.t2 = a.c2[0] * it.x + a.c2[1] * it.y + a.c2[2] * it.z;
+    g.pat = pats + (size_t)(int)it.w * a.h * a.w;
+    return g;
+}
+
+// A grid column (angle phi): the frame's rows dotted with the horizontal direction (cos phi,
+// sin phi, 0), so that for a ring (cos theta = ct, sin theta = st) c_k.d = st q_k + c_k.z ct.
+struct ColGeom {
+    float q0, q1, q2;
+};
+
+__device__ __forceinline__ ColGeom col_geom(const SphArg& a, float2 e) {  // e = exp(-i phi)
+    const float cx = e.x, sy = -e.y;
+    return {a.c0[0] * cx + a.c0[1] * sy, a.c1[0] * cx + a.c1[1] * sy, a.c2[0] * cx + a.c2[1] * sy};
+}
+
+__device__ __forceinline__ float2 sample_dir(const SphArg& a, const ItemGeom& g, float nd, float p0, float p1) {
     float val = 0.f, w = 0.f;
     if (nd > 1e-4f) {
         // (the hardware reciprocal, 1 ulp: the ray parameter of a sample, not a decision)
-        const float s = (a.c2[0] * it.x + a.c2[1] * it.y + a.c2[2] * it.z) * __builtin_amdgcn_rcpf(nd);
-        const float hx = s * d.x - it.x, hy = s * d.y - it.y, hz = s * d.z - it.z;
-        const float row = a.c0[0] * hx + a.c0[1] * hy + a.c0[2] * hz;
-        const float col = a.c1[0] * hx + a.c1[1] * hy + a.c1[2] * hz;
+        const float s = g.t2 * __builtin_amdgcn_rcpf(nd);
+        const float row = s * p0 - g.t0;
+        const float col = s * p1 - g.t1;
         const float dr = row - a.mask[0], dc = col - a.mask[1];
         if (row >= 0.f && row <= (float)(a.h - 1) && col >= 0.f && col <= (float)(a.w - 1) &&
             (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2])) {
@@ -48,7 +70,7 @@ __device__ __forceinline__ float2 sample_point(const SphArg& a, const float* __r
             if ((a.flags & ZB_FLAG_WINDOW) == 0) {
                 const int r0 = min((int)row, a.h - 2), q0 = min((int)col, a.w - 2);
                 const float fr = row - (float)r0, fc = col - (float)q0;
-                const float* p = pats + (size_t)(int)it.w * a.h * a.w + r0 * a.w + q0;
+                const float* p = g.pat + r0 * a.w + q0;
                 const float top = lerpf(p[0], p[1], fc);
                 const float bot = lerpf(p[a.w], p[a.w + 1], fc);
                 val = lerpf(top, bot, fr);
@@ -58,28 +80,44 @@ __device__ __forceinline__ float2 sample_point(const SphArg& a, const float* __r
     return make_float2(val, w);
 }
 
-// Per item: sum of w wq value and of w wq over its whole grid (sphere.wgsl:114-151).
+// The samples of an item at grid column cg on ring y (cos theta = ct) and on its mirror ring
+// n - 1 - y (cos theta = -ct, the same sine).
+__device__ __forceinline__ void sample_pair(const SphArg& a, const ItemGeom& g, const ColGeom& cg, float ct, float2& north, float2& south) {
+    const float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+    const float h0 = st * cg.q0, h1 = st * cg.q1, h2 = st * cg.q2;
+    const float z0 = a.c0[2] * ct, z1 = a.c1[2] * ct, z2 = a.c2[2] * ct;
+    north = sample_dir(a, g, h2 + z2, h0 + z0, h1 + z1);
+    south = sample_dir(a, g, h2 - z2, h0 - z0, h1 - z1);
+}
+
+// Per item: sum of w wq value and of w wq over its whole grid (sphere.wgsl:114-151). Each thread
+// keeps grid columns (phi) and walks the ring pairs (y, n - 1 - y), whose weights are equal.
 __global__ void __launch_bounds__(256) k_sums(SphArg a, const float* __restrict__ ring, const float2* __restrict__ tw,
                                               const float* __restrict__ pats, const float4* __restrict__ items, float2* __restrict__ sums) {
     __shared__ float2 red[4];
     const int b = blockIdx.x;
-    const float4 it = items[b];
+    const ItemGeom g = item_geom(a, pats, items[b]);
     const int nphi = 1 << a.log2nphi;
-    const int total = a.n * nphi;
+    const int nh = a.n >> 1;
+    const int cols = nphi < 256 ? nphi : 256;   // columns a pass covers
+    const int rows = 256 / cols;                // ring pairs a pass covers
+    const int tid = threadIdx.x;
     float sv = 0.f, sw = 0.f;
-    for (int p = threadIdx.x; p < total; p += 256) {
-        const int y = p >> a.log2nphi;
-        const int j = p & (nphi - 1);
-        const float2 g = sample_point(a, ring, tw, pats, it, y, j);
-        const float wq = ring[2 * y + 1];
-        sv += g.x * g.y * wq;
-        sw += g.y * wq;
+    for (int j = tid % cols; j < nphi; j += cols) {
+        const ColGeom cg = col_geom(a, tw[j << (10 - a.log2nphi)]);
+        for (int y = tid / cols; y < nh; y += rows) {
+            float2 gn, gs;
+            sample_pair(a, g, cg, ring[2 * y], gn, gs);
+            const float wq = ring[2 * y + 1];
+            sv += (gn.x * gn.y + gs.x * gs.y) * wq;
+            sw += (gn.y + gs.y) * wq;
+        }
     }
     sv = warp_sum(sv);
     sw = warp_sum(sw);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_float2(sv, sw);
+    if ((tid & 63) == 0) red[tid >> 6] = make_float2(sv, sw);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         float2 t = red[0];
         for (int i = 1; i < 4; ++i) t = make_float2(t.x + red[i].x, t.y + red[i].y);
         sums[b] = t;
@@ -100,15 +138,18 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
     const int b = blockIdx.x / groups;
     const int y0 = (blockIdx.x - b * groups) * G;
     const int nphi = 1 << a.log2nphi;
-    const float4 it = items[b];
+    const ItemGeom g = item_geom(a, pats, items[b]);
     const float mean = window_mean(a, sums[b]);
-    for (int e = threadIdx.x; e < G * nphi; e += 256) {
-        const int i = e >> a.log2nphi;
-        const int j = e & (nphi - 1);
-        const int y = y0 + i;
-        const float2 gn = sample_point(a, ring, tw, pats, it, y, j);
-        const float2 gs = sample_point(a, ring, tw, pats, it, a.n - 1 - y, j);
-        buf[e] = make_float2((gn.x - mean) * gn.y, (gs.x - mean) * gs.y);
+    // each thread keeps grid columns and walks the group's rings
+    const int cols = nphi < 256 ? nphi : 256;
+    const int rows = 256 / cols;
+    for (int j = threadIdx.x % cols; j < nphi; j += cols) {
+        const ColGeom cg = col_geom(a, tw[j << (10 - a.log2nphi)]);
+        for (int i = threadIdx.x / cols; i < G; i += rows) {
+            float2 gn, gs;
+            sample_pair(a, g, cg, ring[2 * (y0 + i)], gn, gs);
+            buf[i * nphi + j] = make_float2((gn.x - mean) * gn.y, (gs.x - mean) * gs.y);
+        }
     }
     __syncthreads();
     fft_dif<256>(buf, a.log2nphi, G, nphi, 1, tw, threadIdx.x);
