@@ -23,9 +23,56 @@ __device__ __forceinline__ bool in_mask(float row, float col, const float* m) {
 
 struct MaskArg { float m[3]; };
 
-// One block of 256 per pattern: mean and norm under the mask in two passes (prep.wgsl:405-443).
-template <typename T>
+// One block of 256 per pattern: mean and norm under the mask (prep.wgsl:405-443). The pattern
+// is read once into registers (PER pixels a thread, PER * 256 >= h w), written once.
+template <typename T, int PER>
 __global__ void __launch_bounds__(256) k_normalize(const T* __restrict__ raw, int h, int w, MaskArg mask, float* __restrict__ out) {
+    __shared__ float red[4];
+    const int b = blockIdx.x;
+    const int hw = h * w;
+    const T* src = raw + (size_t)b * hw;
+    float* dst = out + (size_t)b * hw;
+    const float inv_w = 1.f / (float)w;
+    float v[PER];
+    float s = 0.f, c = 0.f;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        v[k] = 0.f;
+        if (i < hw) {
+            // row and column without an integer division (exact for h w <= 65536)
+            const int row = (int)(((float)i + 0.5f) * inv_w);
+            if (in_mask((float)row, (float)(i - row * w), mask.m)) {
+                v[k] = (float)src[i];
+                s += v[k];
+                c += 1.f;
+            }
+        }
+    }
+    s = block_sum<256>(s, red);
+    c = block_sum<256>(c, red);
+    const float mean = s / fmaxf(c, 1.f);
+    float ss = 0.f;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        const int row = (int)(((float)i + 0.5f) * inv_w);
+        const bool in = i < hw && in_mask((float)row, (float)(i - row * w), mask.m);
+        v[k] = in ? v[k] - mean : 0.f;
+        ss += v[k] * v[k];
+    }
+    ss = block_sum<256>(ss, red);
+    const float inv = 1.f / sqrtf(fmaxf(ss, 1e-30f));
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        if (i < hw) dst[i] = v[k] * inv;
+    }
+}
+
+// The same for frames of more than 16384 pixels, in passes over memory.
+template <typename T>
+__global__ void __launch_bounds__(256) k_normalize_big(const T* __restrict__ raw, int h, int w, MaskArg mask, float* __restrict__ out) {
     __shared__ float red[4];
     const int b = blockIdx.x;
     const int hw = h * w;
@@ -151,12 +198,23 @@ extern "C" int32_t zb_normalize(const void* raw, int32_t pixel, int32_t n, int32
     m.m[1] = mask[1];
     m.m[2] = mask[2];
     hipStream_t s = (hipStream_t)stream;
+    const long hw = (long)h * w;
+#define ZB_NORM(T)                                                                                                            \
+    do {                                                                                                                      \
+        if (hw <= 4096)                                                                                                       \
+            ZB_LAUNCH("k_normalize", (k_normalize<T, 16>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);          \
+        else if (hw <= 16384)                                                                                                 \
+            ZB_LAUNCH("k_normalize", (k_normalize<T, 64>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);          \
+        else                                                                                                                  \
+            ZB_LAUNCH("k_normalize", k_normalize_big<T>, dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);            \
+    } while (0)
     if (pixel == 0)
-        ZB_LAUNCH("k_normalize", k_normalize<uint8_t>, dim3(n), dim3(256), 0, s, (const uint8_t*)raw, h, w, m, out);
+        ZB_NORM(uint8_t);
     else if (pixel == 1)
-        ZB_LAUNCH("k_normalize", k_normalize<uint16_t>, dim3(n), dim3(256), 0, s, (const uint16_t*)raw, h, w, m, out);
+        ZB_NORM(uint16_t);
     else
-        ZB_LAUNCH("k_normalize", k_normalize<float>, dim3(n), dim3(256), 0, s, (const float*)raw, h, w, m, out);
+        ZB_NORM(float);
+#undef ZB_NORM
     return check_launch("zb_normalize");
 }
 
