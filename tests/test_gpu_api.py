@@ -46,6 +46,31 @@ def test_toy_scan_is_recovered_in_the_fundamental_zone():
     assert s.settings["bandwidth"] == 64
 
 
+def test_two_sweeps_accept_most_at_the_first_bandwidth():
+    """sweeps = 2: the first pass at sweep_bandwidth (32) accepts the well-indexed cells, the
+    second (at 64) takes only the rest (src/engine.zig:2426-2465); every cell keeps its best
+    match, and the map is recovered as well as by one sweep at 64."""
+    s = _solver()
+    s.add_phase(NI)
+    truth = s.synthesize(toy_rows=8, toy_cols=12, toy_grains=96, toy_counts=400, toy_seed=7)
+    tq = truth["quaternions"].reshape(-1, 4)
+    s.configure(sweeps=2, sweep_bandwidth=32, bandwidth=64)
+    seen = []
+    r = s.index(progress=lambda st: seen.append((st["sweep"], st["bandwidth"], st["total"])))
+    assert all(bw == (32 if sw == 0 else 64) for sw, bw, _ in seen), seen
+    sweep = r["sweep"].reshape(-1)
+    assert set(np.unique(sweep)) <= {-1, 0, 1}
+    assert (sweep == 0).sum() >= 0.8 * 96, np.bincount(sweep + 1)
+    if (sweep == 1).any():
+        # the second sweep visited only what the first left
+        assert seen[-1][2] == 96 - (sweep == 0).sum()
+    d = np.degrees(s.disorientation(r["quaternions"].reshape(-1, 4), tq))
+    assert np.median(d) < 0.15 and d.max() < 1.0, (np.median(d), d.max())
+    assert (r["phase"] == 0).all()
+    st = s.index_status
+    assert st["sweeps"] == 2 and st["region"] == 96
+
+
 def test_noise_is_not_accepted_and_patterns_are():
     s = _solver()
     s.add_phase(NI)

@@ -257,6 +257,24 @@ def test_settings_reject_unknown_and_out_of_range_atomically():
         s.apply(static_background="average")
 
 
+def test_sweeps_climb_to_the_final_bandwidth():
+    """settings.zig:511-519: three sweeps from 32 to 128 run at 32, 64, 128; one sweep at the
+    bandwidth; the count and first bandwidth are range-checked."""
+    s = Settings()
+    s.apply(sweeps=3, sweep_bandwidth=32, bandwidth=128)
+    assert [s.sweep_bandwidth_of(i) for i in range(3)] == [32, 64, 128]
+    s.apply(sweeps=2, sweep_bandwidth=32, bandwidth=64)
+    assert [s.sweep_bandwidth_of(i) for i in range(2)] == [32, 64]
+    s.apply(sweeps=4, sweep_bandwidth=32, bandwidth=64)
+    # equal ratios: 32 * 2^(1/3) = 40.3, 32 * 2^(2/3) = 50.8
+    assert [s.sweep_bandwidth_of(i) for i in range(4)] == [32, 40, 51, 64]
+    s.apply(sweeps=1)
+    assert s.sweep_bandwidth_of(0) == 64
+    for bad in ({"sweeps": 0}, {"sweeps": 7}, {"sweep_bandwidth": 4}, {"sweep_method": "pca"}):
+        with pytest.raises(SettingError):
+            s.apply(**bad)
+
+
 def test_rotation_conversions_round_trip():
     rng = np.random.default_rng(1)
     q = rot.qstd(rng.normal(size=(100, 4)))

@@ -278,8 +278,7 @@ class Solver:
         return {"quaternions": rec[..., :4].copy(), "ncc": rec[..., 4].copy(), "nmi": rec[..., 5].copy(), "step": rec[..., 6].copy(), "z": z}
 
     def _sweep_bandwidth(self, i):
-        s = self.settings
-        return s["bandwidth"]
+        return self._e.settings.sweep_bandwidth_of(i)
 
     def index_begin(self):
         if not self._e.phases:
@@ -292,9 +291,12 @@ class Solver:
         self._todo = np.arange(n, dtype=np.int64)
         self._done = 0
         self._accepted = 0
+        self._sweep = 0
 
     def index_step(self):
-        """Index the next few batches; returns the cells left (0: done) (src/engine.zig:2426-2465)."""
+        """Index the next few batches of the sweep in progress; when the sweep ends, the cells
+        without an accepted match become the next sweep's, at its bandwidth. Returns the cells
+        left to visit (0: done) (src/engine.zig:2426-2465)."""
         if self._res is None:
             self.index_begin()
         left = len(self._todo) - self._done
@@ -304,9 +306,16 @@ class Solver:
         n = min(left, max(s["batch"], 1) * max(2, min(16, 4096 // max(s["batch"], 1))) if s["batch"] < 4096 else left)
         n = max(n, min(left, 16384))
         cells = self._todo[self._done:self._done + n]
-        out = self.index_cells(cells)
+        out = self.index_cells(cells, bandwidth=self._sweep_bandwidth(self._sweep))
         self._land(cells, out)
         self._done += n
+        if self._done == len(self._todo) and self._sweep + 1 < s["sweeps"]:
+            # the next sweep takes what this one left
+            keep = self._todo[self._res["sweep"][self._todo] < 0]
+            if len(keep):
+                self._todo = keep
+                self._done = 0
+                self._sweep += 1
         return len(self._todo) - self._done
 
     def _land(self, cells, out):
@@ -340,14 +349,17 @@ class Solver:
             res["settled"][c] = step[sel] <= settle
         kept_good = res["nmi"][cells] >= s["accept_nmi"] if s["accept_nmi"] > 0 else res["z"][cells] >= s["accept_z"]
         newly = (res["sweep"][cells] < 0) & kept_good & res["settled"][cells]
-        res["sweep"][cells[newly]] = 0
+        res["sweep"][cells[newly]] = self._sweep
         self._accepted += int(newly.sum())
 
     @property
     def index_status(self):
         total = 0 if self._res is None else len(self._todo)
-        return {"sweep": 0, "sweeps": 1, "bandwidth": self.settings["bandwidth"], "dict_grid": 0, "done": getattr(self, "_done", 0),
-                "total": total, "region": total, "accepted": getattr(self, "_accepted", 0), "method": "spherical"}
+        region = 0 if self._res is None else len(self._res["phase"])
+        sw = getattr(self, "_sweep", 0)
+        return {"sweep": sw, "sweeps": self.settings["sweeps"], "bandwidth": self._sweep_bandwidth(sw), "dict_grid": 0,
+                "done": getattr(self, "_done", 0), "total": total, "region": region, "accepted": getattr(self, "_accepted", 0),
+                "method": "spherical"}
 
     def index(self, progress=None):
         """Index the whole map a few batches at a time; progress(status) after each step, False

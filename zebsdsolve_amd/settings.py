@@ -2,10 +2,11 @@
 
 `apply` rejects unknown keys and values out of range atomically, as Settings.apply/check does
 (src/settings.zig:404-463). Keys of the reference that belong to stages outside this build
-(pattern filters, dictionary and PCA search, binning, sweeps, the fits) are accepted only at the
-values that leave those stages out, and say so otherwise.
+(pattern filters, dictionary and PCA search, binning, the fits) are accepted only at the values
+that leave those stages out, and say so otherwise.
 """
 import copy
+import math
 
 DEFAULTS = {
     # ── a toy scan (synthesize) ──
@@ -31,14 +32,15 @@ DEFAULTS = {
     # ── how the work is cut up ──
     "batch": 256, "tile_mb": 256, "refine_pixels": 64, "wg_fft": 64, "wg_refine": 64,
     # ── sweeps and acceptance ──
-    "sweeps": 1, "accept_nmi": 0.0, "accept_z": 15.0, "accept_step_deg": 0.2,
+    "sweeps": 1, "sweep_bandwidth": 32, "sweep_method": "same", "accept_nmi": 0.0, "accept_z": 15.0, "accept_step_deg": 0.2,
 }
 
 # values a key outside this build may take (the reference's defaults differ for the filters:
 # static_background "average" and dynamic_background "subtract" there)
 _ONLY = {
     "nlpar": (False,), "static_background": ("none",), "dynamic_background": ("none",), "clahe": (False,),
-    "bandpass": (False,), "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",), "sweeps": (1,),
+    "bandpass": (False,), "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",),
+    "sweep_method": ("same", "spherical"),
 }
 
 
@@ -66,6 +68,8 @@ def check(s):
     bad(not (0 <= s["toy_counts"] <= 1e6), "toy_counts")
     bad(abs(s["scan_x_sign"]) != 1 or abs(s["scan_y_sign"]) != 1, "scan_x_sign")
     bad(not (0 <= s["accept_nmi"] <= 1) or not (s["accept_step_deg"] > 0), "accept_nmi")
+    bad(not (1 <= s["sweeps"] <= 6), "sweeps")
+    bad(not (8 <= s["sweep_bandwidth"] <= 256), "sweep_bandwidth")
     for k in ("wg_fft", "wg_refine"):
         bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
     for k, allowed in _ONLY.items():
@@ -104,6 +108,16 @@ class Settings:
 
     def __getitem__(self, k):
         return self.values[k]
+
+    def sweep_bandwidth_of(self, i):
+        """Bandwidth of sweep i of `sweeps`: sweep_bandwidth first, `bandwidth` last, equal ratios
+        between, rounded (src/settings.zig:463-483)."""
+        s = self.values
+        if s["sweeps"] <= 1 or i + 1 >= s["sweeps"]:
+            return s["bandwidth"]
+        f = i / (s["sweeps"] - 1)
+        a, b = float(s["sweep_bandwidth"]), float(s["bandwidth"])
+        return int(math.floor(a * (b / a) ** f + 0.5))
 
     def as_dict(self):
         return dict(self.values)
