@@ -227,6 +227,8 @@ def main():
     from zebsdsolve_amd.geometry import Geometry
 
     e = engine.Engine()
+    # the workload is the path alone: no pattern filters before it
+    e.settings.apply(static_background="none", dynamic_background="none")
     e.geo = Geometry(pc=PC, sample_tilt_deg=70.0)
     e.add_phase(MASTER)
     e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters)

@@ -200,6 +200,27 @@ int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, co
 int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
                 int32_t flags, float* sums, void* stream);
 
+/* The pattern filters before the path. background: [h][w] f32 subtracted from every pattern
+ * (the static background, e.g. the mean of patterns over the map), or NULL; dynamic: 0 none,
+ * 1 subtract, 2 divide by a Gaussian blur of each pattern of deviation dynamic_sigma pixels
+ * (the reference's dynamic_sigma setting times the pattern width). */
+typedef struct zb_prep_opts {
+    const float* background;
+    int32_t dynamic;
+    float dynamic_sigma;
+} zb_prep_opts;
+
+/* zb_normalize after the filters, fused: raw patterns in, the static background subtracted,
+ * the dynamic background subtracted or divided by, zero mean and unit norm under the mask out.
+ * The blur is the reference's: direct separable taps below 12 pixels of deviation, sums over
+ * boxes from 12. One block per pattern with the image in LDS: fails (ZB_EINVAL) for patterns
+ * whose image and blur buffers exceed the device's LDS per block.
+ * Replaces Prep.run + Prep.filter (src/prep.zig:143-204; prep.wgsl load, background, blur_rows,
+ * blur_cols, box_sum, coarse_rows, coarse_cols, spread_rows, spread_cols, combine, normalize)
+ * with static and dynamic backgrounds; CLAHE and the band pass are not built. */
+int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* opts,
+                float* out, void* stream);
+
 /* Simulated patterns: the master (bilinear, [2][side][side] f32) in direction q . ray of every
  * pixel. poses: [n][8] = (t, pattern index, q); out: [n][h][w].
  * Replaces Detector.project (src/detector.zig:137-142: detector.wgsl project). */

@@ -153,6 +153,121 @@ def normalize(pat, mask):
     return v / np.sqrt(max((v * v).sum(), 1e-30))
 
 
+# ── pattern filters before the path (prep.wgsl:72-287; src/prep.zig:105-204) ──
+
+def lattice(rows, cols, want):
+    """Cells spread over the map, row-major (src/engine.zig:1325-1341)."""
+    gy, gx = rows, cols
+    if rows * cols > want:
+        ratio = rows / cols
+        gy = int(min(max(int(np.floor(np.sqrt(want * ratio) + 0.5)), 1), rows))
+        gx = int(min(max(want // gy, 1), cols))
+    return np.array([((2 * r + 1) * rows // (2 * gy)) * cols + (2 * c + 1) * cols // (2 * gx) for r in range(gy) for c in range(gx)],
+                    np.int64)
+
+
+def static_average(patterns, rows, cols, want):
+    """The mean pattern over the lattice cells (src/engine.zig:1344-1385), float64 summed."""
+    cells = lattice(rows, cols, want)
+    flat = np.asarray(patterns).reshape(rows * cols, -1).astype(np.float64)
+    return flat[cells].mean(0).reshape(np.asarray(patterns).shape[-2:])
+
+
+def _gauss_rows_direct(img, sigma):
+    """blur_rows: taps exp(-d^2 / 2 sigma^2) out to max(1, ceil(3 sigma)), renormalized to the
+    taps inside the image (prep.wgsl:92-135)."""
+    h, w = img.shape
+    r = max(1, int(np.ceil(3 * sigma)))
+    k = 1.0 / (2 * sigma * sigma)
+    out = np.empty_like(img)
+    for c in range(w):
+        x = np.arange(max(c - r, 0), min(c + r, w - 1) + 1)
+        g = np.exp(-((x - c) ** 2) * k)
+        out[:, c] = (img[:, x] * g).sum(1) / g.sum()
+    return out
+
+
+def blur_direct(img, sigma):
+    """The separable Gaussian of the narrow path: rows, then columns."""
+    return _gauss_rows_direct(_gauss_rows_direct(np.asarray(img, np.float64), sigma).T, sigma).T
+
+
+def _cell(j, n, s):
+    """Centre (pixels) and width of coarse cell j along an axis of n pixels (prep.wgsl:179-185)."""
+    lo = j * s
+    hi = lo + s
+    if j >= 0 and lo < n:
+        hi = min(hi, n)
+    return 0.5 * (lo + hi - 1), float(hi - lo)
+
+
+def blur_box(img, sigma):
+    """The wide path (sigma >= 12 pixels, prep.wgsl:166-287; src/prep.zig:127-146): pixel sums over
+    s x s boxes (s = floor(sigma / 6)) with their counts, a Gaussian over the coarse cells (with a
+    margin of 3 whole cells outside the image), spread back over the pixels by a Gaussian of
+    deviation s weighted by the cells' widths; sums and counts divided at the end."""
+    img = np.asarray(img, np.float64)
+    h, w = img.shape
+    s = int(np.floor(sigma / 6))
+    cs = np.sqrt(sigma * sigma - s * s - (s * s - 1) / 12.0)
+    pad = 3
+    R = float(np.ceil(4 * cs))
+    k0 = 1.0 / (2 * cs * cs)
+    k2 = 1.0 / (2 * s * s)
+    hs, ws = -(-h // s), -(-w // s)
+    box = np.zeros((hs, ws, 2))
+    for i in range(hs):
+        for j in range(ws):
+            blk = img[i * s:min((i + 1) * s, h), j * s:min((j + 1) * s, w)]
+            box[i, j] = (blk.sum(), blk.size)
+    wp, hp = ws + 2 * pad, hs + 2 * pad
+
+    def coarse(cin, n, m_out):
+        # cin [lines][m_in] along an axis of n pixels -> [lines][m_out] over cells -pad .. m_out - pad - 1
+        m_in = cin.shape[1]
+        out = np.zeros((cin.shape[0], m_out, 2))
+        for t in range(m_out):
+            c, _ = _cell(t - pad, n, s)
+            j0 = max(int(np.floor((c - R) / s)), 0)
+            j1 = min(int(np.floor((c + R) / s)), m_in - 1)
+            for j in range(j0, j1 + 1):
+                d = _cell(j, n, s)[0] - c
+                out[:, t] += np.exp(-d * d * k0) * cin[:, j]
+        return out
+
+    rows_ = coarse(box, w, wp)                                        # [hs][wp]
+    cols_ = coarse(rows_.transpose(1, 0, 2), h, hp).transpose(1, 0, 2)  # [hp][wp]
+
+    def spread(cin, n):
+        # cin [lines][m] over cells -pad ..; -> [lines][n] pixels
+        out = np.zeros((cin.shape[0], n, 2))
+        for x in range(n):
+            own = x // s
+            for t in range(2 * pad + 1):
+                cx, cwid = _cell(own + t - pad, n, s)
+                d = x - cx
+                out[:, x] += cwid * np.exp(-d * d * k2) * cin[:, own + t]
+        return out
+
+    sr = spread(cols_, w)                                             # [hp][w]
+    sc = spread(sr.transpose(1, 0, 2), h).transpose(1, 0, 2)          # [h][w]
+    return sc[..., 0] / sc[..., 1]
+
+
+def prep(raw, mask, background=None, dynamic="none", dynamic_sigma=0.15):
+    """The filter chain then normalization (src/prep.zig:148-204): the static background
+    subtracted, the dynamic one (a Gaussian of dynamic_sigma x the width, the wide path from 12
+    pixels) subtracted or divided by, then zero mean and unit norm under the mask."""
+    img = np.asarray(raw, np.float64).copy()
+    if background is not None:
+        img = img - np.asarray(background, np.float64)
+    if dynamic != "none":
+        sigma = dynamic_sigma * img.shape[1]
+        bl = blur_box(img, sigma) if int(np.floor(sigma / 6)) >= 2 else blur_direct(img, sigma)
+        img = img - bl if dynamic == "subtract" else img / np.maximum(bl, 1e-6)
+    return normalize(img, mask)
+
+
 def project(sheets, side, fr, t, q):
     """Simulated pattern: the master at q . normalize(ray) (detector.wgsl:254-263)."""
     h, w = fr["h"], fr["w"]

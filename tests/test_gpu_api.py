@@ -14,6 +14,8 @@ def _solver():
     import zebsdsolve_amd as ze
     s = ze.Solver()
     s.set_geometry(pc=(0.44, 0.24, 0.53), sample_tilt_deg=70, detector_tilt_deg=0, px_um=70 * 8, step_x_um=20, step_y_um=20)
+    # as the reference's test (tests/test_api.py:199): the synthetic scans without filters
+    s.configure(static_background="none", dynamic_background="none")
     return s
 
 

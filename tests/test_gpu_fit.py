@@ -21,6 +21,7 @@ def fitcase():
     from zebsdsolve_amd import engine, fit
     from zebsdsolve_amd.geometry import Geometry
     e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
     e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
     e.add_phase(NI)
     e.set_patterns(np.zeros((1, N, 60, 60), np.float32))

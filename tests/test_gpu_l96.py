@@ -24,6 +24,7 @@ def test_l96_correlation_against_the_oracle_and_recovery():
     from zebsdsolve_amd.geometry import Geometry
     print("L=96 slab path", flush=True)
     e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
     e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
     e.add_phase(NI)
     e.settings.apply(bandwidth=L)

@@ -25,6 +25,7 @@ def case():
     from zebsdsolve_amd.geometry import Geometry
 
     e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
     e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
     e.add_phase(NI)
     e.set_patterns(np.zeros((1, N, 60, 60), np.float32))
@@ -208,6 +209,7 @@ def test_refinement_on_other_frames(case, h, w, mask_on):
     print(f"refine frame {h}x{w} mask={mask_on}", flush=True)
     n = 6
     e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
     e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
     e.add_phase(NI)
     e.settings.apply(mask=mask_on)

@@ -23,6 +23,7 @@ def _setup(path, L, N):
     from zebsdsolve_amd.geometry import Geometry
 
     e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
     e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
     e.add_phase(path)
     e.settings.apply(bandwidth=L)
@@ -134,6 +135,7 @@ def test_vcorr_matches_slab_path_over_many_tiles(L, N):
     from zebsdsolve_amd.engine import stream_ptr
     print(f"vcorr tiles L={L} N={N}", flush=True)
     e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
     e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
     e.add_phase(NI)
     e.settings.apply(bandwidth=L)

@@ -254,7 +254,14 @@ def test_settings_reject_unknown_and_out_of_range_atomically():
         s.apply(bandwidth=4, refine_iters=9)
     assert s["refine_iters"] == 5
     with pytest.raises(SettingError):
-        s.apply(static_background="average")
+        s.apply(static_background="file")
+    with pytest.raises(SettingError):
+        s.apply(dynamic_background="gaussian")
+    with pytest.raises(SettingError):
+        s.apply(clahe=True)
+    # the reference's defaults for the filters (settings.zig:138-148)
+    assert Settings()["static_background"] == "average" and Settings()["dynamic_background"] == "subtract"
+    assert Settings()["static_patterns"] == 1024 and Settings()["dynamic_sigma"] == 0.15
 
 
 def test_sweeps_climb_to_the_final_bandwidth():

@@ -290,3 +290,66 @@ def test_oracle_recovers_known_orientations_end_to_end():
         assert np.degrees(sym.disorientation(q, truth[b], m.rotations)) < 3.0
         qr, ncc, _ = op.refine(grad, m.side, fr, pn, t, q, iters=8)
         assert np.degrees(sym.disorientation(qr, truth[b], m.rotations)) < 0.02 and ncc > 0.99
+
+
+# ── the pattern filters (prep.wgsl:92-287) ──
+
+def _smooth_image(h, w, seed=0):
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    img = ndimage.gaussian_filter(rng.normal(size=(h, w)), 3) * 50 + 400
+    return img + np.linspace(0, 30, w)[None, :] + np.linspace(0, 10, h)[:, None]
+
+
+def test_direct_blur_is_the_truncated_gaussian():
+    """Inside the image, the separable taps out to ceil(3 sigma) are scipy's Gaussian truncated at
+    the same radius; a constant image stays constant up to the edges (the renormalization)."""
+    from scipy import ndimage
+    img = _smooth_image(60, 70)
+    for sigma in (1.7, 4.3, 9.0):
+        r = max(1, int(np.ceil(3 * sigma)))
+        ref = ndimage.gaussian_filter(img, sigma, mode="constant", truncate=(r - 0.5 + 1e-9) / sigma)
+        got = op.blur_direct(img, sigma)
+        assert np.abs(got[r:-r, r:-r] - ref[r:-r, r:-r]).max() < 1e-9
+        assert np.allclose(op.blur_direct(np.full((30, 40), 7.0), sigma), 7.0, rtol=0, atol=1e-12)
+
+
+def test_box_blur_stays_within_a_fraction_of_a_percent_of_the_gaussian():
+    """The wide path (boxes of floor(sigma / 6) pixels, a coarse Gaussian, a spread) against the
+    Gaussian carried to 6 sigma, renormalized at the edges: the reference's stated 0.2 % rms
+    (prep.wgsl:166-176) on smooth images; a constant stays constant."""
+    from scipy import ndimage
+    for h, w, sigma in ((80, 80, 12.0), (120, 120, 18.0), (90, 70, 13.5)):
+        img = _smooth_image(h, w, seed=h)
+        ref = (ndimage.gaussian_filter(img, sigma, mode="constant", truncate=6) /
+               ndimage.gaussian_filter(np.ones_like(img), sigma, mode="constant", truncate=6))
+        got = op.blur_box(img, sigma)
+        rms = np.sqrt(np.mean((got - ref) ** 2)) / img.std()
+        assert rms < 5e-3, (h, w, sigma, rms)
+        assert np.allclose(op.blur_box(np.full((h, w), 3.0), sigma), 3.0, atol=1e-9)
+
+
+def test_lattice_spreads_cells_over_the_map():
+    """engine.zig:1325-1341: every cell of a small map; about `want` distinct cells of a large one;
+    3 x 4 with 6 wanted: rows 0 and 2, columns 0, 2, 3 (by hand)."""
+    assert np.array_equal(op.lattice(10, 10, 1024), np.arange(100))
+    assert np.array_equal(op.lattice(3, 4, 6), [0, 2, 3, 8, 10, 11])
+    for rows, cols, want in ((48, 64, 1024), (100, 300, 1024), (1, 5000, 256)):
+        c = op.lattice(rows, cols, want)
+        assert len(np.unique(c)) == len(c) <= want and c.min() >= 0 and c.max() < rows * cols
+        assert len(c) >= 0.9 * want
+    # a map far taller than wide: round(sqrt(50 x 700 / 3)) = 108 rows of one column each
+    c = op.lattice(700, 3, 50)
+    assert len(c) == 108 and np.all(c % 3 == 1)
+
+
+def test_prep_chain_order_and_identities():
+    """No filter: prep is the normalization; a static background equal to the pattern leaves
+    nothing (all zeros); a linear ramp is its own blur away from the edges (symmetric taps)."""
+    fr = op.frame(np.eye(3), 40, 50)
+    img = _smooth_image(40, 50)
+    assert np.allclose(op.prep(img, fr["mask"]), op.normalize(img, fr["mask"]))
+    assert np.allclose(op.prep(img, fr["mask"], background=img), 0.0)
+    ramp = np.add.outer(np.arange(40) * 0.7, np.arange(50) * 1.3) + 100
+    r = int(np.ceil(3 * 4.0))
+    assert np.abs(op.blur_direct(ramp, 4.0)[r:-r, r:-r] - ramp[r:-r, r:-r]).max() < 1e-9

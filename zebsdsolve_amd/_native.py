@@ -37,10 +37,15 @@ class RefineOpts(ctypes.Structure):
                 ("scale_max", ctypes.c_float), ("nmi_bins", i32), ("stamp", ctypes.c_uint32)]
 
 
+class PrepOpts(ctypes.Structure):  # zb_prep_opts
+    _fields_ = [("background", ctypes.c_void_p), ("dynamic", i32), ("dynamic_sigma", ctypes.c_float)]
+
+
 _SIGS = {
     "zb_version": ([], ctypes.c_char_p),
     "zb_last_error": ([ctypes.c_char_p, i32], i32),
     "zb_normalize": ([vp, i32, i32, i32, i32, f32p, vp, vp], i32),
+    "zb_prep": ([vp, i32, i32, i32, i32, f32p, ctypes.POINTER(PrepOpts), vp, vp], i32),
     "zb_sh_workspace": ([ctypes.POINTER(Band), i32], i64),
     "zb_sh_coefficients": ([ctypes.POINTER(Band), ctypes.POINTER(Frame), vp, vp, i32, i32, vp, i64, vp, vp, vp], i32),
     "zb_so3_workspace": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), i32], i64),

@@ -82,6 +82,18 @@ class DevicePhase:
         return self._vt
 
 
+def lattice(rows, cols, want):
+    """About `want` cells spread evenly over a rows x cols map, row-major (src/engine.zig:1325-1341):
+    every cell when the map has no more."""
+    gy, gx = rows, cols
+    if rows * cols > want:
+        gy = min(max(int(np.floor(np.sqrt(want * rows / cols) + 0.5)), 1), rows)
+        gx = min(max(want // gy, 1), cols)
+    r = np.arange(gy, dtype=np.int64)[:, None]
+    c = np.arange(gx, dtype=np.int64)[None, :]
+    return (((2 * r + 1) * rows // (2 * gy)) * cols + (2 * c + 1) * cols // (2 * gx)).reshape(-1)
+
+
 def mfma_correlation(L, n):
     """Whether the matrix-core correlation takes this bandwidth (its transform along a is held in
     registers: n <= 128)."""
@@ -130,6 +142,7 @@ class Engine:
         # "slab": the spectrum + slab transforms (every bandwidth, and the fallback above 128)
         self.correlator = "mfma"
         self._lanes = []
+        self._bg = None           # (key, static background on the device)
 
     # ── state ──
 
@@ -142,6 +155,7 @@ class Engine:
     def set_patterns(self, patterns):
         """(rows, cols, h, w) or (n, h, w) of uint8, uint16 or float32, as a numpy array or a
         device tensor (kept where it is)."""
+        self._bg = None
         if isinstance(patterns, torch.Tensor):
             a = patterns
             if a.dim() == 3:
@@ -350,6 +364,42 @@ class Engine:
         o.stamp = self.stamp
         return o
 
+    # ── the pattern filters before the path ──
+
+    def static_background(self):
+        """The static background: the mean pattern over a lattice of `static_patterns` cells of
+        the map, summed in float64 (src/engine.zig:1344-1385), on the device."""
+        rows, cols, h, w = self.shape
+        key = (id(self.patterns), self.shape, int(self.settings["static_patterns"]))
+        if self._bg is None or self._bg[0] != key:
+            cells = lattice(rows, cols, int(self.settings["static_patterns"]))
+            flat = self.patterns.reshape(rows * cols, h, w)
+            if isinstance(flat, torch.Tensor):
+                sel = flat[torch.as_tensor(cells, device=flat.device)].to(torch.float64)
+                tot = sel.sum(0)
+            else:
+                tot = torch.from_numpy(np.asarray(flat)[cells].astype(np.float64).sum(0))
+            avg = (tot * (1.0 / len(cells))).to(torch.float32).to(self._device()).contiguous()
+            self._bg = (key, avg)
+        return self._bg[1]
+
+    def prep_into(self, raw, n, h, w, out, st):
+        """Raw patterns [n, h, w] (device) to the normalized ones in `out`: the pattern filters the
+        settings ask for (zb_prep), else the normalization alone (zb_normalize)."""
+        lib = nat.load()
+        s = self.settings
+        mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
+        kind = TORCH_PIXEL_KIND[raw.dtype]
+        if s["static_background"] == "none" and s["dynamic_background"] == "none":
+            nat.check(lib.zb_normalize(raw.data_ptr(), kind, n, h, w, mask, out.data_ptr(), st), "normalize")
+            return
+        o = nat.PrepOpts()
+        bg = self.static_background() if s["static_background"] == "average" else None
+        o.background = bg.data_ptr() if bg is not None else None
+        o.dynamic = {"none": 0, "subtract": 1, "divide": 2}[s["dynamic_background"]]
+        o.dynamic_sigma = float(s["dynamic_sigma"]) * w
+        nat.check(lib.zb_prep(raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), out.data_ptr(), st), "prep")
+
     def run_batch(self, raw, items, L=None, workspace=None, lane=None):
         """The pipeline on device-resident inputs: raw [n, h, w] (uint8/uint16/float32 tensor) and
         items [n, 4] float32 (t, index into raw). Returns records [n_phases, n, 8] on the device
@@ -408,7 +458,7 @@ class Engine:
                 records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
             raw.record_stream(lane.stream)
             items.record_stream(lane.stream)
-        nat.check(lib.zb_normalize(raw.data_ptr(), kind, raw.shape[0], h, w, mask, pats.data_ptr(), st), "normalize")
+        self.prep_into(raw, raw.shape[0], h, w, pats, st)
         for c0 in range(0, n, chunk):
             nt = min(chunk, n - c0)
             it = items[c0:c0 + nt]
@@ -465,7 +515,7 @@ class Engine:
         pats = torch.empty((n, h, w), dtype=torch.float32, device=dev)
         mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
         st = stream_ptr()
-        nat.check(lib.zb_normalize(raw.data_ptr(), TORCH_PIXEL_KIND[raw.dtype], n, h, w, mask, pats.data_ptr(), st), "normalize")
+        self.prep_into(raw, n, h, w, pats, st)
         t = self.positions(cells)
         items = torch.from_numpy(np.concatenate([t, np.arange(n, dtype=np.float32)[:, None]], 1)).to(dev)
         seeds = torch.from_numpy(np.ascontiguousarray(np.asarray(quats, np.float32).reshape(n, 4))).to(dev)

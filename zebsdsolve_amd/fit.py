@@ -59,7 +59,8 @@ def left_jacobian(phi):
 
 class PoseField:
     """Phase pi's master against given normalized patterns: the sums at any poses. `pats` is a
-    device tensor [n, h, w] of patterns normalized under the engine's mask (zb_normalize)."""
+    device tensor [n, h, w] of patterns filtered and normalized as the engine's settings say
+    (Engine.prep_into)."""
 
     def __init__(self, engine, pi, pats):
         self.e = engine
@@ -72,16 +73,13 @@ class PoseField:
     @classmethod
     def for_cells(cls, engine, pi, cells):
         """The field of the patterns of scan cells, normalized on the device."""
-        from .engine import TORCH_PIXEL_KIND, stream_ptr
-        lib = nat.load()
+        from .engine import stream_ptr
         dev = engine._device()
         cells = np.asarray(cells, np.int64).reshape(-1)
         raw = engine._raw_for(cells)
         h, w = engine.pattern_hw()
         pats = torch.empty((len(cells), h, w), dtype=torch.float32, device=dev)
-        mask = (nat.ctypes.c_float * 3)(*engine.mask_tuple())
-        nat.check(lib.zb_normalize(raw.data_ptr(), TORCH_PIXEL_KIND[raw.dtype], len(cells), h, w, mask, pats.data_ptr(), stream_ptr()),
-                  "normalize")
+        engine.prep_into(raw, len(cells), h, w, pats, stream_ptr())
         return cls(engine, pi, pats)
 
     def sums(self, poses, flags=0):

@@ -2,8 +2,8 @@
 
 `apply` rejects unknown keys and values out of range atomically, as Settings.apply/check does
 (src/settings.zig:404-463). Keys of the reference that belong to stages outside this build
-(pattern filters, dictionary and PCA search, binning, the fits) are accepted only at the values
-that leave those stages out, and say so otherwise.
+(NLPAR, CLAHE, the band pass, dictionary and PCA search, binning, the fits) are accepted only at
+the values that leave those stages out, and say so otherwise.
 """
 import copy
 import math
@@ -17,8 +17,10 @@ DEFAULTS = {
     "scan_x_sign": -1, "scan_y_sign": -1,
     # ── resolution ──
     "binning": 1.0, "full_resolution": "none",
-    # ── averaging and pattern filters (outside this build: only their "off" values) ──
-    "nlpar": False, "static_background": "none", "dynamic_background": "none", "clahe": False, "bandpass": False,
+    # ── averaging and pattern filters, in the order applied (NLPAR, CLAHE and the band pass are
+    # outside this build: only their "off" values) ──
+    "nlpar": False, "static_background": "average", "static_patterns": 1024, "dynamic_background": "subtract",
+    "dynamic_sigma": 0.15, "clahe": False, "bandpass": False,
     # ── mask ──
     "mask": True, "mask_radius": 1.0,
     # ── master ──
@@ -35,10 +37,10 @@ DEFAULTS = {
     "sweeps": 1, "sweep_bandwidth": 32, "sweep_method": "same", "accept_nmi": 0.0, "accept_z": 15.0, "accept_step_deg": 0.2,
 }
 
-# values a key outside this build may take (the reference's defaults differ for the filters:
-# static_background "average" and dynamic_background "subtract" there)
+# values a key outside this build may take ("file" static backgrounds need a scan file, and the
+# readers are outside this build)
 _ONLY = {
-    "nlpar": (False,), "static_background": ("none",), "dynamic_background": ("none",), "clahe": (False,),
+    "nlpar": (False,), "static_background": ("average", "none"), "dynamic_background": ("subtract", "divide", "none"), "clahe": (False,),
     "bandpass": (False,), "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",),
     "sweep_method": ("same", "spherical"),
 }
@@ -69,6 +71,8 @@ def check(s):
     bad(abs(s["scan_x_sign"]) != 1 or abs(s["scan_y_sign"]) != 1, "scan_x_sign")
     bad(not (0 <= s["accept_nmi"] <= 1) or not (s["accept_step_deg"] > 0), "accept_nmi")
     bad(not (1 <= s["sweeps"] <= 6), "sweeps")
+    bad(not (s["static_patterns"] >= 1), "static_patterns")
+    bad(not (s["dynamic_sigma"] > 0), "dynamic_sigma")
     bad(not (8 <= s["sweep_bandwidth"] <= 256), "sweep_bandwidth")
     for k in ("wg_fft", "wg_refine"):
         bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
