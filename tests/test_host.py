@@ -254,7 +254,7 @@ def test_settings_reject_unknown_and_out_of_range_atomically():
         s.apply(bandwidth=4, refine_iters=9)
     assert s["refine_iters"] == 5
     with pytest.raises(SettingError):
-        s.apply(static_background="file")
+        s.apply(static_background="gaussian")
     with pytest.raises(SettingError):
         s.apply(dynamic_background="gaussian")
     with pytest.raises(SettingError):

@@ -115,11 +115,56 @@ class Solver:
     def set_patterns(self, patterns):
         """Patterns (rows, cols, h, w) or (n, h, w): uint8, uint16 or float32, a numpy array or a
         device tensor (kept on the device)."""
+        self._opened = None
+        self._e.file_background = None
         try:
             self._e.set_patterns(patterns)
         except ValueError as ex:
             raise Error(str(ex)) from None
         self._res = None
+        return self
+
+    def open(self, path):
+        """Open a scan file: kikuchipy's h5ebsd (src/scan.zig:640-700): the map of patterns, its
+        shape, steps and tilt, the pattern centre (the mean of the stored per-pattern ones), the
+        detector pixel (its size times the binning) and the stored static background (used when
+        `static_background` is "file"). Other formats are refused."""
+        from . import h5
+        try:
+            with h5.File(str(path)) as f:
+                root = f["/"]
+                scans = [k for k in root.keys() if k.startswith("Scan")]
+                if not scans or "EBSD" not in root[scans[0]]:
+                    raise Error(f"open: {path}: not an h5ebsd scan (no Scan */EBSD)")
+                ebsd = root[scans[0]]["EBSD"]
+                hdr, data = ebsd["Header"], ebsd["Data"]
+                if "pcx" not in hdr or "patterns" not in data:
+                    raise Error(f"open: {path}: an h5ebsd file this build does not read (kikuchipy's is read)")
+
+                def num(g, k, default=np.nan):
+                    return float(np.asarray(g[k].read()).reshape(-1)[0]) if k in g else default
+
+                rows, cols = int(num(hdr, "n_rows")), int(num(hdr, "n_columns"))
+                pats = data["patterns"].read()
+                n, h, w = pats.shape[-3:] if pats.ndim >= 3 else (0, 0, 0)
+                if rows * cols != n:
+                    rows, cols = 1, n
+                pc = tuple(float(np.mean(hdr[k].read())) for k in ("pcx", "pcy", "pcz"))
+                geo = {"pc": pc, "sample_tilt_deg": num(hdr, "sample_tilt", 70.0), "detector_tilt_deg": num(hdr, "elevation_angle", 0.0)}
+                px, binning = num(hdr, "detector_pixel_size", 0.0), num(hdr, "binning", 1.0)
+                if px > 1:  # kikuchipy writes 1 when the size is unknown
+                    geo["px_um"] = px * (binning if binning > 0 else 1)
+                for k, name in (("step_x_um", "step_x"), ("step_y_um", "step_y")):
+                    v = num(hdr, name)
+                    if np.isfinite(v) and v > 0:
+                        geo[k] = v
+                bg = hdr["static_background"].read().astype(np.float32) if "static_background" in hdr else None
+        except h5.H5Error as ex:
+            raise Error(f"open: {ex}") from None
+        self.set_patterns(np.ascontiguousarray(pats.reshape(rows, cols, h, w)))
+        self.set_geometry(**geo)
+        self._e.file_background = bg
+        self._opened = {"path": str(path), "format": "kikuchipy h5ebsd", "has_pc": True, "has_background": bg is not None}
         return self
 
     @property
@@ -129,7 +174,30 @@ class Solver:
         rows, cols, h, w = self._e.shape
         p = self._e.patterns
         dt = str(p.dtype).replace("torch.", "")
-        return {"rows": rows, "cols": cols, "pattern_shape": (h, w), "dtype": dt}
+        info = {"rows": rows, "cols": cols, "pattern_shape": (h, w), "dtype": dt, "format": "patterns", "has_background": False,
+                "has_pc": False}
+        info.update({k: v for k, v in (getattr(self, "_opened", None) or {}).items() if k != "path"})
+        return info
+
+    def pattern(self, cell, processed=True):
+        """One pattern as float32 (h, w): as it is indexed (filtered and normalized) or, with
+        processed=False, as stored; "background": the static background subtracted from every
+        pattern (an error when none is)."""
+        e = self._e
+        if processed == "background":
+            if e.settings["static_background"] == "none":
+                raise Error("pattern: static_background subtracts none")
+            return e.static_background().cpu().numpy()
+        h, w = e.pattern_hw()
+        if not processed:
+            flat = e.patterns.reshape(-1, h, w)[int(cell)]
+            return (flat.float().cpu().numpy() if hasattr(flat, "cpu") else np.asarray(flat, np.float32)).reshape(h, w)
+        raw = e._raw_for(np.array([int(cell)]))
+        import torch
+        from .engine import stream_ptr
+        out = torch.empty((1, h, w), dtype=torch.float32, device=raw.device)
+        e.prep_into(raw, 1, h, w, out, stream_ptr())
+        return out[0].cpu().numpy()
 
     def shape(self, full=False):
         return self._e.pattern_hw()

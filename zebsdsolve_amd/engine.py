@@ -143,6 +143,7 @@ class Engine:
         self.correlator = "mfma"
         self._lanes = []
         self._bg = None           # (key, static background on the device)
+        self.file_background = None  # [h, w] the scan file's static background (static_background "file")
 
     # ── state ──
 
@@ -370,7 +371,14 @@ class Engine:
         """The static background: the mean pattern over a lattice of `static_patterns` cells of
         the map, summed in float64 (src/engine.zig:1344-1385), on the device."""
         rows, cols, h, w = self.shape
-        key = (id(self.patterns), self.shape, int(self.settings["static_patterns"]))
+        kind = self.settings["static_background"]
+        key = (id(self.patterns), self.shape, int(self.settings["static_patterns"]), kind, id(self.file_background))
+        if kind == "file":
+            if self.file_background is None:
+                raise ValueError('static_background "file": the scan stores no background')
+            if self._bg is None or self._bg[0] != key:
+                self._bg = (key, torch.as_tensor(np.asarray(self.file_background, np.float32)).reshape(h, w).to(self._device()).contiguous())
+            return self._bg[1]
         if self._bg is None or self._bg[0] != key:
             cells = lattice(rows, cols, int(self.settings["static_patterns"]))
             flat = self.patterns.reshape(rows * cols, h, w)
@@ -394,7 +402,7 @@ class Engine:
             nat.check(lib.zb_normalize(raw.data_ptr(), kind, n, h, w, mask, out.data_ptr(), st), "normalize")
             return
         o = nat.PrepOpts()
-        bg = self.static_background() if s["static_background"] == "average" else None
+        bg = self.static_background() if s["static_background"] != "none" else None
         o.background = bg.data_ptr() if bg is not None else None
         o.dynamic = {"none": 0, "subtract": 1, "divide": 2}[s["dynamic_background"]]
         o.dynamic_sigma = float(s["dynamic_sigma"]) * w

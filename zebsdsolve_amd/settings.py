@@ -37,10 +37,9 @@ DEFAULTS = {
     "sweeps": 1, "sweep_bandwidth": 32, "sweep_method": "same", "accept_nmi": 0.0, "accept_z": 15.0, "accept_step_deg": 0.2,
 }
 
-# values a key outside this build may take ("file" static backgrounds need a scan file, and the
-# readers are outside this build)
+# values a key may take where this build has only some of the reference's
 _ONLY = {
-    "nlpar": (False,), "static_background": ("average", "none"), "dynamic_background": ("subtract", "divide", "none"), "clahe": (False,),
+    "nlpar": (False,), "static_background": ("average", "file", "none"), "dynamic_background": ("subtract", "divide", "none"), "clahe": (False,),
     "bandpass": (False,), "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",),
     "sweep_method": ("same", "spherical"),
 }
