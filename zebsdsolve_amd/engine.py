@@ -18,6 +18,9 @@ PIXEL_KIND = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32
 TORCH_PIXEL_KIND = {torch.uint8: 0, torch.uint16: 1, torch.float32: 2} if hasattr(torch, "uint16") else {torch.uint8: 0, torch.float32: 2}
 # device memory the correlation's spectra may take per chunk of items
 SPEC_BUDGET = 4 << 30
+# items whose coefficients and refinement go in one launch (the correlation takes them in
+# SPEC_BUDGET chunks)
+BATCH_ITEMS = 16384
 
 
 def device():
@@ -440,48 +443,53 @@ class Engine:
             corr_ws = [lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k)) for ph in phs]
         sheets = [self.baked(pi)[1][0] for pi in range(n_ph)]
         grads_h = [self.grad_half(pi) for pi in range(n_ph)]
-        # bytes per item at scale (the matrix-core path pads items to a multiple of 64)
+        # the correlation's workspace bounds its chunk (bytes per item at scale; the matrix-core
+        # path pads items); the coefficients and the refinement take the whole batch at once, so
+        # that an uneven batch is one launch of each, not one per correlation chunk
         per_item = max(f(4096) for f in corr_ws) // 4096 + 1
         chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
+        big = max(chunk, min(n, BATCH_ITEMS))
         pairs = band.host.pairs
-        need = max(int(lib.zb_sh_workspace(band.c, chunk)), max(f(chunk) for f in corr_ws))
+        need = max(int(lib.zb_sh_workspace(band.c, big)), max(f(chunk) for f in corr_ws))
         if lane is None:
             st = stream_ptr()
             pats = torch.empty((raw.shape[0], h, w), dtype=torch.float32, device=dev)
             records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
-            alm = torch.empty((chunk, pairs, 2), dtype=torch.float32, device=dev)
-            stats = torch.empty((chunk, 4), dtype=torch.float32, device=dev)
-            peaks = torch.empty((chunk, 8), dtype=torch.float32, device=dev)
+            alm = torch.empty((big, pairs, 2), dtype=torch.float32, device=dev)
+            stats = torch.empty((big, 4), dtype=torch.float32, device=dev)
+            peaks = torch.empty((big, 8), dtype=torch.float32, device=dev)
             if workspace is None or workspace.numel() < need:
                 workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
         else:
             lane.stream.wait_stream(torch.cuda.current_stream(dev))
             st = lane.stream.cuda_stream
             pats = lane.buf("pats", (raw.shape[0], h, w), torch.float32)
-            alm = lane.buf("alm", (chunk, pairs, 2), torch.float32)
-            stats = lane.buf("stats", (chunk, 4), torch.float32)
-            peaks = lane.buf("peaks", (chunk, 8), torch.float32)
+            alm = lane.buf("alm", (big, pairs, 2), torch.float32)
+            stats = lane.buf("stats", (big, 4), torch.float32)
+            peaks = lane.buf("peaks", (big, 8), torch.float32)
             workspace = lane.buf("work", (need,), torch.uint8)
             with torch.cuda.stream(lane.stream):
                 records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
             raw.record_stream(lane.stream)
             items.record_stream(lane.stream)
         self.prep_into(raw, raw.shape[0], h, w, pats, st)
-        for c0 in range(0, n, chunk):
-            nt = min(chunk, n - c0)
-            it = items[c0:c0 + nt]
-            nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nt, 3, workspace.data_ptr(), workspace.numel(),
+        for b0 in range(0, n, big):
+            nbt = min(big, n - b0)
+            it = items[b0:b0 + nbt]
+            nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nbt, 3, workspace.data_ptr(), workspace.numel(),
                                              alm.data_ptr(), stats.data_ptr(), st), "coefficients")
             for pi, ph in enumerate(phs):
-                if mfma:
-                    nat.check(lib.zb_vcorr_correlate(vts[pi], band.tw.data_ptr(), alm.data_ptr(), nt, rdens[pi].data_ptr(), nt,
-                                                     workspace.data_ptr(), workspace.numel(), peaks.data_ptr(), st), "vcorr correlate")
-                else:
-                    nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
-                                                   workspace.numel(), peaks.data_ptr(), st), "correlate")
-                rec = records[pi]
-                nat.check(lib.zb_refine(sheets[pi].data_ptr(), grads_h[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(), it.data_ptr(), peaks.data_ptr(),
-                                        8, nt, opts, rec.data_ptr(), c0, st), "refine")
+                for c0 in range(0, nbt, chunk):
+                    nt = min(chunk, nbt - c0)
+                    a_c, p_c = alm[c0:c0 + nt], peaks[c0:c0 + nt]
+                    if mfma:
+                        nat.check(lib.zb_vcorr_correlate(vts[pi], band.tw.data_ptr(), a_c.data_ptr(), nt, rdens[pi].data_ptr(), nt,
+                                                         workspace.data_ptr(), workspace.numel(), p_c.data_ptr(), st), "vcorr correlate")
+                    else:
+                        nat.check(lib.zb_so3_correlate(band.c, ph.c, a_c.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
+                                                       workspace.numel(), p_c.data_ptr(), st), "correlate")
+                nat.check(lib.zb_refine(sheets[pi].data_ptr(), grads_h[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(),
+                                        it.data_ptr(), peaks.data_ptr(), 8, nbt, opts, records[pi].data_ptr(), b0, st), "refine")
         return records, opts.stamp
 
     def _raw_for(self, cells):
