@@ -49,8 +49,12 @@ def test_bad_arguments_fail_with_a_message_and_launch_nothing():
     ph = nat.Phase(3, 32, 21, 2, None, None)  # fold 3 is not a power of two
     band = nat.Band(64, 128, None, None, None, None)
     assert lib.zb_so3_workspace(ctypes.byref(band), ctypes.byref(ph), 4) == -1
-    vt = nat.VTable(96, 256, 64, 1, 96, 128, None)  # n = 256: past the register transform
-    assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "n <= 128" in nat.last_error()
+    vt = nat.VTable(96, 256, 64, 1, 96, 128, 4096)  # 96 degree slots: past the MFMA tile's 64
+    assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "degree slots" in nat.last_error()
+    vt = nat.VTable(96, 256, 16, 2, 48, 32, 4096)  # n = 256 takes two lanes per column of 32
+    assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "nc <= 64" in nat.last_error()
+    vt = nat.VTable(160, 512, 64, 2, 80, 128, None)  # n = 512: past the register transform
+    assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "n <= 256" in nat.last_error()
     rc = lib.zb_vcorr_correlate(ctypes.byref(vt), None, None, 4, None, 4, None, 0, None, None)
     assert rc == -1 and "zb_vcorr_correlate" in nat.last_error()
     opts = nat.RefineOpts(5, 0.035, 1.0, 0.02, 1e4, 64, 1)  # 64 NMI bins: too many

@@ -1,5 +1,6 @@
-"""Bandwidth 96 (n = 256, past the matrix-core correlation's register transform): the VALU
-correlation (zb_so3_*) against the float64 oracle, and the whole path against the truth.
+"""Bandwidth 96 (n = 256): the matrix-core correlation (zb_vcorr_*, its transform along a with
+two lanes per column) and the VALU correlation (zb_so3_*) against the float64 oracle, and the
+whole path against the truth.
 
 Tolerances as at L = 32 and 64 (tests/test_gpu_pipeline.py), stated at each assertion. The
 oracle's rotation tables at this size take ~30 s to build.
@@ -34,7 +35,7 @@ def test_l96_correlation_against_the_oracle_and_recovery():
     pats, truth = cases.simulate(host, side, N, seed=31)
     e.set_patterns(pats.reshape(1, N, 60, 60))
     fr, t = cases.oracle_frame(60, 60)
-    assert not e.use_mfma(L)
+    assert e.use_mfma(L)  # nickel at n = 256: the matrix cores
     # the whole path against the truth (the reference's thresholds)
     rec = e.index_cells(np.arange(N), L)[0]
     rots = e.phases[0].master.rotations
@@ -62,11 +63,16 @@ def test_l96_correlation_against_the_oracle_and_recovery():
     ws2 = torch.empty((lib.zb_so3_workspace(band.c, ph.c, 2),), dtype=torch.uint8, device="cuda")
     nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), 2, e.rden(0, L).data_ptr(), 2, ws2.data_ptr(), ws2.numel(), peaks.data_ptr(),
                                    stream_ptr()), "so3")
-    pk = peaks.cpu().numpy()
+    (vt, _), _ = ph.vtables()
+    peaks2 = torch.empty((2, 8), dtype=torch.float32, device="cuda")
+    ws3 = torch.empty((lib.zb_vcorr_workspace(vt, 2, 0),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), alm.data_ptr(), 2, e.vrden(0, L).data_ptr(), 2, ws3.data_ptr(), ws3.numel(),
+                                     peaks2.data_ptr(), stream_ptr()), "vcorr")
     pnh = pn.cpu().numpy()
-    for b in range(2):
-        ref_alm, _ = op.coefficients(pnh[b].astype(np.float64), fr, t, L)
-        q, v0, _, _ = so.peak(so.volume(f, ref_alm), rden_ref)
-        # the score to float32 accuracy, the orientation within 1/100 of a grid step (1.4 deg)
-        assert abs(pk[b, 4] - v0) < 2e-4 * abs(v0), (b, pk[b, 4], v0)
-        assert np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), q, rots)) < 0.014, b
+    for pk in (peaks.cpu().numpy(), peaks2.cpu().numpy()):
+        for b in range(2):
+            ref_alm, _ = op.coefficients(pnh[b].astype(np.float64), fr, t, L)
+            q, v0, _, _ = so.peak(so.volume(f, ref_alm), rden_ref)
+            # the score to float32 accuracy, the orientation within 1/100 of a grid step (1.4 deg)
+            assert abs(pk[b, 4] - v0) < 2e-4 * abs(v0), (b, pk[b, 4], v0)
+            assert np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), q, rots)) < 0.014, b

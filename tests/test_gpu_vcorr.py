@@ -113,9 +113,9 @@ def test_vcorr_argument_checks():
     import torch
     from zebsdsolve_amd import _native as nat
     lib = nat.load()
-    bad = nat.VTable(64, 256, 64, 1, 64, 128, 0)
+    bad = nat.VTable(128, 512, 64, 2, 64, 128, 0)  # n = 512: past the register transform
     assert lib.zb_vcorr_workspace(bad, 4, 0) < 0
-    assert "n <= 128" in nat.last_error()
+    assert "n <= 256" in nat.last_error()
     x = torch.zeros(16, device="cuda")
     good = nat.VTable(32, 64, 16, 2, 16, 32, x.data_ptr())
     need = lib.zb_vcorr_workspace(good, 4, 0)
@@ -124,7 +124,7 @@ def test_vcorr_argument_checks():
     assert rc == -2 and "workspace" in nat.last_error()
 
 
-@pytest.mark.parametrize("L,N", [(32, 300), (64, 200), (64, 1500)])
+@pytest.mark.parametrize("L,N", [(32, 300), (64, 200), (64, 1500), (96, 301)])
 def test_vcorr_matches_slab_path_over_many_tiles(L, N):
     """Several item tiles and a ragged last one (N not a multiple of 128), with many blocks in
     flight at once: random unit-norm coefficients through both correlators, and the matrix-core

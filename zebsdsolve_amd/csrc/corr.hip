@@ -36,6 +36,7 @@
 // check.
 #include "common.h"
 #include "tw128.h"
+#include "tw256.h"
 
 namespace zb {
 
@@ -435,6 +436,171 @@ __global__ void __launch_bounds__(256) k_apeak(VArg a, const float2* __restrict_
     }
 }
 
+// a per-lane choice of two values (by value: a conditional over two lvalues would select an
+// address and push the register array out to scratch memory)
+__device__ __forceinline__ float2 pick(bool p, float2 x, float2 y) { return make_float2(p ? x.x : y.x, p ? x.y : y.y); }
+__device__ __forceinline__ float2 tw128c(int k) { return make_float2(kTwCos[k & 127], -kTwSin[k & 127]); }
+__device__ __forceinline__ float2 tw256c(int k) { return make_float2(kTw256Cos[k], -kTw256Sin[k]); }
+
+// k_apeak for n = 256 (a 128-point half-length transform), whose 128 complex values a column
+// would need do not fit one lane's registers: two lanes per column, lane (hf, c), hf = lane / 32,
+// register r holding the column's order q = 2 r + hf. The in-place build pairs q with 128 - q,
+// which sits in the same lane (register 64 - r for hf 0, 63 - r for hf 1); each half then runs
+// the 64-point transform of its own orders (E of the even ones, O of the odd ones) in
+// registers, and the one stage that mixes the halves, X[k] = E[k] + w_k O[k] and
+// X[k + 64] = E[k] - w_k O[k], trades values once (v_permlane32_swap) on the way to the peak
+// search. One wave per (j, 32 columns, item), four items a block on one j and column group,
+// whose 32 denominator columns go straight into LDS ahead of the spectrum loads (ONEVOL).
+// MODE 0: the peak search of k_apeak; MODE 1: the volume, natural order.
+template <int MODE, bool ONEVOL>
+__global__ void __launch_bounds__(256, 2) k_apeak128(VArg a, const float2* __restrict__ v, const float* __restrict__ rden, int rden_div,
+                                                  unsigned long long* __restrict__ best, float2* __restrict__ vol) {
+    constexpr int Q = 64, N = 256;
+    extern __shared__ float4 rlds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const bool hf = lane >= 32;
+    const int cl = lane & 31;
+    const int ncw = a.nc >> 5;
+    const int groups = (a.n_items + 3) >> 2;
+    const int id = blockIdx.x;
+    const int g = id % groups;
+    const int rest = id / groups;
+    const int cw = rest % ncw;
+    const int j = rest / ncw;
+    const int b0 = g * 4;
+    const int b = b0 + (tid >> 6);
+    const bool live = b < a.n_items;
+    const int c = cw * 32 + cl;
+    constexpr bool one_vol = MODE == 0 && ONEVOL;
+    if constexpr (one_vol) {
+        // float4 slot (a4 32 + column) of the group's 32 columns, 64 slots an instruction
+        const float4* src = reinterpret_cast<const float4*>(rden + (((size_t)(b0 / rden_div) * a.nb + j) * a.nc + cw * 32) * N);
+#pragma unroll
+        for (int k = tid >> 6; k < 32; k += 4) {
+            const int sidx = k * 64 + lane;
+            const int a4 = sidx >> 5, cc = sidx & 31;
+            __builtin_amdgcn_global_load_lds((const void*)(src + cc * (N / 4) + a4), (void __attribute__((address_space(3)))*)(rlds + k * 64), 16, 0,
+                                             0);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const int wc = a.cols / 2;
+    const unsigned rs = (unsigned)(a.ni_pad * wc) * 8u;
+    const auto vrs = buf(v + (size_t)j * a.L * a.ni_pad * wc, (long long)a.L * rs);
+    // orders q >= L lie past the resource's L rows and load as zeros (no lane-divergent branch);
+    // the whole offset goes in the per-lane register, which the bounds check covers
+    const unsigned vl = (unsigned)((live ? b : 0) * wc + c) * 8u + (hf ? rs : 0u);
+    float2 z[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) z[r] = buf_f32x2(vrs, vl + (unsigned)r * 2u * rs, 0u);
+    // the half-length input: with s = X_q + conj X_{128-q}, d = X_q - conj X_{128-q} and
+    // t = w_q d (w_q = exp(-2 pi i q / 256)), Z_q = s + i t and Z_{128-q} = conj s + i conj t.
+    // X_0 is taken real (X_128 = 0 for L <= 128); order 64 (hf 0, r 32) pairs with itself.
+    auto pair = [](float2 x, float2 y, float2 w, float2& lo, float2& hi) {
+        const float2 sm = make_float2(x.x + y.x, x.y - y.y);
+        const float2 t = cmul(make_float2(x.x - y.x, x.y + y.y), w);
+        lo = make_float2(sm.x - t.y, sm.y + t.x);
+        hi = make_float2(sm.x + t.y, t.x - sm.y);
+    };
+    {
+        float2 lo, hi;
+        pair(z[0], z[Q - 1], tw256c(1), lo, hi);  // hf 1: orders 1 and 127
+        const float x0 = z[0].x;
+        z[0] = pick(hf, lo, make_float2(x0, x0));
+        z[Q - 1] = pick(hf, hi, z[Q - 1]);
+    }
+#pragma unroll
+    for (int r = 1; r < Q / 2; ++r) {
+        // hf 0: orders 2 r and 128 - 2 r (register 64 - r); hf 1: 2 r + 1 and 127 - 2 r (63 - r)
+        const float2 y = pick(hf, z[Q - 1 - r], z[Q - r]);
+        float2 lo, hi;
+        pair(z[r], y, pick(hf, tw256c(2 * r + 1), tw128c(r)), lo, hi);
+        z[r] = lo;
+        z[Q - 1 - r] = pick(hf, hi, z[Q - 1 - r]);
+        z[Q - r] = pick(hf, z[Q - r], hi);
+    }
+    z[Q / 2] = pick(hf, z[Q / 2], zterm(z[Q / 2], make_float2(z[Q / 2].x, -z[Q / 2].y), 32));
+    // the 64-point transforms within each half: natural in, bit-reversed out
+#pragma unroll
+    for (int hh = Q / 2; hh >= 1; hh >>= 1) {
+#pragma unroll
+        for (int blk = 0; blk < Q; blk += 2 * hh) {
+#pragma unroll
+            for (int jj = 0; jj < hh; ++jj) {
+                const float2 u = z[blk + jj], w = z[blk + jj + hh];
+                z[blk + jj] = cadd(u, w);
+                z[blk + jj + hh] = twmul(csub(u, w), jj * (64 / hh));
+            }
+        }
+    }
+    // z[r] = E[k] (hf 0) or O[k] (hf 1), k = bitrev6(r); registers r and r + 32 hold k and k + 1.
+    // Swapping lanes 32..63 of register r with lanes 0..31 of register r + 32 leaves E[k], O[k]
+    // in the low half and E[k + 1], O[k + 1] in the high half: each lane combines its own k'.
+#pragma unroll
+    for (int r = 0; r < Q / 2; ++r) {
+        const int k = brev_c<6>(r);
+        const auto sx = __builtin_amdgcn_permlane32_swap(__float_as_uint(z[r].x), __float_as_uint(z[r + Q / 2].x), false, false);
+        const auto sy = __builtin_amdgcn_permlane32_swap(__float_as_uint(z[r].y), __float_as_uint(z[r + Q / 2].y), false, false);
+        const float2 e = make_float2(__uint_as_float(sx[0]), __uint_as_float(sy[0]));
+        const float2 o = make_float2(__uint_as_float(sx[1]), __uint_as_float(sy[1]));
+        const float2 t = cmul(o, pick(hf, tw128c(k + 1), tw128c(k)));
+        z[r] = cadd(e, t);
+        z[r + Q / 2] = csub(e, t);
+    }
+    // register r of half hf: X[K + hf] = (C[2 (K + hf)], C[2 (K + hf) + 1]) with
+    // K = bitrev6(r mod 32) + 64 (r / 32), always even
+    auto kk = [](int r) { return brev_c<6>(r & 31) + 64 * (r >> 5); };
+    if (MODE == 1) {
+        if (!live) return;
+        float2* out = vol + (((size_t)b * a.nb + j) * a.nc + c) * (N / 2) + (hf ? 1 : 0);
+#pragma unroll
+        for (int r = 0; r < Q; ++r) out[kk(r)] = z[r];
+        return;
+    }
+    float m = -3.4e38f;
+    if constexpr (one_vol) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        // values 2 (K + hf), +1: float4 slot (K / 2) 32 + column, floats 2 hf and 2 hf + 1
+        const float* rd = reinterpret_cast<const float*>(rlds) + cl * 4 + (hf ? 2 : 0);
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            const float2 w = *reinterpret_cast<const float2*>(rd + (kk(r) >> 1) * 128);
+            z[r] = make_float2(z[r].x * w.x, z[r].y * w.y);
+            m = fmaxf(m, fmaxf(z[r].x, z[r].y));
+        }
+    } else {
+        const float2* rd = reinterpret_cast<const float2*>(rden + (((size_t)((live ? b : 0) / rden_div) * a.nb + j) * a.nc + c) * N) + (hf ? 1 : 0);
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            const float2 w = rd[kk(r)];
+            z[r] = make_float2(z[r].x * w.x, z[r].y * w.y);
+            m = fmaxf(m, fmaxf(z[r].x, z[r].y));
+        }
+    }
+    if (!live) return;  // wave-uniform (after the block's barrier)
+    float smax = m;
+    for (int o = 1; o < 64; o <<= 1) smax = fmaxf(smax, __shfl_xor(smax, o, 64));
+    const unsigned long long cur = __hip_atomic_load(best + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(cur == 0ull || smax >= from_ord((unsigned)(cur >> 32)))) return;  // wave-uniform
+    unsigned idx = 0xffffffffu;
+    if (m == smax) {
+        const unsigned base = (unsigned)(j * a.nc + c) * (unsigned)N + (hf ? 2u : 0u);
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            const unsigned f2 = 2u * (unsigned)kk(r);
+            if (z[r].x == smax) idx = min(idx, base + f2);
+            if (z[r].y == smax) idx = min(idx, base + f2 + 1);
+        }
+    }
+    for (int o = 1; o < 64; o <<= 1) idx = min(idx, (unsigned)__shfl_xor((int)idx, o, 64));
+    if (lane == 0) {
+        const unsigned long long key = ((unsigned long long)ord_bits(smax) << 32) | (unsigned long long)(0xffffffffu - idx);
+        atomicMax(best + b, key);
+    }
+}
+
 __global__ void k_vclear(unsigned long long* p, int n) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -536,8 +702,8 @@ static int ilog2v(int v) {
 static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
     if (!t || t->L < 2 || ilog2v(t->n) < 4 || t->n < 2 * t->L - 1 || ilog2v(t->nc) < 1 || t->nc > 64 || t->n / t->nc > 4 ||
         !(t->l_step == 1 || t->l_step == 2) || t->slots != (t->L + t->l_step - 1) / t->l_step || t->cols != 32 * ((2 * t->nc + 31) / 32) ||
-        t->n / 2 > 64 || !t->q) {
-        set_error("%s: bad table (L %d, n %d, nc %d, slots %d, cols %d; this path takes n <= 128, nc <= 64)", who, t ? t->L : 0, t ? t->n : 0,
+        t->n / 2 > 128 || (t->n == 256 && t->nc < 32) || !t->q) {
+        set_error("%s: bad table (L %d, n %d, nc %d, slots %d, cols %d; this path takes n <= 256, nc <= 64)", who, t ? t->L : 0, t ? t->n : 0,
                   t ? t->nc : 0, t ? t->slots : 0, t ? t->cols : 0);
         return ZB_EINVAL;
     }
@@ -560,6 +726,14 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
     // (j, m')) do not sit a power of two apart in HBM
     a.ni_pad = (n_items + kItemPad - 1) / kItemPad * kItemPad + 32;
     a.tiles = 1;
+    // the transform along a addresses one j's rows of V with 32-bit byte offsets (at n = 256 up
+    // to row 127, past the L loaded ones, where the bounds check returns zeros)
+    const int64_t rows = a.n == 256 ? 128 : a.L;
+    if ((int64_t)a.ni_pad * (a.cols / 2) * 8 * rows > 0x7fffffffLL) {
+        set_error("%s: %d items in one call: at most %lld at this size", who, n_items,
+                  (long long)(0x7fffffffLL / ((int64_t)(a.cols / 2) * 8 * rows) - kItemPad - 32));
+        return ZB_EINVAL;
+    }
     return ZB_OK;
 }
 
@@ -631,6 +805,17 @@ extern "C" int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int
 
 template <int MODE>
 static int launch_apeak(const VArg& a, const float* v, const float* rden, int rden_div, unsigned long long* best, float* vol, hipStream_t s) {
+    if (a.n == 256) {
+        // two lanes per column: one wave per (j, 32 columns, item), four items a block
+        const bool onevol = MODE == 0 && (rden_div >= a.n_items || rden_div % 4 == 0);
+        const dim3 grid((unsigned)((long)a.nb * (a.nc / 32) * ((a.n_items + 3) / 4)));
+        const size_t lds = onevol ? (size_t)32 * 256 * sizeof(float) : 0;
+        if (onevol)
+            ZB_LAUNCH("k_apeak", (k_apeak128<MODE, true>), grid, dim3(256), lds, s, a, (const float2*)v, rden, rden_div, best, (float2*)vol);
+        else
+            ZB_LAUNCH("k_apeak", (k_apeak128<MODE, false>), grid, dim3(256), lds, s, a, (const float2*)v, rden, rden_div, best, (float2*)vol);
+        return check_launch("k_apeak");
+    }
     const int per_wave = 64 >> a.log2nc;
     // up to four waves a block, at most 32 slabs (ni_pad is a multiple of 32)
     const int waves = per_wave >= 32 ? 1 : (32 / per_wave < 4 ? 32 / per_wave : 4);

@@ -97,10 +97,15 @@ def lattice(rows, cols, want):
     return (((2 * r + 1) * rows // (2 * gy)) * cols + (2 * c + 1) * cols // (2 * gx)).reshape(-1)
 
 
-def mfma_correlation(L, n):
-    """Whether the matrix-core correlation takes this bandwidth (its transform along a is held in
-    registers: n <= 128)."""
-    return n <= 128
+def mfma_correlation(L, n, nc=None, slots=None):
+    """Whether the matrix-core correlation takes this shape: its transform along a is held in
+    registers (n <= 128 one lane per column, n = 256 two lanes per column with at least 32
+    columns), the table's columns 2 nc <= 128 and its degree slots <= 64."""
+    if nc is not None and (nc > 64 or (n == 256 and nc < 32)):
+        return False
+    if slots is not None and slots > 64:
+        return False
+    return n <= 256
 
 
 class Lane:
@@ -352,8 +357,17 @@ class Engine:
             self._lanes.append(Lane(dev))
         return self._lanes[:k]
 
-    def use_mfma(self, L):
-        return self.correlator == "mfma" and mfma_correlation(L, self.band(L).host.n)
+    def use_mfma(self, L, pi=None):
+        """The matrix-core correlation for phase pi at L (every phase when pi is None)."""
+        if self.correlator != "mfma":
+            return False
+        n = self.band(L).host.n
+        phs = range(len(self.phases)) if pi is None else [pi]
+        for k in phs:
+            h = self.phase_band(k, L).host
+            if not mfma_correlation(L, n, h.nc, (L + h.l_step - 1) // h.l_step):
+                return False
+        return mfma_correlation(L, n)
 
     # ── the batch pipeline ──
 
@@ -433,14 +447,12 @@ class Engine:
         opts = self.refine_opts()
         # per-phase tables (built on the current stream the first time)
         phs = [self.phase_band(pi, L) for pi in range(n_ph)]
-        mfma = self.use_mfma(L)
-        if mfma:
-            rdens = [self.vrden(pi, L) for pi in range(n_ph)]
-            vts = [ph.vtables()[0][0] for ph in phs]
-            corr_ws = [lambda k, vt=vt: int(lib.zb_vcorr_workspace(vt, k, 0)) for vt in vts]
-        else:
-            rdens = [self.rden(pi, L) for pi in range(n_ph)]
-            corr_ws = [lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k)) for ph in phs]
+        # the correlation per phase: on the matrix cores where its shape allows, else the VALU form
+        mfma = [self.use_mfma(L, pi) for pi in range(n_ph)]
+        rdens = [self.vrden(pi, L) if mfma[pi] else self.rden(pi, L) for pi in range(n_ph)]
+        vts = [ph.vtables()[0][0] if mfma[pi] else None for pi, ph in enumerate(phs)]
+        corr_ws = [(lambda k, vt=vts[pi]: int(lib.zb_vcorr_workspace(vt, k, 0))) if mfma[pi] else
+                   (lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k))) for pi, ph in enumerate(phs)]
         sheets = [self.baked(pi)[1][0] for pi in range(n_ph)]
         grads_h = [self.grad_half(pi) for pi in range(n_ph)]
         # the correlation's workspace bounds its chunk (bytes per item at scale; the matrix-core
@@ -482,7 +494,7 @@ class Engine:
                 for c0 in range(0, nbt, chunk):
                     nt = min(chunk, nbt - c0)
                     a_c, p_c = alm[c0:c0 + nt], peaks[c0:c0 + nt]
-                    if mfma:
+                    if mfma[pi]:
                         nat.check(lib.zb_vcorr_correlate(vts[pi], band.tw.data_ptr(), a_c.data_ptr(), nt, rdens[pi].data_ptr(), nt,
                                                          workspace.data_ptr(), workspace.numel(), p_c.data_ptr(), st), "vcorr correlate")
                     else:
