@@ -202,6 +202,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--correlator", choices=("mfma", "slab"), default="mfma",
                     help="the SO(3) correlation as a GEMM on the matrix cores, or the spectrum + slab transforms")
+    ap.add_argument("--precision", choices=("fp32", "fp16", "bf16"), default="fp32",
+                    help="products of the correlation's GEMM: float32, or 16-bit with float32 accumulation (hi + lo for l >= 64)")
+    ap.add_argument("--size", type=int, default=60, help="pattern side in pixels (60: the plain FP32 form; 120: the precision form)")
     ap.add_argument("--lanes", type=int, default=1, help="streams the chunks are spread over (1: one stream)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
@@ -222,6 +225,9 @@ def main():
               "exchange in the hot loop; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    global H, W
+    H = W = args.size
+
     import torch
     from zebsdsolve_amd import _native as nat, engine, sym
     from zebsdsolve_amd.geometry import Geometry
@@ -231,7 +237,7 @@ def main():
     e.settings.apply(static_background="none", dynamic_background="none")
     e.geo = Geometry(pc=PC, sample_tilt_deg=70.0)
     e.add_phase(MASTER)
-    e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters)
+    e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters, correlation_precision=args.precision)
     e.correlator = args.correlator
     e.set_patterns(np.zeros((1, 1, H, W), np.float32))
     n = args.patterns
@@ -332,12 +338,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.precision == "fp32" else f"{args.precision} GEMM products (f32 accumulate), the rest f32",
         "data": "synthetic: Ni master projected on the GPU at seeded random orientations, PC jitter +-0.01, Poisson 400 counts",
         "config": {"workload": f"{n} patterns {H}x{W} f32, Ni (Fm-3m, bundled ebsdsim master), L={args.bandwidth}, "
                                f"{args.iters} Gauss-Newton iterations, chunks U[1000, 8000]",
                    "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks),
-                   "correlator": "mfma" if e.use_mfma(args.bandwidth) else "slab", "streams": max(args.lanes, 1)},
+                   "correlator": "mfma" if e.use_mfma(args.bandwidth) else "slab", "streams": max(args.lanes, 1),
+                   "precision": args.precision},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
                      "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1)},
         "roofline": roofline,

@@ -80,20 +80,36 @@ typedef struct zb_phase {
  * [t][b][32 r + x][i] = row r, slot 4t + i, column 32 (g nblk + b) + x (zero for slots past the
  * last). Built once per phase and bandwidth (zebsdsolve_amd/tables.py VTableHost). No
  * counterpart in the reference's tables; it replaces W_f of spherical.PhaseGpu
- * (src/spherical.zig:192-242) on this path. */
+ * (src/spherical.zig:192-242) on this path.
+ *
+ * With lp_kind 1 (FP16) or 2 (BF16), zb_vcorr_correlate multiplies in 16 bits with FP32
+ * accumulation, from qlp: the same rows with each value x as hi = round16(x) and
+ * lo = round16(x - hi), tiled by (cols, G, nblk) from zb_vtable16_shape: per (j, m', column
+ * group g < cols / (32 nblk)), the G groups of 8 slots in order, each [nblk][64][8] hi values,
+ * element [b][32 r + x][i] = row r, slot 8t + i, column 32 (g nblk + b) + x, followed, for groups
+ * t >= lp_split / 8, by the same for lo. Those groups take hi hi + hi lo + lo hi (the item's
+ * coefficients split the same way), the others hi hi. Then q is used only by
+ * zb_vcorr_denominators (the window's volumes stay FP32). lp_kind 0: FP32 products from q. */
 typedef struct zb_vtable {
     int32_t L;
-    int32_t n;      /* fft_size(L), at most 128 on this path */
+    int32_t n;      /* fft_size(L), at most 256 on this path */
     int32_t nc;     /* n / fold */
     int32_t l_step; /* 2 when the master has no odd l, else 1 */
     int32_t slots;  /* ceil(L / l_step) */
     int32_t cols;
     const float* q; /* tiled, (n/2 + 1) L S 2 cols floats */
+    int32_t lp_kind;  /* 0 FP32 products, 1 FP16, 2 BF16 */
+    int32_t lp_split; /* first slot (a multiple of 8) of the groups with hi + lo products */
+    const void* qlp;  /* the 16-bit table (lp_kind != 0), 2-byte values */
 } zb_vtable;
 
 /* The tiling of zb_vtable.q for nc points along c and `slots` degree slots: cols, S (slots per
- * wave, a power of two >= slots, at least 8) and nblk (column blocks of 32 per wave). */
+ * wave, the first of 8, 16, 32, 48, 64 >= slots) and nblk (column blocks of 32 per wave). */
 int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* S, int32_t* nblk);
+
+/* The tiling of zb_vtable.qlp: cols, groups (of 8 slots: 2 ceil(slots / 16)) and nblk (1 when
+ * cols = 32, else 2). */
+int32_t zb_vtable16_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* groups, int32_t* nblk);
 
 /* Gauss-Newton refinement options (detector.RefineOptions, src/detector.zig:57-70). */
 typedef struct zb_refine_opts {

@@ -103,3 +103,60 @@ def test_table_tiling_rule_is_the_librarys():
         bb, x = divmod(rest, 32)
         want = T[j, m, s, r, col] if s < slots else 0.0
         assert t[j, m, g, s // 4, bb, 32 * r + x, s % 4] == want
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
+def test_table16_tiling_rule_is_the_librarys():
+    """The 16-bit table: the host's shape rule is the library's, each hi value sits where the
+    header says, the lo values follow their group's hi from the split group on, and hi + lo
+    gives back the value."""
+    import numpy as np
+    from zebsdsolve_amd import _native as nat, tables
+    lib = nat.load()
+    c, G, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    for nc in (2, 4, 8, 16, 32, 64):
+        for slots in (1, 5, 8, 9, 16, 17, 24, 32, 33, 48, 64):
+            assert lib.zb_vtable16_shape(nc, slots, ctypes.byref(c), ctypes.byref(G), ctypes.byref(b)) == 0
+            assert (c.value, G.value, b.value) == tables.vgemm16_shape(nc, slots)
+    assert lib.zb_vtable16_shape(48, 8, ctypes.byref(c), ctypes.byref(G), ctypes.byref(b)) == -1
+    rng = np.random.default_rng(1)
+    L, slots, cols = 3, 40, 64
+    T = rng.normal(size=(L, slots, 2, cols))
+    _, groups, nblk = tables.vgemm16_shape(32, slots)
+    assert (groups, nblk) == (6, 2)
+    ncg = cols // (32 * nblk)
+    for kind, rel in ((1, 2.0 ** -21), (2, 2.0 ** -15)):
+        split = 32  # groups 4 and 5 carry lo
+        flat = tables.tile_vtable16(T, groups, nblk, split, kind)
+        per = nblk * 64 * 8
+        assert flat.size == L * ncg * (groups + 2) * per
+        blocks = flat.reshape(L, ncg, groups + 2, nblk, 64, 8)
+        val = (lambda u: u.view(np.float16).astype(np.float64)) if kind == 1 else \
+              (lambda u: (u.astype(np.uint32) << np.uint32(16)).view(np.float32).astype(np.float64))
+        for _ in range(300):
+            m, s, r, col = rng.integers(L), rng.integers(8 * groups), rng.integers(2), rng.integers(cols)
+            g, rest = divmod(col, 32 * nblk)
+            bb, x = divmod(rest, 32)
+            t = s // 8
+            pos = t if t < 4 else 4 + 2 * (t - 4)
+            hi = val(blocks[m, g, pos, bb, 32 * r + x, s % 8])
+            want = T[m, s, r, col] if s < slots else 0.0
+            if t < 4:
+                assert abs(hi - want) <= 2.0 ** (-8 if kind == 2 else -11) * abs(want)
+            else:
+                lo = val(blocks[m, g, pos + 1, bb, 32 * r + x, s % 8])
+                assert abs(hi + lo - want) <= rel * abs(want) + 1e-30
+
+
+def test_round16_is_nearest_even():
+    """FP16 is numpy's; BF16 keeps the top 16 bits of the float32, nearest with ties to even."""
+    import numpy as np
+    from zebsdsolve_amd import tables
+    x = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, 1.0 + 2.0 ** -8 + 2.0 ** -20, -2.5, 0.0, 3.0e38])
+    bits, v = tables.round16(x, 2)
+    assert list(bits) == [0x3F80, 0x3F80, 0x3F82, 0x3F81, 0xC020, 0x0000, 0x7F62]
+    assert v[1] == 1.0 and v[2] == 1.0 + 2.0 ** -6 and v[3] == 1.0 + 2.0 ** -7
+    h, hv = tables.round16(np.array([1.0 + 2.0 ** -11, 65504.0]), 1)
+    assert hv[0] == 1.0 and hv[1] == 65504.0
+    with pytest.raises(ValueError):
+        tables.round16(np.array([70000.0]), 1)

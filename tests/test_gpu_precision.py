@@ -1,0 +1,142 @@
+"""The correlation's GEMM with 16-bit products (FP16 and BF16 in, FP32 accumulate, hi + lo
+products for the degrees from split_degree on) against the FP32 GEMM and the float64 oracle,
+and the whole path at L = 96 against the truth.
+
+Tolerances are stated at each assertion. What was measured on the MI355X at the time of
+writing (256-512 simulated nickel patterns, scores relative to the FP32 GEMM's): hi-only FP16
+products p99 5e-5, BF16 4.5e-4; hi + lo on every degree 3e-7 (FP16), 1.5e-6 (BF16).
+"""
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.conftest import GAN, NI, gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+KIND = {"fp16": 1, "bf16": 2}
+
+
+def _engine(path, L, N, seed=21, size=60):
+    from zebsdsolve_amd import engine
+    from zebsdsolve_amd.geometry import Geometry
+    e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")  # the path alone, as the oracle
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(path)
+    e.settings.apply(bandwidth=L)
+    e.set_patterns(np.zeros((1, N, size, size), np.float32))
+    _, (_, _, host) = e.baked(0)
+    side = e.phases[0].master.side
+    pats, truth = cases.simulate(host, side, N, seed=seed, h=size, w=size)
+    e.set_patterns(pats.reshape(1, N, size, size))
+    return e, host, side, pats, truth
+
+
+def _correlate(e, L, vt, alm):
+    import torch
+    from zebsdsolve_amd import _native as nat
+    from zebsdsolve_amd.engine import stream_ptr
+    lib = nat.load()
+    N = alm.shape[0]
+    band = e.band(L)
+    peaks = torch.empty((N, 8), dtype=torch.float32, device="cuda")
+    ws = torch.empty((lib.zb_vcorr_workspace(vt, N, 0),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), alm.data_ptr(), N, e.vrden(0, L).data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                     peaks.data_ptr(), stream_ptr()), "vcorr")
+    return peaks.cpu().numpy()
+
+
+CASES = [("Ni", NI, 64), ("GaN", GAN, 64), ("Ni", NI, 96)]
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("name,path,L", CASES, ids=[f"{c[0]}-L{c[2]}" for c in CASES])
+def test_16bit_products_against_fp32(name, path, L, prec):
+    from tests.test_gpu_vcorr import _coefficients
+    from zebsdsolve_amd import sym
+    print(f"16-bit products {name} L={L} {prec}", flush=True)
+    N = 256
+    e, host, side, pats, truth = _engine(path, L, N)
+    alm, _ = _coefficients(e, L, pats)
+    ph = e.phase_band(0, L)
+    rots = e.phases[0].master.rotations
+    ref = _correlate(e, L, ph.vtables()[0][0], alm)
+    # hi + lo on every degree: FP32 accuracy (the scores to 1e-5, the same sub-cell peak)
+    pk = _correlate(e, L, ph.vtable16(KIND[prec], 0), alm)
+    rel = np.abs(pk[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])
+    assert rel.max() < 1e-5, rel.max()
+    d = np.degrees(sym.disorientation(pk[:, :4].astype(np.float64), ref[:, :4].astype(np.float64), rots))
+    assert d.max() < 0.002, d.max()
+    # the default split (hi + lo from l = 64): scores to 2e-4 (FP16) or 1.5e-3 (BF16) at the
+    # 99th percentile, the peak's orientation within 0.005 deg at the median, at most 2 % of the
+    # items on another cell (near-ties)
+    pk = _correlate(e, L, ph.vtable16(KIND[prec], 64), alm)
+    rel = np.abs(pk[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])
+    assert np.percentile(rel, 99) < (2e-4 if prec == "fp16" else 1.5e-3), np.percentile(rel, 99)
+    d = np.degrees(sym.disorientation(pk[:, :4].astype(np.float64), ref[:, :4].astype(np.float64), rots))
+    assert np.median(d) < 0.005 and np.mean(d > 0.1) <= 0.02, (np.median(d), np.mean(d > 0.1))
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_16bit_l96_against_the_oracle_and_truth(prec):
+    """Config of the precision form (120 x 120 patterns, L = 96, 8 iterations): the peaks of three
+    patterns against the float64 oracle, and the whole path against the truth and against the
+    FP32 form."""
+    import torch
+    from oracle import pipeline as op
+    from tests.test_gpu_vcorr import _coefficients  # noqa: F401  (same normalization path)
+    from zebsdsolve_amd import _native as nat, sym
+    from zebsdsolve_amd.engine import stream_ptr
+    print(f"16-bit L=96 120x120 {prec}", flush=True)
+    L, N, S = 96, 24, 120
+    e, host, side, pats, truth = _engine(NI, L, N, seed=33, size=S)
+    e.settings.apply(refine_iters=8)
+    rots = e.phases[0].master.rotations
+    rec32 = e.index_cells(np.arange(N), L)[0]
+    e.settings.apply(correlation_precision=prec)
+    rec = e.index_cells(np.arange(N), L)[0]
+    # the reference's thresholds against the truth (test_api.py:212), and the FP32 form's result
+    # to 0.01 deg on every pattern (the refinement starts from the same cell)
+    d = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), truth, rots))
+    assert np.median(d) < 0.15 and d.max() < 1.0, d
+    d32 = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), rec32[:, :4].astype(np.float64), rots))
+    assert d32.max() < 0.01, d32
+    assert np.abs(rec[:, 4] - rec32[:, 4]).max() < 1e-4
+    # three peaks against the oracle
+    lib = nat.load()
+    band = e.band(L)
+    fr, t = cases.oracle_frame(S, S)
+    f, f2, fold, inv = op.harmonics(host, side, L)
+    so = op.So3(L, fold)
+    wa, wst = op.coefficients(None, fr, t, L, center=False, norm=False, window=True)
+    rden_ref = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])
+    raw = torch.from_numpy(pats[:3]).cuda()
+    pn = torch.empty_like(raw)
+    mask = (nat.ctypes.c_float * 3)(*e.mask_tuple())
+    nat.check(lib.zb_normalize(raw.data_ptr(), 2, 3, S, S, mask, pn.data_ptr(), stream_ptr()), "normalize")
+    items = torch.from_numpy(np.concatenate([e.positions(np.arange(3)), np.arange(3, dtype=np.float32)[:, None]], 1)).cuda()
+    alm = torch.empty((3, band.host.pairs, 2), dtype=torch.float32, device="cuda")
+    stats = torch.empty((3, 4), dtype=torch.float32, device="cuda")
+    ws = torch.empty((lib.zb_sh_workspace(band.c, 3),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_sh_coefficients(band.c, e.frame(), pn.data_ptr(), items.data_ptr(), 3, 3, ws.data_ptr(), ws.numel(), alm.data_ptr(),
+                                     stats.data_ptr(), stream_ptr()), "coefficients")
+    pk = _correlate(e, L, e.corr_vtable(e.phase_band(0, L)), alm)
+    pnh = pn.cpu().numpy()
+    for b in range(3):
+        ref_alm, _ = op.coefficients(pnh[b].astype(np.float64), fr, t, L)
+        q, v0, _, _ = so.peak(so.volume(f, ref_alm), rden_ref)
+        # the score to 3e-4 (FP16) or 2e-3 (BF16), the orientation within 1/100 of a grid step
+        assert abs(pk[b, 4] - v0) < (3e-4 if prec == "fp16" else 2e-3) * abs(v0), (b, pk[b, 4], v0)
+        assert np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), q, rots)) < 0.014, b
+
+
+def test_16bit_table_rejects_what_the_kernel_cannot_take():
+    """A 16-bit table with a split off the 16-slot grid, an unknown kind, or no table pointer is
+    refused before anything is launched."""
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
+    for kind, split, ptr in ((1, 8, 4096), (3, 0, 4096), (1, 0, None)):
+        vt = nat.VTable(64, 128, 32, 2, 32, 64, 4096, kind, split, ptr)
+        assert lib.zb_vcorr_workspace(vt, 4, 0) < 0
+        assert "16-bit table" in nat.last_error()

@@ -29,7 +29,8 @@ class Phase(ctypes.Structure):
 
 
 class VTable(ctypes.Structure):
-    _fields_ = [("L", i32), ("n", i32), ("nc", i32), ("l_step", i32), ("slots", i32), ("cols", i32), ("q", vp)]
+    _fields_ = [("L", i32), ("n", i32), ("nc", i32), ("l_step", i32), ("slots", i32), ("cols", i32), ("q", vp),
+                ("lp_kind", i32), ("lp_split", i32), ("qlp", vp)]
 
 
 class RefineOpts(ctypes.Structure):
@@ -52,6 +53,7 @@ _SIGS = {
     "zb_so3_correlate": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_so3_denominators": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, vp, i32, vp, i64, vp, vp], i32),
     "zb_vtable_shape": ([i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
+    "zb_vtable16_shape": ([i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
     "zb_vcorr_workspace": ([ctypes.POINTER(VTable), i32, i32], i64),
     "zb_vcorr_correlate": ([ctypes.POINTER(VTable), vp, vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_vcorr_denominators": ([ctypes.POINTER(VTable), ctypes.POINTER(VTable), vp, vp, i32, vp, i64, vp, vp], i32),

@@ -68,6 +68,7 @@ constexpr int kItemPad = 128;
 
 struct VArg {
     int L, n, log2n, nc, log2nc, nb, slots, cols, l_step, pairs, tiles, n_items, ni_pad;
+    int lp_gs;  // 16-bit products: the first group of 8 slots that takes the hi + lo product
 };
 
 __device__ __forceinline__ int xcd_block(int id, int G) {
@@ -263,6 +264,205 @@ __global__ void __launch_bounds__(64 * kWaves) k_vgemm(VArg a, const float* __re
     }
     constexpr int F4MAX = (S / 4) * NBLK * 64 + 1;  // one j's rows and a spare slot
     vgemm_bucket<MB, NBLK, S>(min(s0, S - 1) >> 2, a, mp, tile, cg, tab, areg, v, vlds, vlds + F4MAX);
+}
+
+// The GEMM with 16-bit products (FP16 or BF16 in, FP32 accumulate):
+// v_mfma_f32_32x32x16_{f16,bf16}, whose 16 k per lane pair of halves are here the (re, im)
+// rows of 8 degree slots, re in lanes 0..31 and im in lanes 32..63 as in the FP32 form. A value
+// x is carried as hi = round16(x) and lo = round16(x - hi); groups of 8 slots from lp_gs on take
+// hi hi + hi lo + lo hi (three products, the error of a ~22-bit (FP16) or ~16-bit (BF16)
+// operand), the ones below it hi hi alone. The table (zb_vtable.qlp): per (j, m', column group)
+// the groups in order, each its hi pieces [nblk][64 lanes][8 slots] of 16 bytes, then (from
+// lp_gs on) its lo pieces. V is FP32, as in the FP32 form.
+template <bool BF>
+struct Lp16;
+template <>
+struct Lp16<false> {
+    typedef _Float16 T;
+    typedef _Float16 V8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ floatx16 mfma(V8 a, V8 b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Lp16<true> {
+    typedef __bf16 T;
+    typedef __bf16 V8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ floatx16 mfma(V8 a, V8 b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+};
+
+template <int MB, int NBLK, int G8, int GS, int G0, bool BF>
+__device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int cg, const floatx4* __restrict__ tab,
+                                            const typename Lp16<BF>::V8 (&ah)[MB][G8], const typename Lp16<BF>::V8 (&al)[MB][G8 - GS + 1],
+                                            float* __restrict__ v, floatx4* lds0, floatx4* lds1) {
+    typedef typename Lp16<BF>::V8 V8;
+    constexpr int NQ = G8 - G0;                        // groups of 8 slots from G0
+    constexpr int PB = NBLK * 64;                      // 16-byte pieces of a group's hi (or lo) rows
+    // pieces before group g in one (j, m', column group) block
+    constexpr auto off = [](int g) { return (g + (g > GS ? g - GS : 0)) * PB; };
+    constexpr int F4 = off(G8) - off(G0);
+    constexpr int PER = (F4 + 64 * kWaves - 1) / (64 * kWaves);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int r32 = lane & 31;
+    const int half = lane >> 5;
+    const int ncg = a.cols / (NBLK * 32);
+    const size_t blk = (size_t)off(G8);
+    const size_t first = ((size_t)mp * ncg + cg) * blk + (size_t)off(G0);
+    const size_t total = (size_t)a.nb * a.L * ncg * blk;
+    const auto trs = buf(tab + first, (long long)(total - first) * 16);
+    const unsigned jstride = (unsigned)((size_t)a.L * ncg * blk * 16);
+    floatx4 pre[PER];
+    auto stage_load = [&](int j) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int idx = min(tid + 64 * kWaves * p, F4 - 1);
+            pre[p] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(trs, idx * 16, (int)((unsigned)j * jstride), 0));
+        }
+    };
+    auto stage_store = [&](floatx4* dst) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int idx = tid + 64 * kWaves * p;
+            dst[idx < F4 ? idx : F4] = pre[p];  // past the rows: the spare slot after them
+        }
+    };
+    const int w = a.cols;
+    const int c0 = cg * NBLK * 32;
+    const size_t vrow = (size_t)a.ni_pad * w;
+    unsigned loff[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + half * 4) * w + c0 + r32);
+    typedef floatx16 Acc[MB][NBLK][1];
+    auto step = [&](int j, Acc& acc, Acc& other, const floatx4* cur, floatx4* nxt) {
+        stage_load(min(j + 1, a.nb - 1));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[mb][nb][0][r] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int g = G0 + k;
+            const floatx4* rows = cur + (off(g) - off(G0));
+            V8 bh[NBLK];
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb) bh[nb] = __builtin_bit_cast(V8, rows[nb * 64 + lane]);
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb][0] = Lp16<BF>::mfma(ah[mb][g], bh[nb], acc[mb][nb][0]);
+            if (g >= GS) {  // known once unrolled
+                V8 bl[NBLK];
+#pragma unroll
+                for (int nb = 0; nb < NBLK; ++nb) bl[nb] = __builtin_bit_cast(V8, rows[PB + nb * 64 + lane]);
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                    for (int nb = 0; nb < NBLK; ++nb) {
+                        acc[mb][nb][0] = Lp16<BF>::mfma(ah[mb][g], bl[nb], acc[mb][nb][0]);
+                        acc[mb][nb][0] = Lp16<BF>::mfma(al[mb][g >= GS ? g - GS : 0], bh[nb], acc[mb][nb][0]);
+                    }
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < NBLK; ++nb)
+                asm volatile("" ::"a"(other[mb][nb][0]));
+        __builtin_amdgcn_sched_barrier(0);
+        const auto vrs = buf(v + ((size_t)j * a.L + mp) * vrow, (long long)vrow * 4);
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    const floatx16& x = acc[mb][nb][0];
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x[r]), vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w + nb * 32) * 4), 0);
+                }
+        stage_store(nxt);
+        __syncthreads();
+    };
+    stage_load(0);
+    stage_store(lds0);
+    __syncthreads();
+    Acc acc0, acc1;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc1[mb][nb][0][r] = 0.f;
+    for (int j = 0; j < a.nb; j += 2) {
+        step(j, acc0, acc1, lds0, lds1);
+        step(min(j + 1, a.nb - 1), acc1, acc0, lds1, lds0);
+    }
+}
+
+template <int MB, int NBLK, int G8, int GS, bool BF, int G0 = 0>
+__device__ __forceinline__ void vgemm16_bucket(int bucket, const VArg& a, int mp, int tile, int cg, const floatx4* __restrict__ tab,
+                                               const typename Lp16<BF>::V8 (&ah)[MB][G8], const typename Lp16<BF>::V8 (&al)[MB][G8 - GS + 1],
+                                               float* __restrict__ v, floatx4* lds0, floatx4* lds1) {
+    if constexpr (G0 < G8) {
+        if (bucket == G0) vgemm16_run<MB, NBLK, G8, GS, G0, BF>(a, mp, tile, cg, tab, ah, al, v, lds0, lds1);
+        else vgemm16_bucket<MB, NBLK, G8, GS, BF, G0 + 1>(bucket, a, mp, tile, cg, tab, ah, al, v, lds0, lds1);
+    }
+}
+
+// k_vgemm with 16-bit products: the same blocks, item tiles and V; G8 groups of 8 slots, those
+// from GS on with hi + lo products.
+template <int MB, int NBLK, int G8, int GS, bool BF>
+__global__ void __launch_bounds__(64 * kWaves) k_vgemm16(VArg a, const floatx4* __restrict__ tab, const float2* __restrict__ coef, float* __restrict__ v) {
+    typedef typename Lp16<BF>::T T;
+    typedef typename Lp16<BF>::V8 V8;
+    extern __shared__ floatx4 vlds[];
+    const int ncg = a.cols / (NBLK * 32);
+    const int per_m = a.tiles * ncg;
+    int mp, rem;
+    if ((a.L & 7) == 0) {
+        const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+        const int mi = k / per_m;
+        rem = k - mi * per_m;
+        mp = 8 * mi + ((mi & 1) ? 7 - x : x);
+    } else {
+        const int logical = xcd_block(blockIdx.x, gridDim.x);
+        mp = logical / per_m;
+        rem = logical - mp * per_m;
+    }
+    const int cg = rem / a.tiles;
+    const int tile = (rem - cg * a.tiles) * kWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int r32 = lane & 31;
+    const int s0 = (mp + a.l_step - 1) / a.l_step;
+    const int pbase = mp * a.L - (mp * (mp - 1)) / 2 - mp;
+    // A: row = item, the 8 k of this lane half = component `half` of slots 8 g .. 8 g + 7, as
+    // hi and lo; zero outside [s0, slots) and past the last item
+    V8 ah[MB][G8], al[MB][G8 - GS + 1];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const int item = (tile * MB + mb) * 32 + r32;
+        const bool ok = item < a.n_items;
+#pragma unroll
+        for (int g = 0; g < G8; ++g) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int s = 8 * g + i;
+                float x = 0.f;
+                if (ok && s >= s0 && s < a.slots) {
+                    const float2 c = coef[(size_t)item * a.pairs + pbase + s * a.l_step];
+                    x = half ? c.y : c.x;
+                }
+                const T h = (T)x;
+                ah[mb][g][i] = h;
+                if (g >= GS) al[mb][g >= GS ? g - GS : 0][i] = (T)(x - (float)h);
+            }
+        }
+    }
+    // two buffers of one j's rows from group 0 (hi, and lo from GS on) and a spare slot
+    constexpr int pieces = (2 * G8 - GS) * NBLK * 64 + 1;
+    vgemm16_bucket<MB, NBLK, G8, GS, BF>(min(s0, 8 * G8 - 1) >> 3, a, mp, tile, cg, tab, ah, al, v, vlds, vlds + pieces);
 }
 
 __device__ __forceinline__ unsigned ord_bits(float v) {
@@ -711,6 +911,11 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
         set_error("%s: %d degree slots: at most 64", who, t->slots);
         return ZB_EINVAL;
     }
+    if (t->lp_kind && (t->lp_kind < 1 || t->lp_kind > 2 || !t->qlp || t->lp_split < 0 || t->lp_split % 16)) {
+        set_error("%s: bad 16-bit table (kind %d, split slot %d; kinds 1 FP16, 2 BF16, the split a multiple of 16)", who, t->lp_kind, t->lp_split);
+        return ZB_EINVAL;
+    }
+    a.lp_gs = t->lp_kind ? t->lp_split / 8 : 0;
     a.L = t->L;
     a.n = t->n;
     a.log2n = ilog2v(t->n);
@@ -766,10 +971,70 @@ static void launch_vgemm_t(const VArg& a, const float* q, const float* alm, floa
     ZB_LAUNCH("k_vgemm", (k_vgemm<MB, NBLK, S>), dim3((unsigned)(tiles * a.L * ncg)), dim3(64 * kWaves), lds, s, b, q, (const float2*)alm, v);
 }
 
+template <int NBLK, int G8, int GS, bool BF>
+static void launch_vgemm16_t(const VArg& a, const void* q, const float* alm, float* v, hipStream_t s) {
+    const int tiles = (a.n_items + kWaves * 32 - 1) / (kWaves * 32);
+    VArg b = a;
+    b.tiles = tiles;
+    const int ncg = a.cols / (NBLK * 32);
+    const size_t lds = 2 * ((size_t)(2 * G8 - GS) * NBLK * 64 + 1) * 16;
+    ZB_LAUNCH("k_vgemm", (k_vgemm16<1, NBLK, G8, GS, BF>), dim3((unsigned)(tiles * a.L * ncg)), dim3(64 * kWaves), lds, s, b, (const floatx4*)q,
+              (const float2*)alm, v);
+}
+
+// The 16-bit table's tiling: G8 groups of 8 slots (an even count), nblk column blocks of 32.
+static void vgemm16_shape(int nc, int slots, int& cols, int& g8, int& nblk) {
+    cols = 32 * ((2 * nc + 31) / 32);
+    g8 = 2 * ((slots + 15) / 16);
+    nblk = cols == 32 ? 1 : 2;
+}
+
+template <int NBLK, int G8, bool BF, int GS = 0>
+static void launch_vgemm16_gs(const VArg& a, const void* q, const float* alm, float* v, hipStream_t s) {
+    if constexpr (GS < G8) {
+        if (a.lp_gs > GS) return launch_vgemm16_gs<NBLK, G8, BF, GS + 2>(a, q, alm, v, s);
+    }
+    launch_vgemm16_t<NBLK, G8, (GS < G8 ? GS : G8), BF>(a, q, alm, v, s);
+}
+
+template <int NBLK, bool BF>
+static void launch_vgemm16_g(int g8, const VArg& a, const void* q, const float* alm, float* v, hipStream_t s) {
+    if (g8 == 2) launch_vgemm16_gs<NBLK, 2, BF>(a, q, alm, v, s);
+    else if (g8 == 4) launch_vgemm16_gs<NBLK, 4, BF>(a, q, alm, v, s);
+    else if (g8 == 6) launch_vgemm16_gs<NBLK, 6, BF>(a, q, alm, v, s);
+    else launch_vgemm16_gs<NBLK, 8, BF>(a, q, alm, v, s);
+}
+
+static int launch_vgemm16(const VArg& a, int kind, const void* q, const float* alm, float* v, hipStream_t s) {
+    int cols, g8, nblk;
+    vgemm16_shape(a.nc, a.slots, cols, g8, nblk);
+    if (nblk == 1) {
+        if (kind == 2) launch_vgemm16_g<1, true>(g8, a, q, alm, v, s);
+        else launch_vgemm16_g<1, false>(g8, a, q, alm, v, s);
+    } else {
+        if (kind == 2) launch_vgemm16_g<2, true>(g8, a, q, alm, v, s);
+        else launch_vgemm16_g<2, false>(g8, a, q, alm, v, s);
+    }
+    return check_launch("k_vgemm");
+}
+
+extern "C" int32_t zb_vtable16_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* groups, int32_t* nblk) {
+    if (nc < 2 || nc > 64 || (nc & (nc - 1)) || slots < 1 || slots > 64 || !cols || !groups || !nblk) {
+        set_error("zb_vtable16_shape: nc %d, slots %d", nc, slots);
+        return ZB_EINVAL;
+    }
+    int c, g, b;
+    vgemm16_shape(nc, slots, c, g, b);
+    *cols = c;
+    *groups = g;
+    *nblk = b;
+    return ZB_OK;
+}
+
 // The choice of (S, NBLK) fixes the table's tiling (zb_vtable_shape).
 static void vgemm_shape(int nc, int slots, int& cols, int& S, int& nblk) {
     cols = 32 * ((2 * nc + 31) / 32);
-    S = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : 64;
+    S = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : slots <= 48 ? 48 : 64;
     nblk = (cols == 32 || S == 64) ? 1 : 2;
 }
 
@@ -782,9 +1047,9 @@ static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* 
     if (S == 64) {
         ZB_VG(1, 64);
     } else if (nblk == 1) {
-        if (S == 8) ZB_VG(1, 8); else if (S == 16) ZB_VG(1, 16); else ZB_VG(1, 32);
+        if (S == 8) ZB_VG(1, 8); else if (S == 16) ZB_VG(1, 16); else if (S == 32) ZB_VG(1, 32); else ZB_VG(1, 48);
     } else {
-        if (S == 8) ZB_VG(2, 8); else if (S == 16) ZB_VG(2, 16); else ZB_VG(2, 32);
+        if (S == 8) ZB_VG(2, 8); else if (S == 16) ZB_VG(2, 16); else if (S == 32) ZB_VG(2, 32); else ZB_VG(2, 48);
     }
 #undef ZB_VG
     return check_launch("k_vgemm");
@@ -862,7 +1127,7 @@ extern "C" int32_t zb_vcorr_correlate(const zb_vtable* t, const float* twiddle, 
     float* v = (float*)work;
     unsigned long long* best = (unsigned long long*)((char*)work + al256(v_bytes(a)));
     ZB_LAUNCH("k_vclear", k_vclear, dim3((n_items + 255) / 256), dim3(256), 0, s, best, n_items);
-    if (int rc = launch_vgemm(a, t->q, alm, v, s)) return rc;
+    if (int rc = t->lp_kind ? launch_vgemm16(a, t->lp_kind, t->qlp, alm, v, s) : launch_vgemm(a, t->q, alm, v, s)) return rc;
     if (int rc = launch_apeak<0>(a, v, rden, rden_div, best, nullptr, s)) return rc;
     ZB_LAUNCH("k_vfinish", k_vfinish, dim3(n_items), dim3(64), 0, s, a, (const float2*)v, (const float2*)twiddle, rden, rden_div,
               (const unsigned long long*)best, peaks);

@@ -71,18 +71,34 @@ class DevicePhase:
         self.band = band
         self.dev = dev
         self._vt = None
+        self._vt16 = {}
         self.vrden = {}
 
     def vtables(self):
         """The GEMM tables of the master and of its square (zb_vtable), built on first use."""
         if self._vt is None:
             out = []
+            self._vth = []
             for coef in (self.host.hm.f, self.host.hm.f2):
                 t = tables_mod.VTableHost(self.band.host, self.host, coef)
                 q = _dev(t.tiled, self.dev)
                 out.append((nat.VTable(t.L, t.n, t.nc, t.l_step, t.slots, t.cols, q.data_ptr()), q))
+                self._vth.append(t)
             self._vt = out
         return self._vt
+
+    def vtable16(self, kind, split_degree):
+        """The master's GEMM table with 16-bit products (kind 1 FP16, 2 BF16; hi + lo products for
+        degrees from split_degree on), built on first use."""
+        key = (kind, split_degree)
+        if key not in self._vt16:
+            (vt, q), _ = self.vtables()
+            t = self._vth[0]
+            split = tables_mod.split_slot(split_degree, t.l_step)
+            q16 = torch.from_numpy(t.table16(kind, split).view(np.int16)).to(self.dev)
+            v16 = nat.VTable(t.L, t.n, t.nc, t.l_step, t.slots, t.cols, q.data_ptr(), kind, split, q16.data_ptr())
+            self._vt16[key] = (v16, q, q16)
+        return self._vt16[key][0]
 
 
 def lattice(rows, cols, want):
@@ -357,6 +373,13 @@ class Engine:
             self._lanes.append(Lane(dev))
         return self._lanes[:k]
 
+    def corr_vtable(self, ph):
+        """The GEMM table the correlation multiplies with, at the precision the settings name."""
+        kind = {"fp32": 0, "fp16": 1, "bf16": 2}[self.settings["correlation_precision"]]
+        if kind == 0:
+            return ph.vtables()[0][0]
+        return ph.vtable16(kind, int(self.settings["split_degree"]))
+
     def use_mfma(self, L, pi=None):
         """The matrix-core correlation for phase pi at L (every phase when pi is None)."""
         if self.correlator != "mfma":
@@ -450,7 +473,7 @@ class Engine:
         # the correlation per phase: on the matrix cores where its shape allows, else the VALU form
         mfma = [self.use_mfma(L, pi) for pi in range(n_ph)]
         rdens = [self.vrden(pi, L) if mfma[pi] else self.rden(pi, L) for pi in range(n_ph)]
-        vts = [ph.vtables()[0][0] if mfma[pi] else None for pi, ph in enumerate(phs)]
+        vts = [self.corr_vtable(ph) if mfma[pi] else None for pi, ph in enumerate(phs)]
         corr_ws = [(lambda k, vt=vts[pi]: int(lib.zb_vcorr_workspace(vt, k, 0))) if mfma[pi] else
                    (lambda k, ph=ph: int(lib.zb_so3_workspace(band.c, ph.c, k))) for pi, ph in enumerate(phs)]
         sheets = [self.baked(pi)[1][0] for pi in range(n_ph)]

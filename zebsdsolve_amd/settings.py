@@ -31,6 +31,10 @@ DEFAULTS = {
     "refine": True, "refine_iters": 8, "refine_step": 2.0,
     # ── phases ──
     "phase_by": "nmi", "nmi_bins": 16,
+    # ── products of the correlation's GEMM (this build's own; the reference multiplies in
+    # float32): "fp32", or 16-bit "fp16" / "bf16" with float32 accumulation and hi + lo products
+    # for the degrees from split_degree on ──
+    "correlation_precision": "fp32", "split_degree": 64,
     # ── how the work is cut up ──
     "batch": 256, "tile_mb": 256, "refine_pixels": 64, "wg_fft": 64, "wg_refine": 64,
     # ── sweeps and acceptance ──
@@ -73,6 +77,8 @@ def check(s):
     bad(not (s["static_patterns"] >= 1), "static_patterns")
     bad(not (s["dynamic_sigma"] > 0), "dynamic_sigma")
     bad(not (8 <= s["sweep_bandwidth"] <= 256), "sweep_bandwidth")
+    bad(s["correlation_precision"] not in ("fp32", "fp16", "bf16"), "correlation_precision")
+    bad(not (0 <= s["split_degree"] <= 256), "split_degree")
     for k in ("wg_fft", "wg_refine"):
         bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
     for k, allowed in _ONLY.items():

@@ -117,12 +117,10 @@ def d_beta(L, fold, wigner, betas):
 def vgemm_shape(nc, slots):
     """(cols, S, nblk) of the matrix-core correlation for nc points along c and `slots` degree
     slots, by the rule its kernel launch follows (zb_vtable_shape in the C ABI): cols = 32
-    ceil(2 nc / 32) table columns, S = the power of two >= slots (at least 8) slots per wave,
+    ceil(2 nc / 32) table columns, S = the first of 8, 16, 32, 48, 64 >= slots slots per wave,
     nblk column blocks of 32 per wave (1 when cols = 32 or S = 64, else 2)."""
     cols = 32 * ((2 * nc + 31) // 32)
-    S = 8
-    while S < slots:
-        S *= 2
+    S = next(x for x in (8, 16, 32, 48, 64) if x >= slots)
     nblk = 1 if (cols == 32 or S == 64) else 2
     return cols, S, nblk
 
@@ -138,6 +136,52 @@ def tile_vtable(T, S, nblk):
     Tp[:, :, :slots] = T
     Tp = Tp.reshape(nb, L, S // 4, 4, 2, ncg, nblk, 32)
     return np.ascontiguousarray(np.transpose(Tp, (0, 1, 5, 2, 6, 4, 7, 3))).reshape(-1)
+
+
+def vgemm16_shape(nc, slots):
+    """(cols, groups, nblk) of the 16-bit table (zb_vtable16_shape in the C ABI): groups of 8
+    slots, an even count, 2 ceil(slots / 16); nblk 1 when cols = 32, else 2."""
+    cols = 32 * ((2 * nc + 31) // 32)
+    return cols, 2 * ((slots + 15) // 16), 1 if cols == 32 else 2
+
+
+def round16(x, kind):
+    """float64 values to FP16 (kind 1) or BF16 (kind 2), nearest with ties to even: (the 16-bit
+    patterns as uint16, their values as float64). BF16 rounds the float32 nearest to x."""
+    if kind == 1:
+        if np.abs(x).max(initial=0.0) > 65504.0:
+            raise ValueError("the table exceeds the FP16 range: use BF16")
+        h = x.astype(np.float16)
+        return h.view(np.uint16), h.astype(np.float64)
+    u = x.astype(np.float32).view(np.uint32)
+    r = ((u + np.uint32(0x7FFF) + ((u >> np.uint32(16)) & np.uint32(1))) >> np.uint32(16)).astype(np.uint16)
+    return r, (r.astype(np.uint32) << np.uint32(16)).view(np.float32).astype(np.float64)
+
+
+def split_slot(split_degree, l_step):
+    """First slot (a multiple of 16) of the 16-bit products with hi + lo: the group of 16 slots
+    holding degree split_degree, so every degree >= split_degree takes them."""
+    return ((split_degree + l_step - 1) // l_step) // 16 * 16
+
+
+def tile_vtable16(T, groups, nblk, split, kind):
+    """The natural table of one j, [L][slots][2][cols] float64, as zb_vtable.qlp's pieces for
+    that j: per (m', column group cg), the groups t of 8 slots in order, each [nblk][64 lanes][8]
+    hi values, lane = 32 r + x holding row r of column cg nblk 32 + b 32 + x for slots
+    8 t .. 8 t + 7; groups t >= split / 8 followed by the same for lo = round16(x - hi)."""
+    L, slots, _, cols = T.shape
+    ncg = cols // (32 * nblk)
+    Tp = np.zeros((L, 8 * groups, 2, cols))
+    Tp[:, :slots] = T
+    Tp = np.transpose(Tp.reshape(L, groups, 8, 2, ncg, nblk, 32), (0, 4, 1, 5, 3, 6, 2))  # [m'][cg][t][b][r][x][i]
+    hb, hv = round16(Tp, kind)
+    lb, _ = round16(Tp - hv, kind)
+    P = nblk * 64 * 8
+    H = hb.reshape(L, ncg, groups, P)
+    Lo = lb.reshape(L, ncg, groups, P)
+    gs = min(split // 8, groups)
+    both = np.stack((H[:, :, gs:], Lo[:, :, gs:]), axis=3).reshape(L, ncg, 2 * (groups - gs), P)
+    return np.concatenate((H[:, :, :gs], both), axis=2).reshape(-1)
 
 
 class VTableHost:
@@ -180,3 +224,19 @@ class VTableHost:
         self.table = T
         _, self.S, self.nblk = vgemm_shape(nc, self.slots)
         self.tiled = tile_vtable(T, self.S, self.nblk)
+
+    def table16(self, kind, split):
+        """zb_vtable.qlp for FP16 (kind 1) or BF16 (kind 2) with hi + lo products from slot
+        `split` on: the rows rebuilt in float64 from Q, one j at a time."""
+        _, groups, nblk = vgemm16_shape(self.nc, self.slots)
+        nc = self.nc
+        out = []
+        for Qj in self.Q:                                            # [m', c, s]
+            Qt = np.transpose(Qj, (0, 2, 1))                          # [m', s, c]
+            T = np.zeros((self.L, self.slots, 2, self.cols))
+            T[:, :, 0, 0:2 * nc:2] = Qt.real
+            T[:, :, 0, 1:2 * nc:2] = Qt.imag
+            T[:, :, 1, 0:2 * nc:2] = Qt.imag
+            T[:, :, 1, 1:2 * nc:2] = -Qt.real
+            out.append(tile_vtable16(T, groups, nblk, split, kind))
+        return np.concatenate(out)
