@@ -82,14 +82,16 @@ typedef struct zb_phase {
  * counterpart in the reference's tables; it replaces W_f of spherical.PhaseGpu
  * (src/spherical.zig:192-242) on this path.
  *
- * With lp_kind 1 (FP16) or 2 (BF16), zb_vcorr_correlate multiplies in 16 bits with FP32
- * accumulation, from qlp: the same rows with each value x as hi = round16(x) and
- * lo = round16(x - hi), tiled by (cols, G, nblk) from zb_vtable16_shape: per (j, m', column
- * group g < cols / (32 nblk)), the G groups of 8 slots in order, each [nblk][64][8] hi values,
- * element [b][32 r + x][i] = row r, slot 8t + i, column 32 (g nblk + b) + x, followed, for groups
- * t >= lp_split / 8, by the same for lo. Those groups take hi hi + hi lo + lo hi (the item's
- * coefficients split the same way), the others hi hi. Then q is used only by
- * zb_vcorr_denominators (the window's volumes stay FP32). lp_kind 0: FP32 products from q. */
+ * With lp_kind 1 (FP16) or 2 (BF16) (nc >= 32), zb_vcorr_correlate multiplies in 16 bits with
+ * FP32 accumulation and keeps V as pairs of the same 16-bit type (the transform along a, the
+ * peak and its neighbours in FP32), from qlp: the same rows with each value x as
+ * hi = round16(x) and lo = round16(x - hi), tiled by (cols, G, 2) from zb_vtable16_shape: per
+ * (j, m', column group g < nc / 32), the G groups of 8 slots in order, each [2][64][8] hi
+ * values, element [b][32 r + x][i] = row r, slot 8t + i, column 2 (32 g + x) + b (b = 0 the
+ * real, 1 the imaginary part of complex column 32 g + x), followed, for groups t >= lp_split / 8,
+ * by the same for lo. Those groups take hi hi + hi lo + lo hi (the item's coefficients split
+ * the same way), the others hi hi. Then q is used only by zb_vcorr_denominators (the window's
+ * volumes stay FP32). lp_kind 0: FP32 products and V from q. */
 typedef struct zb_vtable {
     int32_t L;
     int32_t n;      /* fft_size(L), at most 256 on this path */

@@ -108,8 +108,8 @@ def test_table_tiling_rule_is_the_librarys():
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
 def test_table16_tiling_rule_is_the_librarys():
     """The 16-bit table: the host's shape rule is the library's, each hi value sits where the
-    header says, the lo values follow their group's hi from the split group on, and hi + lo
-    gives back the value."""
+    header says (block 0 the real and block 1 the imaginary parts of 32 complex columns), the lo
+    values follow their group's hi from the split group on, and hi + lo gives back the value."""
     import numpy as np
     from zebsdsolve_amd import _native as nat, tables
     lib = nat.load()
@@ -135,8 +135,8 @@ def test_table16_tiling_rule_is_the_librarys():
               (lambda u: (u.astype(np.uint32) << np.uint32(16)).view(np.float32).astype(np.float64))
         for _ in range(300):
             m, s, r, col = rng.integers(L), rng.integers(8 * groups), rng.integers(2), rng.integers(cols)
-            g, rest = divmod(col, 32 * nblk)
-            bb, x = divmod(rest, 32)
+            c, bb = divmod(col, 2)  # complex column c, real (0) or imaginary (1) part
+            g, x = divmod(c, 32)
             t = s // 8
             pos = t if t < 4 else 4 + 2 * (t - 4)
             hi = val(blocks[m, g, pos, bb, 32 * r + x, s % 8])

@@ -1,10 +1,12 @@
-"""The correlation's GEMM with 16-bit products (FP16 and BF16 in, FP32 accumulate, hi + lo
-products for the degrees from split_degree on) against the FP32 GEMM and the float64 oracle,
-and the whole path at L = 96 against the truth.
+"""The correlation's 16-bit form (FP16 or BF16 products with FP32 accumulation, hi + lo products
+for the degrees from split_degree on, V kept as 16-bit pairs, the transform along a in FP32)
+against the FP32 form and the float64 oracle, and the whole path at L = 96 against the truth.
 
-Tolerances are stated at each assertion. What was measured on the MI355X at the time of
-writing (256-512 simulated nickel patterns, scores relative to the FP32 GEMM's): hi-only FP16
-products p99 5e-5, BF16 4.5e-4; hi + lo on every degree 3e-7 (FP16), 1.5e-6 (BF16).
+Tolerances are stated at each assertion. Measured on the MI355X at the time of writing (256-512
+simulated nickel and GaN patterns, peak scores relative to the FP32 form's): FP16 p99 9e-5, max
+1.4e-4; BF16 p99 8.3e-4, max 1.1e-3, whatever the split (the 16-bit V dominates: with FP32 V,
+hi + lo on every degree gave 3e-7 and 1.5e-6). Peaks on another cell than the FP32 form's
+(near-ties): up to 0.2 % of the items with FP16 and 2 % with BF16, at L = 64.
 """
 import numpy as np
 import pytest
@@ -62,20 +64,18 @@ def test_16bit_products_against_fp32(name, path, L, prec):
     ph = e.phase_band(0, L)
     rots = e.phases[0].master.rotations
     ref = _correlate(e, L, ph.vtables()[0][0], alm)
-    # hi + lo on every degree: FP32 accuracy (the scores to 1e-5, the same sub-cell peak)
-    pk = _correlate(e, L, ph.vtable16(KIND[prec], 0), alm)
-    rel = np.abs(pk[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])
-    assert rel.max() < 1e-5, rel.max()
-    d = np.degrees(sym.disorientation(pk[:, :4].astype(np.float64), ref[:, :4].astype(np.float64), rots))
-    assert d.max() < 0.002, d.max()
-    # the default split (hi + lo from l = 64): scores to 2e-4 (FP16) or 1.5e-3 (BF16) at the
-    # 99th percentile, the peak's orientation within 0.005 deg at the median, at most 2 % of the
-    # items on another cell (near-ties)
-    pk = _correlate(e, L, ph.vtable16(KIND[prec], 64), alm)
-    rel = np.abs(pk[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])
-    assert np.percentile(rel, 99) < (2e-4 if prec == "fp16" else 1.5e-3), np.percentile(rel, 99)
-    d = np.degrees(sym.disorientation(pk[:, :4].astype(np.float64), ref[:, :4].astype(np.float64), rots))
-    assert np.median(d) < 0.005 and np.mean(d > 0.1) <= 0.02, (np.median(d), np.mean(d > 0.1))
+    fp16 = prec == "fp16"
+    for split in (64, 0):  # the default (hi + lo from l = 64), and hi + lo on every degree
+        pk = _correlate(e, L, ph.vtable16(KIND[prec], split), alm)
+        # scores to 2.5e-4 (FP16) or 2e-3 (BF16) at the 99th percentile and twice that at most
+        rel = np.abs(pk[:, 4] - ref[:, 4]) / np.abs(ref[:, 4])
+        bound = 2.5e-4 if fp16 else 2e-3
+        assert np.percentile(rel, 99) < bound and rel.max() < 2 * bound, (split, np.percentile(rel, 99), rel.max())
+        # the peak's orientation within 0.002 deg (FP16) or 0.005 deg (BF16) at the median; at
+        # most 1 % (FP16) or 4 % (BF16) of the items on another cell (near-ties)
+        d = np.degrees(sym.disorientation(pk[:, :4].astype(np.float64), ref[:, :4].astype(np.float64), rots))
+        assert np.median(d) < (0.002 if fp16 else 0.005), (split, np.median(d))
+        assert np.mean(d > 0.1) <= (0.01 if fp16 else 0.04), (split, np.mean(d > 0.1))
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
@@ -96,13 +96,16 @@ def test_16bit_l96_against_the_oracle_and_truth(prec):
     rec32 = e.index_cells(np.arange(N), L)[0]
     e.settings.apply(correlation_precision=prec)
     rec = e.index_cells(np.arange(N), L)[0]
-    # the reference's thresholds against the truth (test_api.py:212), and the FP32 form's result
-    # to 0.01 deg on every pattern (the refinement starts from the same cell)
+    # the reference's thresholds against the truth (test_api.py:212), and the FP32 form's result:
+    # every orientation to 0.01 deg, the NCC to 1e-5 at the median and 2e-3 at most (eight
+    # Gauss-Newton iterations from a peak a little off the FP32 one can stop short of the same
+    # optimum on a noise-free pattern: 1.1e-3 was seen once with BF16)
     d = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), truth, rots))
     assert np.median(d) < 0.15 and d.max() < 1.0, d
     d32 = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), rec32[:, :4].astype(np.float64), rots))
     assert d32.max() < 0.01, d32
-    assert np.abs(rec[:, 4] - rec32[:, 4]).max() < 1e-4
+    dn = np.abs(rec[:, 4] - rec32[:, 4])
+    assert np.median(dn) < 1e-5 and dn.max() < 2e-3, dn
     # three peaks against the oracle
     lib = nat.load()
     band = e.band(L)
@@ -126,17 +129,17 @@ def test_16bit_l96_against_the_oracle_and_truth(prec):
     for b in range(3):
         ref_alm, _ = op.coefficients(pnh[b].astype(np.float64), fr, t, L)
         q, v0, _, _ = so.peak(so.volume(f, ref_alm), rden_ref)
-        # the score to 3e-4 (FP16) or 2e-3 (BF16), the orientation within 1/100 of a grid step
-        assert abs(pk[b, 4] - v0) < (3e-4 if prec == "fp16" else 2e-3) * abs(v0), (b, pk[b, 4], v0)
+        # the score to 5e-4 (FP16) or 3e-3 (BF16), the orientation within 1/100 of a grid step
+        assert abs(pk[b, 4] - v0) < (5e-4 if prec == "fp16" else 3e-3) * abs(v0), (b, pk[b, 4], v0)
         assert np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), q, rots)) < 0.014, b
 
 
 def test_16bit_table_rejects_what_the_kernel_cannot_take():
-    """A 16-bit table with a split off the 16-slot grid, an unknown kind, or no table pointer is
-    refused before anything is launched."""
+    """A 16-bit table with a split off the 16-slot grid, an unknown kind, no table pointer or
+    fewer than 32 points along c is refused before anything is launched."""
     from zebsdsolve_amd import _native as nat
     lib = nat.load()
-    for kind, split, ptr in ((1, 8, 4096), (3, 0, 4096), (1, 0, None)):
-        vt = nat.VTable(64, 128, 32, 2, 32, 64, 4096, kind, split, ptr)
+    for L, n, nc, kind, split, ptr in ((64, 128, 32, 1, 8, 4096), (64, 128, 32, 3, 0, 4096), (64, 128, 32, 1, 0, None), (32, 64, 16, 1, 0, 4096)):
+        vt = nat.VTable(L, n, nc, 2, L // 2, 2 * nc, 4096, kind, split, ptr)
         assert lib.zb_vcorr_workspace(vt, 4, 0) < 0
         assert "16-bit table" in nat.last_error()

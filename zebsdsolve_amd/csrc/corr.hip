@@ -69,6 +69,49 @@ constexpr int kItemPad = 128;
 struct VArg {
     int L, n, log2n, nc, log2nc, nb, slots, cols, l_step, pairs, tiles, n_items, ni_pad;
     int lp_gs;  // 16-bit products: the first group of 8 slots that takes the hi + lo product
+    int vk;     // V's elements: 0 FP32 pairs (cols floats a row), 1 FP16 / 2 BF16 pairs (nc a row)
+};
+
+// V's complex elements: FP32 (re, im) (8 bytes) or FP16 / BF16 pairs packed in 32 bits
+template <int VT>
+struct VEl {
+    static constexpr unsigned bytes = VT ? 4u : 8u;
+    static __device__ __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        if constexpr (VT == 0) {
+            return buf_f32x2(r, voff, soff);
+        } else {
+            return unpack(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+        }
+    }
+    static __device__ __forceinline__ float2 at(const void* v, size_t i) {
+        if constexpr (VT == 0) return reinterpret_cast<const float2*>(v)[i];
+        else return unpack(reinterpret_cast<const unsigned*>(v)[i]);
+    }
+    static __device__ __forceinline__ float2 unpack(unsigned u) {
+        if constexpr (VT == 1) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            const h2 h = __builtin_bit_cast(h2, u);
+            return make_float2((float)h[0], (float)h[1]);
+        } else {
+            return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+        }
+    }
+    // (re, im) rounded to nearest, ties to even
+    static __device__ __forceinline__ unsigned pack(float re, float im) {
+        if constexpr (VT == 1) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            h2 h;
+            h[0] = (_Float16)re;
+            h[1] = (_Float16)im;
+            return __builtin_bit_cast(unsigned, h);
+        } else {
+            typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+            b2 h;
+            h[0] = (__bf16)re;
+            h[1] = (__bf16)im;
+            return __builtin_bit_cast(unsigned, h);
+        }
+    }
 };
 
 __device__ __forceinline__ int xcd_block(int id, int G) {
@@ -292,7 +335,8 @@ struct Lp16<true> {
 template <int MB, int NBLK, int G8, int GS, int G0, bool BF>
 __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int cg, const floatx4* __restrict__ tab,
                                             const typename Lp16<BF>::V8 (&ah)[MB][G8], const typename Lp16<BF>::V8 (&al)[MB][G8 - GS + 1],
-                                            float* __restrict__ v, floatx4* lds0, floatx4* lds1) {
+                                            unsigned* __restrict__ v, floatx4* lds0, floatx4* lds1) {
+    static_assert(NBLK == 2, "the 16-bit form pairs the real and imaginary column blocks");
     typedef typename Lp16<BF>::V8 V8;
     constexpr int NQ = G8 - G0;                        // groups of 8 slots from G0
     constexpr int PB = NBLK * 64;                      // 16-byte pieces of a group's hi (or lo) rows
@@ -325,12 +369,12 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
             dst[idx < F4 ? idx : F4] = pre[p];  // past the rows: the spare slot after them
         }
     };
-    const int w = a.cols;
-    const int c0 = cg * NBLK * 32;
+    // V row of one (j, m'): ni_pad items x nc 32-bit (re, im) pairs; the lane's complex column
+    const int w = a.nc;
     const size_t vrow = (size_t)a.ni_pad * w;
     unsigned loff[MB];
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + half * 4) * w + c0 + r32);
+    for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + half * 4) * w + cg * 32 + r32);
     typedef floatx16 Acc[MB][NBLK][1];
     auto step = [&](int j, Acc& acc, Acc& other, const floatx4* cur, floatx4* nxt) {
         stage_load(min(j + 1, a.nb - 1));
@@ -371,16 +415,17 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
             for (int nb = 0; nb < NBLK; ++nb)
                 asm volatile("" ::"a"(other[mb][nb][0]));
         __builtin_amdgcn_sched_barrier(0);
+        // C[item][column]: block 0 the real parts and block 1 the imaginary parts of complex
+        // column cg 32 + r32, packed as one 32-bit pair: each store writes two items' whole
+        // 128-byte runs of 32 pairs
         const auto vrs = buf(v + ((size_t)j * a.L + mp) * vrow, (long long)vrow * 4);
 #pragma unroll
-        for (int nb = 0; nb < NBLK; ++nb)
+        for (int r = 0; r < 16; ++r)
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb) {
-                    const floatx16& x = acc[mb][nb][0];
-                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x[r]), vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w + nb * 32) * 4), 0);
-                }
+            for (int mb = 0; mb < MB; ++mb) {
+                const unsigned u = VEl<BF ? 2 : 1>::pack(acc[mb][0][0][r], acc[mb][1][0][r]);
+                __builtin_amdgcn_raw_buffer_store_b32(u, vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w) * 4), 0);
+            }
         stage_store(nxt);
         __syncthreads();
     };
@@ -403,7 +448,7 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
 template <int MB, int NBLK, int G8, int GS, bool BF, int G0 = 0>
 __device__ __forceinline__ void vgemm16_bucket(int bucket, const VArg& a, int mp, int tile, int cg, const floatx4* __restrict__ tab,
                                                const typename Lp16<BF>::V8 (&ah)[MB][G8], const typename Lp16<BF>::V8 (&al)[MB][G8 - GS + 1],
-                                               float* __restrict__ v, floatx4* lds0, floatx4* lds1) {
+                                               unsigned* __restrict__ v, floatx4* lds0, floatx4* lds1) {
     if constexpr (G0 < G8) {
         if (bucket == G0) vgemm16_run<MB, NBLK, G8, GS, G0, BF>(a, mp, tile, cg, tab, ah, al, v, lds0, lds1);
         else vgemm16_bucket<MB, NBLK, G8, GS, BF, G0 + 1>(bucket, a, mp, tile, cg, tab, ah, al, v, lds0, lds1);
@@ -413,7 +458,7 @@ __device__ __forceinline__ void vgemm16_bucket(int bucket, const VArg& a, int mp
 // k_vgemm with 16-bit products: the same blocks, item tiles and V; G8 groups of 8 slots, those
 // from GS on with hi + lo products.
 template <int MB, int NBLK, int G8, int GS, bool BF>
-__global__ void __launch_bounds__(64 * kWaves) k_vgemm16(VArg a, const floatx4* __restrict__ tab, const float2* __restrict__ coef, float* __restrict__ v) {
+__global__ void __launch_bounds__(64 * kWaves) k_vgemm16(VArg a, const floatx4* __restrict__ tab, const float2* __restrict__ coef, unsigned* __restrict__ v) {
     typedef typename Lp16<BF>::T T;
     typedef typename Lp16<BF>::V8 V8;
     extern __shared__ floatx4 vlds[];
@@ -508,8 +553,8 @@ __device__ __forceinline__ float2 zterm(float2 x, float2 yc, int k) {
 
 // ONEVOL: every block's items share one denominator volume (one volume for the call, or volumes
 // of whole blocks), whose columns for the block's j then go through LDS.
-template <int H, int LH, int MODE, bool ONEVOL>
-__global__ void __launch_bounds__(256) k_apeak(VArg a, const float2* __restrict__ v, const float* __restrict__ rden, int rden_div,
+template <int H, int LH, int MODE, bool ONEVOL, int VT = 0>
+__global__ void __launch_bounds__(256) k_apeak(VArg a, const void* __restrict__ v, const float* __restrict__ rden, int rden_div,
                                                unsigned long long* __restrict__ best, float2* __restrict__ vol) {
     // MODE 0 with one volume: the denominators of the block's j, float4 slot (a4 nc + c) holding
     // column c's values 4 a4 .. 4 a4 + 3 (adjacent columns in adjacent slots for the reads)
@@ -546,13 +591,23 @@ __global__ void __launch_bounds__(256) k_apeak(VArg a, const float2* __restrict_
         }
     }
     __builtin_amdgcn_sched_barrier(0);
-    const int wc = a.cols / 2;                              // complex pairs per item row
-    const unsigned rs = (unsigned)(a.ni_pad * wc) * 8u;    // bytes per m' row
-    const auto vrs = buf(v + (size_t)j * a.L * a.ni_pad * wc, (long long)a.L * rs);
-    const unsigned vl = (unsigned)(b * wc + c) * 8u;
+    const int wc = VT ? a.nc : a.cols / 2;                  // complex pairs per item row
+    constexpr unsigned EB = VEl<VT>::bytes;
+    const unsigned rs = (unsigned)(a.ni_pad * wc) * EB;     // bytes per m' row
+    const auto vrs = buf((const char*)v + (size_t)j * a.L * rs, (long long)a.L * rs);
+    const unsigned vl = (unsigned)(b * wc + c) * EB;
     float2 z[H];
 #pragma unroll
-    for (int q = 0; q < H; ++q) z[q] = q < a.L ? buf_f32x2(vrs, vl, (unsigned)q * rs) : make_float2(0.f, 0.f);
+    for (int q = 0; q < H; ++q) {
+        if constexpr (VT == 0) {
+            z[q] = q < a.L ? VEl<VT>::load(vrs, vl, (unsigned)q * rs) : make_float2(0.f, 0.f);
+        } else {
+            // no branch around a load whose value is converted (the conversion would wait for
+            // it there): rows q >= L lie past the resource's L rows and load as zeros, the whole
+            // offset in the per-lane register, which the bounds check covers
+            z[q] = VEl<VT>::load(vrs, vl + (unsigned)q * rs, 0u);
+        }
+    }
     // the real transform along a (C = Re V_0 + 2 Re sum_{m' > 0} V_m' e^{-i m' a}) from a complex
     // one of half its length, built in place pairwise (q, h - q); X_0 is taken real
     constexpr int KS = 64 / H;  // w_q = exp(-2 pi i q / n) = exp(-2 pi i (q KS) / 128)
@@ -652,8 +707,8 @@ __device__ __forceinline__ float2 tw256c(int k) { return make_float2(kTw256Cos[k
 // search. One wave per (j, 32 columns, item), four items a block on one j and column group,
 // whose 32 denominator columns go straight into LDS ahead of the spectrum loads (ONEVOL).
 // MODE 0: the peak search of k_apeak; MODE 1: the volume, natural order.
-template <int MODE, bool ONEVOL>
-__global__ void __launch_bounds__(256, 2) k_apeak128(VArg a, const float2* __restrict__ v, const float* __restrict__ rden, int rden_div,
+template <int MODE, bool ONEVOL, int VT = 0>
+__global__ void __launch_bounds__(256, 2) k_apeak128(VArg a, const void* __restrict__ v, const float* __restrict__ rden, int rden_div,
                                                   unsigned long long* __restrict__ best, float2* __restrict__ vol) {
     constexpr int Q = 64, N = 256;
     extern __shared__ float4 rlds[];
@@ -685,15 +740,16 @@ __global__ void __launch_bounds__(256, 2) k_apeak128(VArg a, const float2* __res
         }
     }
     __builtin_amdgcn_sched_barrier(0);
-    const int wc = a.cols / 2;
-    const unsigned rs = (unsigned)(a.ni_pad * wc) * 8u;
-    const auto vrs = buf(v + (size_t)j * a.L * a.ni_pad * wc, (long long)a.L * rs);
+    const int wc = VT ? a.nc : a.cols / 2;
+    constexpr unsigned EB = VEl<VT>::bytes;
+    const unsigned rs = (unsigned)(a.ni_pad * wc) * EB;
+    const auto vrs = buf((const char*)v + (size_t)j * a.L * rs, (long long)a.L * rs);
     // orders q >= L lie past the resource's L rows and load as zeros (no lane-divergent branch);
     // the whole offset goes in the per-lane register, which the bounds check covers
-    const unsigned vl = (unsigned)((live ? b : 0) * wc + c) * 8u + (hf ? rs : 0u);
+    const unsigned vl = (unsigned)((live ? b : 0) * wc + c) * EB + (hf ? rs : 0u);
     float2 z[Q];
 #pragma unroll
-    for (int r = 0; r < Q; ++r) z[r] = buf_f32x2(vrs, vl + (unsigned)r * 2u * rs, 0u);
+    for (int r = 0; r < Q; ++r) z[r] = VEl<VT>::load(vrs, vl + (unsigned)r * 2u * rs, 0u);
     // the half-length input: with s = X_q + conj X_{128-q}, d = X_q - conj X_{128-q} and
     // t = w_q d (w_q = exp(-2 pi i q / 256)), Z_q = s + i t and Z_{128-q} = conj s + i conj t.
     // X_0 is taken real (X_128 = 0 for L <= 128); order 64 (hf 0, r 32) pairs with itself.
@@ -812,7 +868,8 @@ __device__ __forceinline__ float vertex_v(float lo, float mid, float hi) {
     return fminf(fmaxf(0.5f * (lo - hi) / den, -0.5f), 0.5f);
 }
 
-__global__ void __launch_bounds__(64) k_vfinish(VArg a, const float2* __restrict__ v, const float2* __restrict__ tw, const float* __restrict__ rden,
+template <int VT>
+__global__ void __launch_bounds__(64) k_vfinish(VArg a, const void* __restrict__ v, const float2* __restrict__ tw, const float* __restrict__ rden,
                                                 int rden_div, const unsigned long long* __restrict__ best, float* __restrict__ peaks) {
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
@@ -836,8 +893,8 @@ __global__ void __launch_bounds__(64) k_vfinish(VArg a, const float2* __restrict
     const int iam = (ia + n - 1) & (n - 1), iap = (ia + 1) & (n - 1);
     const int ts = ZB_TW / n;
     float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const int wc = a.cols / 2;
-    auto V = [&](int jj, int mp, int cc) { return v[(((size_t)jj * a.L + mp) * a.ni_pad + b) * wc + cc]; };
+    const int wc = VT ? a.nc : a.cols / 2;
+    auto V = [&](int jj, int mp, int cc) { return VEl<VT>::at(v, (((size_t)jj * a.L + mp) * a.ni_pad + b) * wc + cc); };
     auto term = [&](float2 x, int mp, int at) {
         const float2 e = tw[((mp * at) & (n - 1)) * ts];
         return x.x * e.x - x.y * e.y;
@@ -911,11 +968,13 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
         set_error("%s: %d degree slots: at most 64", who, t->slots);
         return ZB_EINVAL;
     }
-    if (t->lp_kind && (t->lp_kind < 1 || t->lp_kind > 2 || !t->qlp || t->lp_split < 0 || t->lp_split % 16)) {
-        set_error("%s: bad 16-bit table (kind %d, split slot %d; kinds 1 FP16, 2 BF16, the split a multiple of 16)", who, t->lp_kind, t->lp_split);
+    if (t->lp_kind && (t->lp_kind < 1 || t->lp_kind > 2 || !t->qlp || t->lp_split < 0 || t->lp_split % 16 || t->nc < 32)) {
+        set_error("%s: bad 16-bit table (kind %d, split slot %d, nc %d; kinds 1 FP16, 2 BF16, the split a multiple of 16, nc >= 32)", who,
+                  t->lp_kind, t->lp_split, t->nc);
         return ZB_EINVAL;
     }
     a.lp_gs = t->lp_kind ? t->lp_split / 8 : 0;
+    a.vk = t->lp_kind;
     a.L = t->L;
     a.n = t->n;
     a.log2n = ilog2v(t->n);
@@ -944,7 +1003,7 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
 
 static int64_t al256(int64_t v) { return (v + 255) / 256 * 256; }
 // V for the padded items: cols floats per item row (complex pairs, then the table's zero columns)
-static int64_t v_bytes(const VArg& a) { return (int64_t)a.ni_pad * a.nb * a.L * a.cols * 4; }
+static int64_t v_bytes(const VArg& a) { return (int64_t)a.ni_pad * a.nb * a.L * (a.vk ? 2 * a.nc : a.cols) * (a.vk ? 2 : 4); }
 // V and the per-item best; the denominators add two volumes
 static int64_t vcorr_ws(const VArg& a, int volumes) {
     return al256(v_bytes(a)) + al256((int64_t)a.n_items * 8) + (volumes ? 2 * al256((int64_t)a.n_items * a.nb * a.n * a.nc * 4) : 0);
@@ -972,14 +1031,14 @@ static void launch_vgemm_t(const VArg& a, const float* q, const float* alm, floa
 }
 
 template <int NBLK, int G8, int GS, bool BF>
-static void launch_vgemm16_t(const VArg& a, const void* q, const float* alm, float* v, hipStream_t s) {
+static void launch_vgemm16_t(const VArg& a, const void* q, const float* alm, void* v, hipStream_t s) {
     const int tiles = (a.n_items + kWaves * 32 - 1) / (kWaves * 32);
     VArg b = a;
     b.tiles = tiles;
     const int ncg = a.cols / (NBLK * 32);
     const size_t lds = 2 * ((size_t)(2 * G8 - GS) * NBLK * 64 + 1) * 16;
     ZB_LAUNCH("k_vgemm", (k_vgemm16<1, NBLK, G8, GS, BF>), dim3((unsigned)(tiles * a.L * ncg)), dim3(64 * kWaves), lds, s, b, (const floatx4*)q,
-              (const float2*)alm, v);
+              (const float2*)alm, (unsigned*)v);
 }
 
 // The 16-bit table's tiling: G8 groups of 8 slots (an even count), nblk column blocks of 32.
@@ -990,7 +1049,7 @@ static void vgemm16_shape(int nc, int slots, int& cols, int& g8, int& nblk) {
 }
 
 template <int NBLK, int G8, bool BF, int GS = 0>
-static void launch_vgemm16_gs(const VArg& a, const void* q, const float* alm, float* v, hipStream_t s) {
+static void launch_vgemm16_gs(const VArg& a, const void* q, const float* alm, void* v, hipStream_t s) {
     if constexpr (GS < G8) {
         if (a.lp_gs > GS) return launch_vgemm16_gs<NBLK, G8, BF, GS + 2>(a, q, alm, v, s);
     }
@@ -998,23 +1057,20 @@ static void launch_vgemm16_gs(const VArg& a, const void* q, const float* alm, fl
 }
 
 template <int NBLK, bool BF>
-static void launch_vgemm16_g(int g8, const VArg& a, const void* q, const float* alm, float* v, hipStream_t s) {
+static void launch_vgemm16_g(int g8, const VArg& a, const void* q, const float* alm, void* v, hipStream_t s) {
     if (g8 == 2) launch_vgemm16_gs<NBLK, 2, BF>(a, q, alm, v, s);
     else if (g8 == 4) launch_vgemm16_gs<NBLK, 4, BF>(a, q, alm, v, s);
     else if (g8 == 6) launch_vgemm16_gs<NBLK, 6, BF>(a, q, alm, v, s);
     else launch_vgemm16_gs<NBLK, 8, BF>(a, q, alm, v, s);
 }
 
-static int launch_vgemm16(const VArg& a, int kind, const void* q, const float* alm, float* v, hipStream_t s) {
+// V as 16-bit pairs of the products' type (the real and imaginary column blocks of a group
+// together: nblk 2, nc >= 32)
+static int launch_vgemm16(const VArg& a, int kind, const void* q, const float* alm, void* v, hipStream_t s) {
     int cols, g8, nblk;
     vgemm16_shape(a.nc, a.slots, cols, g8, nblk);
-    if (nblk == 1) {
-        if (kind == 2) launch_vgemm16_g<1, true>(g8, a, q, alm, v, s);
-        else launch_vgemm16_g<1, false>(g8, a, q, alm, v, s);
-    } else {
-        if (kind == 2) launch_vgemm16_g<2, true>(g8, a, q, alm, v, s);
-        else launch_vgemm16_g<2, false>(g8, a, q, alm, v, s);
-    }
+    if (kind == 2) launch_vgemm16_g<2, true>(g8, a, q, alm, v, s);
+    else launch_vgemm16_g<2, false>(g8, a, q, alm, v, s);
     return check_launch("k_vgemm");
 }
 
@@ -1068,17 +1124,21 @@ extern "C" int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int
     return ZB_OK;
 }
 
-template <int MODE>
-static int launch_apeak(const VArg& a, const float* v, const float* rden, int rden_div, unsigned long long* best, float* vol, hipStream_t s) {
+template <int MODE, int VT = 0>
+static int launch_apeak(const VArg& a, const void* v, const float* rden, int rden_div, unsigned long long* best, float* vol, hipStream_t s) {
+    if constexpr (MODE == 0 && VT == 0) {
+        if (a.vk == 1) return launch_apeak<0, 1>(a, v, rden, rden_div, best, vol, s);
+        if (a.vk == 2) return launch_apeak<0, 2>(a, v, rden, rden_div, best, vol, s);
+    }
     if (a.n == 256) {
         // two lanes per column: one wave per (j, 32 columns, item), four items a block
         const bool onevol = MODE == 0 && (rden_div >= a.n_items || rden_div % 4 == 0);
         const dim3 grid((unsigned)((long)a.nb * (a.nc / 32) * ((a.n_items + 3) / 4)));
         const size_t lds = onevol ? (size_t)32 * 256 * sizeof(float) : 0;
         if (onevol)
-            ZB_LAUNCH("k_apeak", (k_apeak128<MODE, true>), grid, dim3(256), lds, s, a, (const float2*)v, rden, rden_div, best, (float2*)vol);
+            ZB_LAUNCH("k_apeak", (k_apeak128<MODE, true, VT>), grid, dim3(256), lds, s, a, v, rden, rden_div, best, (float2*)vol);
         else
-            ZB_LAUNCH("k_apeak", (k_apeak128<MODE, false>), grid, dim3(256), lds, s, a, (const float2*)v, rden, rden_div, best, (float2*)vol);
+            ZB_LAUNCH("k_apeak", (k_apeak128<MODE, false, VT>), grid, dim3(256), lds, s, a, v, rden, rden_div, best, (float2*)vol);
         return check_launch("k_apeak");
     }
     const int per_wave = 64 >> a.log2nc;
@@ -1095,13 +1155,17 @@ static int launch_apeak(const VArg& a, const float* v, const float* rden, int rd
 #define ZB_AP(HH, LL)                                                                                                              \
     do {                                                                                                                           \
         if (onevol)                                                                                                                \
-            ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE, true>), grid, dim3(64 * waves), lds, s, a, (const float2*)v, rden, rden_div, best, \
+            ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE, true, VT>), grid, dim3(64 * waves), lds, s, a, v, rden, rden_div, best,        \
                       (float2*)vol);                                                                                               \
         else                                                                                                                       \
-            ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE, false>), grid, dim3(64 * waves), lds, s, a, (const float2*)v, rden, rden_div,     \
-                      best, (float2*)vol);                                                                                         \
+            ZB_LAUNCH("k_apeak", (k_apeak<HH, LL, MODE, false, VT>), grid, dim3(64 * waves), lds, s, a, v, rden, rden_div, best,       \
+                      (float2*)vol);                                                                                               \
     } while (0)
-    if (H == 8) ZB_AP(8, 3); else if (H == 16) ZB_AP(16, 4); else if (H == 32) ZB_AP(32, 5); else ZB_AP(64, 6);
+    if constexpr (VT == 0) {
+        if (H == 8) ZB_AP(8, 3); else if (H == 16) ZB_AP(16, 4); else if (H == 32) ZB_AP(32, 5); else ZB_AP(64, 6);
+    } else {
+        ZB_AP(64, 6);  // 16-bit V: nc >= 32, so n >= 128
+    }
 #undef ZB_AP
     return check_launch("k_apeak");
 }
@@ -1129,8 +1193,15 @@ extern "C" int32_t zb_vcorr_correlate(const zb_vtable* t, const float* twiddle, 
     ZB_LAUNCH("k_vclear", k_vclear, dim3((n_items + 255) / 256), dim3(256), 0, s, best, n_items);
     if (int rc = t->lp_kind ? launch_vgemm16(a, t->lp_kind, t->qlp, alm, v, s) : launch_vgemm(a, t->q, alm, v, s)) return rc;
     if (int rc = launch_apeak<0>(a, v, rden, rden_div, best, nullptr, s)) return rc;
-    ZB_LAUNCH("k_vfinish", k_vfinish, dim3(n_items), dim3(64), 0, s, a, (const float2*)v, (const float2*)twiddle, rden, rden_div,
-              (const unsigned long long*)best, peaks);
+    if (a.vk == 0)
+        ZB_LAUNCH("k_vfinish", k_vfinish<0>, dim3(n_items), dim3(64), 0, s, a, (const void*)v, (const float2*)twiddle, rden, rden_div,
+                  (const unsigned long long*)best, peaks);
+    else if (a.vk == 1)
+        ZB_LAUNCH("k_vfinish", k_vfinish<1>, dim3(n_items), dim3(64), 0, s, a, (const void*)v, (const float2*)twiddle, rden, rden_div,
+                  (const unsigned long long*)best, peaks);
+    else
+        ZB_LAUNCH("k_vfinish", k_vfinish<2>, dim3(n_items), dim3(64), 0, s, a, (const void*)v, (const float2*)twiddle, rden, rden_div,
+                  (const unsigned long long*)best, peaks);
     return check_launch("zb_vcorr_correlate/finish");
 }
 
@@ -1139,6 +1210,7 @@ extern "C" int32_t zb_vcorr_denominators(const zb_vtable* t_f, const zb_vtable* 
     VArg a, a2;
     if (int rc = v_arg(t_f, n_windows, a, "zb_vcorr_denominators")) return rc;
     if (int rc = v_arg(t_f2, n_windows, a2, "zb_vcorr_denominators")) return rc;
+    a.vk = a2.vk = 0;  // the window's volumes from the FP32 tables q
     if (a.L != a2.L || a.n != a2.n || a.nc != a2.nc || a.l_step != a2.l_step || n_windows < 0 ||
         (n_windows > 0 && (!w_alm || !w_stats || !work || !rden))) {
         set_error("zb_vcorr_denominators: bad arguments");

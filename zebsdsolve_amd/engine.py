@@ -376,7 +376,8 @@ class Engine:
     def corr_vtable(self, ph):
         """The GEMM table the correlation multiplies with, at the precision the settings name."""
         kind = {"fp32": 0, "fp16": 1, "bf16": 2}[self.settings["correlation_precision"]]
-        if kind == 0:
+        # the 16-bit form pairs 32 complex columns a group: below nc = 32 the FP32 form
+        if kind == 0 or ph.host.nc < 32:
             return ph.vtables()[0][0]
         return ph.vtable16(kind, int(self.settings["split_degree"]))
 

@@ -31,9 +31,10 @@ DEFAULTS = {
     "refine": True, "refine_iters": 8, "refine_step": 2.0,
     # ── phases ──
     "phase_by": "nmi", "nmi_bins": 16,
-    # ── products of the correlation's GEMM (this build's own; the reference multiplies in
-    # float32): "fp32", or 16-bit "fp16" / "bf16" with float32 accumulation and hi + lo products
-    # for the degrees from split_degree on ──
+    # ── the correlation's GEMM (this build's own; the reference computes in float32): "fp32",
+    # or "fp16" / "bf16": 16-bit products with float32 accumulation (hi + lo products for the
+    # degrees from split_degree on) and the GEMM's output kept in the same 16-bit type, the
+    # transform along a in float32; phases with fewer than 32 points along c stay "fp32" ──
     "correlation_precision": "fp32", "split_degree": 64,
     # ── how the work is cut up ──
     "batch": 256, "tile_mb": 256, "refine_pixels": 64, "wg_fft": 64, "wg_refine": 64,

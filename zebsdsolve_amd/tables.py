@@ -166,14 +166,18 @@ def split_slot(split_degree, l_step):
 
 def tile_vtable16(T, groups, nblk, split, kind):
     """The natural table of one j, [L][slots][2][cols] float64, as zb_vtable.qlp's pieces for
-    that j: per (m', column group cg), the groups t of 8 slots in order, each [nblk][64 lanes][8]
-    hi values, lane = 32 r + x holding row r of column cg nblk 32 + b 32 + x for slots
-    8 t .. 8 t + 7; groups t >= split / 8 followed by the same for lo = round16(x - hi)."""
+    that j: per (m', column group cg of 32 complex columns), the groups t of 8 slots in order,
+    each [2 blocks][64 lanes][8] hi values, lane = 32 r + x holding row r of column
+    2 (32 cg + x) + b (block 0 the real and block 1 the imaginary part of complex column
+    32 cg + x) for slots 8 t .. 8 t + 7; groups t >= split / 8 followed by the same for
+    lo = round16(x - hi)."""
     L, slots, _, cols = T.shape
-    ncg = cols // (32 * nblk)
+    if nblk != 2 or cols % 64:
+        raise ValueError("the 16-bit table pairs the real and imaginary column blocks: nc >= 32")
+    ncg = cols // 64
     Tp = np.zeros((L, 8 * groups, 2, cols))
     Tp[:, :slots] = T
-    Tp = np.transpose(Tp.reshape(L, groups, 8, 2, ncg, nblk, 32), (0, 4, 1, 5, 3, 6, 2))  # [m'][cg][t][b][r][x][i]
+    Tp = np.transpose(Tp.reshape(L, groups, 8, 2, ncg, 32, 2), (0, 4, 1, 6, 3, 5, 2))  # [m'][cg][t][b][r][x][i]
     hb, hv = round16(Tp, kind)
     lb, _ = round16(Tp - hv, kind)
     P = nblk * 64 * 8
@@ -230,6 +234,10 @@ class VTableHost:
         `split` on: the rows rebuilt in float64 from Q, one j at a time."""
         _, groups, nblk = vgemm16_shape(self.nc, self.slots)
         nc = self.nc
+        # V = Q conj(g) with unit-norm coefficients: |V| <= the l2 norm of a row of Q, which FP16
+        # V must hold (twice over)
+        if kind == 1 and 2.0 * np.sqrt((np.abs(self.Q) ** 2).sum(axis=-1)).max() > 65504.0:
+            raise ValueError("V could exceed the FP16 range: use BF16")
         out = []
         for Qj in self.Q:                                            # [m', c, s]
             Qt = np.transpose(Qj, (0, 2, 1))                          # [m', s, c]
