@@ -162,8 +162,8 @@ class Engine:
         self.bands = {}
         self.stamp = 0
         self.dev = None
-        # "mfma": the correlation as a GEMM on the matrix cores (bandwidths with n <= 128);
-        # "slab": the spectrum + slab transforms (every bandwidth, and the fallback above 128)
+        # "mfma": the correlation as a GEMM on the matrix cores (n <= 256, at most 64 degree slots);
+        # "slab": the spectrum + slab transforms (every bandwidth, and the fallback where the GEMM does not fit)
         self.correlator = "mfma"
         self._lanes = []
         self._bg = None           # (key, static background on the device)
