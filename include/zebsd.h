@@ -226,6 +226,17 @@ typedef struct zb_prep_opts {
     const float* background;
     int32_t dynamic;
     float dynamic_sigma;
+    /* CLAHE (settings clahe, clahe_grid 1..16, clahe_bins 2..256, clahe_clip > 0): the image
+     * stretched to [0, 1] and equalized over grid x grid tiles */
+    int32_t clahe;
+    int32_t clahe_grid;
+    int32_t clahe_bins;
+    float clahe_clip;
+    /* band pass (settings bandpass, bp_fine >= 0, bp_coarse > bp_fine, pixels): the Gaussian of
+     * bp_fine (the image itself when bp_fine <= 0.05) minus the Gaussian of bp_coarse */
+    int32_t bandpass;
+    float bp_fine;
+    float bp_coarse;
 } zb_prep_opts;
 
 /* zb_normalize after the filters, fused: raw patterns in, the static background subtracted,
@@ -233,11 +244,28 @@ typedef struct zb_prep_opts {
  * The blur is the reference's: direct separable taps below 12 pixels of deviation, sums over
  * boxes from 12. One block per pattern with the image in LDS: fails (ZB_EINVAL) for patterns
  * whose image and blur buffers exceed the device's LDS per block.
+ * CLAHE and the band pass follow, in the reference's order; CLAHE's histograms take
+ * grid^2 bins words of the LDS, and the band pass parks its coarse Gaussian in `out`.
  * Replaces Prep.run + Prep.filter (src/prep.zig:143-204; prep.wgsl load, background, blur_rows,
- * blur_cols, box_sum, coarse_rows, coarse_cols, spread_rows, spread_cols, combine, normalize)
- * with static and dynamic backgrounds; CLAHE and the band pass are not built. */
+ * blur_cols, box_sum, coarse_rows, coarse_cols, spread_rows, spread_cols, combine, stretch,
+ * clahe_hist, clahe_cdf, clahe_apply, normalize). */
 int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* opts,
                 float* out, void* stream);
+
+/* CLAHE of n h x w float images in place, in global memory (for images larger than zb_prep's
+ * LDS: a master's two sheets): each image stretched to [0, 1] by its extremes, then equalized
+ * over grid x grid tiles (1..64) of bins levels (2..256) with the histograms clipped at clip x
+ * the mean count, as zb_prep's CLAHE. work: zb_clahe_workspace bytes (the tables).
+ * Replaces Prep.filter with only CLAHE on, as bakeMaster runs it on the sheets
+ * (src/engine.zig:1741-1743; prep.wgsl stretch, clahe_hist, clahe_cdf, clahe_apply). */
+int64_t zb_clahe_workspace(int32_t n, int32_t grid, int32_t bins);
+int32_t zb_clahe(float* img, int32_t n, int32_t h, int32_t w, int32_t grid, int32_t bins, float clip, void* work, int64_t work_bytes,
+                 void* stream);
+
+/* The mean of a [2][side][side] master over its point group: each texel the average of the
+ * sheets (bilinear) at op . d over ops [n_ops][3][3] (row-major, device memory); dst != src.
+ * Replaces Prep.symmetrize (src/prep.zig:241-249; mastersph.wgsl:111-122). */
+int32_t zb_master_symmetrize(const float* src, float* dst, int32_t side, const float* ops, int32_t n_ops, void* stream);
 
 /* Simulated patterns: the master (bilinear, [2][side][side] f32) in direction q . ray of every
  * pixel. poses: [n][8] = (t, pattern index, q); out: [n][h][w].

@@ -254,18 +254,102 @@ def blur_box(img, sigma):
     return sc[..., 0] / sc[..., 1]
 
 
-def prep(raw, mask, background=None, dynamic="none", dynamic_sigma=0.15):
-    """The filter chain then normalization (src/prep.zig:148-204): the static background
-    subtracted, the dynamic one (a Gaussian of dynamic_sigma x the width, the wide path from 12
-    pixels) subtracted or divided by, then zero mean and unit norm under the mask."""
+def gauss(img, sigma):
+    """The reference's Gaussian of deviation sigma pixels: the box path from 12 pixels
+    (src/prep.zig:105-140)."""
+    return blur_box(img, sigma) if int(np.floor(sigma / 6)) >= 2 else blur_direct(img, sigma)
+
+
+def stretch(img):
+    """To [0, 1] by the image's extremes (prep.wgsl:289-315), in float32 as there."""
+    x = np.asarray(img, np.float32)
+    lo = x.min()
+    k = np.float32(1.0) / np.maximum(x.max() - lo, np.float32(1e-20))
+    return (x - lo) * k
+
+
+def _reflect(i, n, pad):
+    if n <= 1:
+        return np.zeros_like(i)
+    x = np.abs(i - pad)
+    x = np.where(x >= n, 2 * n - 2 - x, x)
+    return np.clip(x, 0, n - 1)
+
+
+def clahe(img, tiles, bins, clip):
+    """Contrast-limited adaptive histogram equalization of a [0, 1] image (prep.wgsl:317-397),
+    in float32 in the same order of operations: the image padded by reflection at its top and
+    left to whole tiles; each tile's histogram clipped at clip x the mean count, what was cut
+    spread evenly (the remainder one count each to the first levels); every pixel read from the
+    four nearest tiles' tables, each between its two nearest levels."""
+    x = np.asarray(img, np.float32)
+    h, w = x.shape
+    f32 = np.float32
+    pad_h, pad_w = (tiles - h % tiles) % tiles, (tiles - w % tiles) % tiles
+    hp, wp = h + pad_h, w + pad_w
+    th, tw = max(1, hp // tiles), max(1, wp // tiles)
+    py, px = np.meshgrid(np.arange(hp), np.arange(wp), indexing="ij")
+    v = np.clip(x[_reflect(py, h, pad_h), _reflect(px, w, pad_w)], f32(0), f32(1))
+    b = np.clip(v * f32(bins - 1), f32(0), f32(bins - 1)).astype(np.int64)
+    tile = (py // th) * tiles + px // tw
+    hist = np.zeros((tiles * tiles, bins), np.int64)
+    np.add.at(hist, (tile.ravel(), b.ravel()), 1)
+    voxels = f32(max(th * tw, 1))
+    nb = f32(bins)
+    limit = np.maximum(f32(clip) * voxels / nb, f32(1))
+    cdf = np.zeros((tiles * tiles, bins), np.float32)
+    for t in range(tiles * tiles):
+        cut = np.minimum(hist[t].astype(np.float32), limit)
+        sumc = f32(0)
+        for c in cut:
+            sumc = f32(sumc + c)
+        clipped = f32(voxels - sumc)
+        residual = int(f32(clipped - nb * np.floor(clipped / nb)))
+        redist = f32(f32(clipped - f32(residual)) / nb)
+        acc = f32(0)
+        for i in range(bins):
+            c = f32(cut[i] + redist)
+            if i < residual:
+                c = f32(c + f32(1))
+            acc = f32(acc + c)
+            cdf[t, i] = np.clip(acc / voxels, f32(0), f32(1))
+    r, c = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    v = np.clip(x, f32(0), f32(1))
+    gy = (r.astype(np.float32) + f32(0.5)) / f32(h) * f32(tiles - 1)
+    gx = (c.astype(np.float32) + f32(0.5)) / f32(w) * f32(tiles - 1)
+    y0, x0 = np.floor(gy).astype(np.int64), np.floor(gx).astype(np.int64)
+    y1, x1 = np.minimum(y0 + 1, tiles - 1), np.minimum(x0 + 1, tiles - 1)
+    fy, fx = gy - y0.astype(np.float32), gx - x0.astype(np.float32)
+    bf = v * f32(bins - 1)
+    b0 = np.clip(np.floor(bf), f32(0), f32(bins - 2)).astype(np.int64)
+    fb = bf - b0.astype(np.float32)
+    mix = lambda a, b_, t: a * (f32(1) - t) + b_ * t  # noqa: E731
+
+    def at(ty, tx):
+        t = ty * tiles + tx
+        return mix(cdf[t, b0], cdf[t, b0 + 1], fb)
+    return mix(mix(at(y0, x0), at(y0, x1), fx), mix(at(y1, x0), at(y1, x1), fx), fy)
+
+
+def prep(raw, mask, background=None, dynamic="none", dynamic_sigma=0.15, clahe_opts=None, bandpass=None, norm=True):
+    """The filter chain then normalization, in the reference's order (src/prep.zig:148-204):
+    the static background subtracted; the dynamic one (a Gaussian of dynamic_sigma x the width)
+    subtracted or divided by; CLAHE (clahe_opts = (grid, bins, clip)) after stretching to
+    [0, 1]; the band pass (bandpass = (fine, coarse) sigmas in pixels: the fine Gaussian, or the
+    image itself when fine <= 0.05, minus the coarse one); then zero mean and unit norm under
+    the mask."""
     img = np.asarray(raw, np.float64).copy()
     if background is not None:
         img = img - np.asarray(background, np.float64)
     if dynamic != "none":
-        sigma = dynamic_sigma * img.shape[1]
-        bl = blur_box(img, sigma) if int(np.floor(sigma / 6)) >= 2 else blur_direct(img, sigma)
+        bl = gauss(img, dynamic_sigma * img.shape[1])
         img = img - bl if dynamic == "subtract" else img / np.maximum(bl, 1e-6)
-    return normalize(img, mask)
+    if clahe_opts is not None:
+        img = clahe(stretch(img), *clahe_opts).astype(np.float64)
+    if bandpass is not None and bandpass[1] > 0:
+        fine = gauss(img, bandpass[0]) if bandpass[0] > 0.05 else img
+        img = fine - gauss(img, bandpass[1])
+    return normalize(img, mask) if norm else img
 
 
 def project(sheets, side, fr, t, q):

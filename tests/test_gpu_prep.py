@@ -100,3 +100,56 @@ def test_toy_scan_through_the_default_filters():
         r = s.index()
         d = np.degrees(s.disorientation(r["quaternions"].reshape(-1, 4), tq))
         assert np.median(d) < 0.15 and d.max() < 1.0, (kw, np.median(d), d.max())
+
+
+def test_global_clahe_matches_the_oracle():
+    """zb_clahe (the master's CLAHE, in global memory) on two images of odd size against the
+    oracle's restatement of prep.wgsl: the stretch and the levels in float32 as there, so only
+    the blends' last bits differ."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import _native as nat, engine
+    lib = nat.load()
+    rng = np.random.default_rng(9)
+    h, w = 101, 77
+    imgs = (_patterns(2, h, w, seed=4) + rng.normal(0, 20, (2, h, w))).astype(np.float32)
+    for grid, bins, clip in ((8, 64, 4.0), (3, 256, 40.0), (16, 16, 1.5)):
+        x = torch.from_numpy(imgs).cuda()
+        ws = torch.empty((lib.zb_clahe_workspace(2, grid, bins),), dtype=torch.uint8, device="cuda")
+        nat.check(lib.zb_clahe(x.data_ptr(), 2, h, w, grid, bins, clip, ws.data_ptr(), ws.numel(), engine.stream_ptr()), "clahe")
+        got = x.cpu().numpy()
+        for b in range(2):
+            ref = op.clahe(op.stretch(imgs[b]), grid, bins, clip)
+            assert np.abs(got[b] - ref).max() < 1e-5, (grid, bins, clip, b, np.abs(got[b] - ref).max())
+    x = torch.from_numpy(imgs).cuda()
+    assert lib.zb_clahe(x.data_ptr(), 2, h, w, 4, 64, 40.0, x.data_ptr(), 10, engine.stream_ptr()) < 0
+    assert "workspace" in nat.last_error()
+
+
+def test_master_symmetrize_is_the_point_group_mean():
+    """zb_master_symmetrize against the oracle's bilinear lookup averaged over the master's
+    operators at 3000 texels; and the nickel master, which obeys its point group, moves only by
+    interpolation (0.069 of its unit deviation, rms, unsmoothed)."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import _native as nat, engine, master as mm
+    from tests.conftest import NI
+    lib = nat.load()
+    m = mm.load(NI)
+    side = m.side
+    rng = np.random.default_rng(2)
+    smooth = np.asarray(m.raw, np.float32)
+    src = torch.from_numpy(smooth.reshape(-1)).cuda()
+    dst = torch.empty_like(src)
+    ops = torch.from_numpy(np.asarray(m.ops, np.float32).reshape(-1)).cuda()
+    nat.check(lib.zb_master_symmetrize(src.data_ptr(), dst.data_ptr(), side, ops.data_ptr(), len(m.ops), engine.stream_ptr()), "symmetrize")
+    got = dst.cpu().numpy()
+    tid = rng.integers(0, 2 * side * side, 3000)
+    pix = tid % (side * side)
+    x = 2.0 * (pix % side) / (side - 1) - 1.0
+    y = 2.0 * (pix // side) / (side - 1) - 1.0
+    d = op.square_to_sphere(x, y, tid >= side * side)
+    want = np.mean([op.sheet_at(smooth.reshape(2, side, side).astype(np.float64), side, d @ np.asarray(g).T) for g in m.ops], axis=0)
+    assert np.abs(got[tid] - want).max() < 1e-4, np.abs(got[tid] - want).max()
+    assert np.sqrt(np.mean((got - smooth.reshape(-1)) ** 2)) < 0.1
+    assert lib.zb_master_symmetrize(src.data_ptr(), src.data_ptr(), side, ops.data_ptr(), len(m.ops), None) < 0

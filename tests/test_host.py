@@ -258,10 +258,18 @@ def test_settings_reject_unknown_and_out_of_range_atomically():
     with pytest.raises(SettingError):
         s.apply(dynamic_background="gaussian")
     with pytest.raises(SettingError):
-        s.apply(clahe=True)
-    # the reference's defaults for the filters (settings.zig:138-148)
-    assert Settings()["static_background"] == "average" and Settings()["dynamic_background"] == "subtract"
-    assert Settings()["static_patterns"] == 1024 and Settings()["dynamic_sigma"] == 0.15
+        s.apply(nlpar=True)
+    # CLAHE and the band pass: the reference's ranges (settings.zig:430, 456)
+    s.apply(clahe=True, bandpass=True)
+    for bad in ({"clahe_grid": 17}, {"clahe_bins": 1}, {"clahe_clip": 0.0}, {"bp_fine": 8.0}, {"bp_coarse": 0.5}):
+        with pytest.raises(SettingError):
+            s.apply(**bad)
+    # the reference's defaults for the filters (settings.zig:138-159)
+    d = Settings()
+    assert d["static_background"] == "average" and d["dynamic_background"] == "subtract"
+    assert d["static_patterns"] == 1024 and d["dynamic_sigma"] == 0.15
+    assert (d["clahe"], d["clahe_grid"], d["clahe_bins"], d["clahe_clip"]) == (False, 4, 64, 40.0)
+    assert (d["bandpass"], d["bp_fine"], d["bp_coarse"]) == (False, 1.0, 8.0)
 
 
 def test_sweeps_climb_to_the_final_bandwidth():

@@ -353,3 +353,76 @@ def test_prep_chain_order_and_identities():
     ramp = np.add.outer(np.arange(40) * 0.7, np.arange(50) * 1.3) + 100
     r = int(np.ceil(3 * 4.0))
     assert np.abs(op.blur_direct(ramp, 4.0)[r:-r, r:-r] - ramp[r:-r, r:-r]).max() < 1e-9
+
+
+def test_clahe_with_one_tile_and_no_clip_is_histogram_equalization():
+    """One tile and a clip no bin reaches: every count kept, so a pixel's table value is the
+    fraction of the image at or below its level (the empirical CDF), read between levels; on a
+    ramp with equal counts per level that CDF is (b + 1) / bins."""
+    from oracle import pipeline as op
+    bins = 16
+    # each level b holds the same 8 pixels, values at the level's centre
+    vals = np.minimum(np.repeat((np.arange(bins) + 0.25) / (bins - 1), 8), 1.0)
+    img = vals.reshape(16, 8).astype(np.float32)
+    out = op.clahe(img, 1, bins, 1e6)
+    b0 = np.clip(np.floor(img * (bins - 1)), 0, bins - 2).astype(int)
+    fb = img * (bins - 1) - b0
+    want = ((b0 + 1) * (1 - fb) + (b0 + 2) * fb) / bins
+    assert np.abs(out - want).max() < 1e-6
+    # distinct random values: the table is the rank fraction, and the output is monotone
+    rng = np.random.default_rng(3)
+    x = rng.random((20, 20)).astype(np.float32)
+    y = op.clahe(x, 1, 64, 1e6)
+    order = np.argsort(x.ravel())
+    assert np.all(np.diff(y.ravel()[order]) >= -1e-7)
+    bins_of = np.clip(x.ravel() * 63, 0, 63).astype(int)
+    cdf = np.cumsum(np.bincount(bins_of, minlength=64)) / 400.0
+    b0 = np.clip(np.floor(x.ravel() * 63), 0, 62).astype(int)
+    f = x.ravel() * 63 - b0
+    assert np.abs(y.ravel() - (cdf[b0] * (1 - f) + cdf[b0 + 1] * f)).max() < 1e-6
+
+
+def test_clahe_clip_spreads_what_it_cuts():
+    """A constant 8 x 8 image at 0.5 (level 31 of 64), one tile: with clip 40 the full level
+    (64 counts) is cut to 40 and the 24 cut counts go one each to levels 0..23 (fewer than the
+    levels: no even share), so the table is (24 + 40) / 64 = 1 from level 31 on and the output
+    is 1. With clip 8, 56 are cut and go one each to levels 0..55: level 31 holds 8 + 1, the
+    table reads (31 + 9) / 64 at level 31 and (32 + 9) / 64 at 32, and 0.5 (31.5 levels) takes
+    their mean."""
+    from oracle import pipeline as op
+    img = np.full((8, 8), 0.5, np.float32)
+    assert np.allclose(op.clahe(img, 1, 64, 40.0), 1.0)
+    out = op.clahe(img, 1, 64, 8.0)
+    assert np.allclose(out, 0.5 * (31 + 9) / 64 + 0.5 * (32 + 9) / 64)
+
+
+def test_clahe_pads_by_reflection_to_whole_tiles():
+    """A 6 x 6 image in a 4 x 4 grid is padded by 2 rows and columns, reflected at the top and
+    left (padded index i reads |i - 2|): every tile's histogram holds 2 x 2 pixels. A left-right
+    ramp of 6 levels: the first tile column holds image columns 2, 1 (the reflection), the next
+    0, 1, then 2, 3 and 4, 5; with 6 levels and no clip each tile's table is 1/2 at its lower
+    level and 1 from its upper one, so the first pixel (level 0, between tiles 0 and 1 at
+    gx = 0.25) reads 0.75 x 0 + 0.25 x 1/2 from the two tables."""
+    from oracle import pipeline as op
+    assert list(op._reflect(np.arange(8), 6, 2)) == [2, 1, 0, 1, 2, 3, 4, 5]
+    img = np.tile(np.linspace(0, 1, 6, dtype=np.float32), (6, 1))
+    out = op.clahe(img, 4, 6, 1e6)
+    assert out.shape == (6, 6) and np.allclose(out, out[:1])  # every row the same ramp
+    # pixel (0, 0): level 0; tile 0 holds levels {1, 2} (table 0 at level 0), tile 1 holds
+    # levels {0, 1} (table 1/2 at level 0); gx = 0.5 / 6 * 3 = 0.25 between tile columns 0 and 1
+    assert abs(out[0, 0] - (0.75 * 0.0 + 0.25 * 0.5)) < 1e-6
+
+
+def test_bandpass_of_a_constant_is_zero_and_a_difference_of_blurs():
+    """Both Gaussians renormalize their taps inside the image, so a constant passes both
+    unchanged and the band pass removes it; otherwise it is the fine blur minus the coarse one
+    (the image itself below 0.05 pixels)."""
+    from oracle import pipeline as op
+    c = np.full((40, 50), 3.0)
+    assert np.abs(op.prep(c, None, bandpass=(1.0, 8.0), norm=False)).max() < 1e-12
+    rng = np.random.default_rng(5)
+    x = rng.random((40, 50))
+    bp = op.prep(x, None, bandpass=(1.0, 14.0), norm=False)
+    assert np.allclose(bp, op.blur_direct(x, 1.0) - op.blur_box(x, 14.0))
+    bp0 = op.prep(x, None, bandpass=(0.0, 8.0), norm=False)
+    assert np.allclose(bp0, x - op.blur_direct(x, 8.0))

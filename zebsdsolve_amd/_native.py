@@ -39,7 +39,9 @@ class RefineOpts(ctypes.Structure):
 
 
 class PrepOpts(ctypes.Structure):  # zb_prep_opts
-    _fields_ = [("background", ctypes.c_void_p), ("dynamic", i32), ("dynamic_sigma", ctypes.c_float)]
+    _fields_ = [("background", ctypes.c_void_p), ("dynamic", i32), ("dynamic_sigma", ctypes.c_float),
+                ("clahe", i32), ("clahe_grid", i32), ("clahe_bins", i32), ("clahe_clip", ctypes.c_float),
+                ("bandpass", i32), ("bp_fine", ctypes.c_float), ("bp_coarse", ctypes.c_float)]
 
 
 _SIGS = {
@@ -61,6 +63,9 @@ _SIGS = {
     "zb_sums": ([vp, i32, ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),
     "zb_project": ([vp, i32, ctypes.POINTER(Frame), vp, i32, vp, vp], i32),
     "zb_master_blur": ([vp, vp, i32, ctypes.c_float, vp], i32),
+    "zb_clahe_workspace": ([i32, i32, i32], i64),
+    "zb_clahe": ([vp, i32, i32, i32, i32, i32, ctypes.c_float, vp, i64, vp], i32),
+    "zb_master_symmetrize": ([vp, vp, i32, vp, i32, vp], i32),
     "zb_master_gradient": ([vp, i32, vp, vp], i32),
     "zb_master_pack": ([vp, i32, vp, vp], i32),
     "zb_profile": ([i32], i32),

@@ -1,5 +1,7 @@
 // Patterns in, masters baked: per-pattern normalization under the mask, simulated patterns,
 // and the master's smoothing and gradient on the sphere (setup, once per master and geometry).
+#include <algorithm>
+
 #include "common.h"
 
 namespace zb {
@@ -113,10 +115,7 @@ __global__ void __launch_bounds__(256) k_normalize_big(const T* __restrict__ raw
 // blurred on the coarse grid (deviation sqrt(sigma^2 - s^2 - (s^2 - 1) / 12), out to 4 of them,
 // with a margin of 3 whole cells) and spread back by a Gaussian of deviation s; sums of pixels
 // and of counts travel together and are divided at the end.
-struct PrepArg {
-    int h, w;
-    float mask[3];
-    int dyn;       // 0 none, 1 subtract, 2 divide
+struct BlurArg {
     int box;       // 0: direct taps, 1: the box path
     int radius;    // direct: taps each way
     float k;       // direct: 1 / (2 sigma^2)
@@ -124,6 +123,21 @@ struct PrepArg {
     float reach;   // box: ceil(4 sigma_coarse), pixels
     float k0, k2;  // box: 1 / (2 sigma_coarse^2), 1 / (2 s^2)
     int hs, ws, hp, wp;
+};
+
+struct PrepArg {
+    int h, w;
+    float mask[3];
+    int dyn;  // 0 none, 1 subtract, 2 divide
+    BlurArg dynb;
+    // CLAHE: tiles x tiles, bins levels, clip; the image padded by pad_h, pad_w to tiles of
+    // tile_h x tile_w
+    int clahe, tiles, bins;
+    float clip;
+    int pad_h, pad_w, tile_h, tile_w;
+    // band pass: the fine Gaussian (or the image itself) minus the coarse one
+    int bp, bp_fine;
+    BlurArg fine, coarse;
 };
 
 // centre (pixels) and width of coarse cell j along an axis of n pixels (cells outside are whole)
@@ -134,123 +148,305 @@ __device__ __forceinline__ float2 prep_cell(int j, int n, int s) {
     return make_float2(0.5f * (float)(lo + hi - 1), (float)(hi - lo));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_prep(const T* __restrict__ raw, const float* __restrict__ bg, PrepArg a, float* __restrict__ out) {
-    extern __shared__ float lds[];
-    __shared__ float red[4];
-    const int hw = a.h * a.w;
+// The Gaussian of the h x w image `img` (LDS, read only), through the scratch `aux` (LDS); the
+// last pass hands each pixel's blurred value to emit(i, value), pixel i on thread i mod 256.
+template <typename Emit>
+__device__ void gauss(const float* img, float* aux, int h, int w, const BlurArg& g, Emit emit) {
+    const int hw = h * w;
     const int tid = threadIdx.x;
-    float* img = lds;
-    float* aux = lds + ((hw + 3) & ~3);
-    const T* src = raw + (size_t)blockIdx.x * hw;
-    for (int i = tid; i < hw; i += 256) img[i] = (float)src[i] - (bg ? bg[i] : 0.f);
-    __syncthreads();
-    auto combine = [&](float v, float b) { return a.dyn == 1 ? v - b : v / fmaxf(b, 1e-6f); };
-    if (a.dyn && !a.box) {
+    if (!g.box) {
         float* wt = aux + ((hw + 3) & ~3);  // exp(-d^2 k), d <= radius
-        for (int d = tid; d <= a.radius; d += 256) wt[d] = expf(-(float)(d * d) * a.k);
+        for (int d = tid; d <= g.radius; d += 256) wt[d] = expf(-(float)(d * d) * g.k);
         __syncthreads();
         for (int i = tid; i < hw; i += 256) {
-            const int r = i / a.w, c = i - (i / a.w) * a.w;
+            const int r = i / w, c = i - (i / w) * w;
             float acc = 0.f, ws = 0.f;
-            const int x1 = min(c + a.radius, a.w - 1);
-            for (int x = max(c - a.radius, 0); x <= x1; ++x) {
-                const float g = wt[abs(x - c)];
-                acc += g * img[r * a.w + x];
-                ws += g;
+            const int x1 = min(c + g.radius, w - 1);
+            for (int x = max(c - g.radius, 0); x <= x1; ++x) {
+                const float q = wt[abs(x - c)];
+                acc += q * img[r * w + x];
+                ws += q;
             }
             aux[i] = acc / ws;
         }
         __syncthreads();
         for (int i = tid; i < hw; i += 256) {
-            const int r = i / a.w, c = i - (i / a.w) * a.w;
+            const int r = i / w, c = i - (i / w) * w;
             float acc = 0.f, ws = 0.f;
-            const int y1 = min(r + a.radius, a.h - 1);
-            for (int y = max(r - a.radius, 0); y <= y1; ++y) {
-                const float g = wt[abs(y - r)];
-                acc += g * aux[y * a.w + c];
-                ws += g;
+            const int y1 = min(r + g.radius, h - 1);
+            for (int y = max(r - g.radius, 0); y <= y1; ++y) {
+                const float q = wt[abs(y - r)];
+                acc += q * aux[y * w + c];
+                ws += q;
             }
-            img[i] = combine(img[i], acc / ws);
+            emit(i, acc / ws);
         }
         __syncthreads();
-    } else if (a.dyn) {
-        const int s = a.s, pad = a.pad;
-        const int big = a.hp * (a.wp > a.w ? a.wp : a.w);
-        float2* c0 = reinterpret_cast<float2*>(aux);
-        float2* c1 = c0 + big;
-        // pixel sums and counts over the s x s boxes: [hs][ws]
-        for (int t = tid; t < a.hs * a.ws; t += 256) {
-            const int i = t / a.ws, j = t - (t / a.ws) * a.ws;
-            const int r1 = min((i + 1) * s, a.h), q1 = min((j + 1) * s, a.w);
-            float acc = 0.f;
-            for (int r = i * s; r < r1; ++r)
-                for (int q = j * s; q < q1; ++q) acc += img[r * a.w + q];
-            c0[t] = make_float2(acc, (float)((r1 - i * s) * (q1 - j * s)));
+        return;
+    }
+    const int s = g.s, pad = g.pad;
+    const int big = g.hp * (g.wp > w ? g.wp : w);
+    float2* c0 = reinterpret_cast<float2*>(aux);
+    float2* c1 = c0 + big;
+    // pixel sums and counts over the s x s boxes: [hs][ws]
+    for (int t = tid; t < g.hs * g.ws; t += 256) {
+        const int i = t / g.ws, j = t - (t / g.ws) * g.ws;
+        const int r1 = min((i + 1) * s, h), q1 = min((j + 1) * s, w);
+        float acc = 0.f;
+        for (int r = i * s; r < r1; ++r)
+            for (int q = j * s; q < q1; ++q) acc += img[r * w + q];
+        c0[t] = make_float2(acc, (float)((r1 - i * s) * (q1 - j * s)));
+    }
+    __syncthreads();
+    // along the rows onto the coarse grid with its margin: [hs][wp]
+    for (int t = tid; t < g.hs * g.wp; t += 256) {
+        const int line = t / g.wp, tc = t - (t / g.wp) * g.wp;
+        const float cc = prep_cell(tc - pad, w, s).x;
+        const int j0 = max((int)floorf((cc - g.reach) / (float)s), 0);
+        const int j1 = min((int)floorf((cc + g.reach) / (float)s), g.ws - 1);
+        float2 acc = make_float2(0.f, 0.f);
+        for (int j = j0; j <= j1; ++j) {
+            const float d = prep_cell(j, w, s).x - cc;
+            const float q = expf(-d * d * g.k0);
+            const float2 v = c0[line * g.ws + j];
+            acc = make_float2(acc.x + q * v.x, acc.y + q * v.y);
         }
-        __syncthreads();
-        // along the rows onto the coarse grid with its margin: [hs][wp]
-        for (int t = tid; t < a.hs * a.wp; t += 256) {
-            const int line = t / a.wp, tc = t - (t / a.wp) * a.wp;
-            const float cc = prep_cell(tc - pad, a.w, s).x;
-            const int j0 = max((int)floorf((cc - a.reach) / (float)s), 0);
-            const int j1 = min((int)floorf((cc + a.reach) / (float)s), a.ws - 1);
-            float2 acc = make_float2(0.f, 0.f);
-            for (int j = j0; j <= j1; ++j) {
-                const float d = prep_cell(j, a.w, s).x - cc;
-                const float g = expf(-d * d * a.k0);
-                const float2 v = c0[line * a.ws + j];
-                acc = make_float2(acc.x + g * v.x, acc.y + g * v.y);
-            }
-            c1[t] = acc;
+        c1[t] = acc;
+    }
+    __syncthreads();
+    // along the columns: [hp][wp]
+    for (int t = tid; t < g.hp * g.wp; t += 256) {
+        const int tr = t / g.wp, col = t - (t / g.wp) * g.wp;
+        const float cc = prep_cell(tr - pad, h, s).x;
+        const int i0 = max((int)floorf((cc - g.reach) / (float)s), 0);
+        const int i1 = min((int)floorf((cc + g.reach) / (float)s), g.hs - 1);
+        float2 acc = make_float2(0.f, 0.f);
+        for (int i = i0; i <= i1; ++i) {
+            const float d = prep_cell(i, h, s).x - cc;
+            const float q = expf(-d * d * g.k0);
+            const float2 v = c1[i * g.wp + col];
+            acc = make_float2(acc.x + q * v.x, acc.y + q * v.y);
         }
-        __syncthreads();
-        // along the columns: [hp][wp]
-        for (int t = tid; t < a.hp * a.wp; t += 256) {
-            const int tr = t / a.wp, col = t - (t / a.wp) * a.wp;
-            const float cc = prep_cell(tr - pad, a.h, s).x;
-            const int i0 = max((int)floorf((cc - a.reach) / (float)s), 0);
-            const int i1 = min((int)floorf((cc + a.reach) / (float)s), a.hs - 1);
-            float2 acc = make_float2(0.f, 0.f);
-            for (int i = i0; i <= i1; ++i) {
-                const float d = prep_cell(i, a.h, s).x - cc;
-                const float g = expf(-d * d * a.k0);
-                const float2 v = c1[i * a.wp + col];
-                acc = make_float2(acc.x + g * v.x, acc.y + g * v.y);
-            }
-            c0[t] = acc;
+        c0[t] = acc;
+    }
+    __syncthreads();
+    // spread over the pixels of each row from the 2 pad + 1 cells around: [hp][w]
+    for (int t = tid; t < g.hp * w; t += 256) {
+        const int tr = t / w, x = t - (t / w) * w;
+        const int own = x / s;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int q = 0; q <= 2 * pad; ++q) {
+            const float2 cw = prep_cell(own + q - pad, w, s);
+            const float d = (float)x - cw.x;
+            const float e = cw.y * expf(-d * d * g.k2);
+            const float2 v = c0[tr * g.wp + own + q];
+            acc = make_float2(acc.x + e * v.x, acc.y + e * v.y);
         }
-        __syncthreads();
-        // spread over the pixels of each row from the 2 pad + 1 cells around: [hp][w]
-        for (int t = tid; t < a.hp * a.w; t += 256) {
-            const int tr = t / a.w, x = t - (t / a.w) * a.w;
-            const int own = x / s;
-            float2 acc = make_float2(0.f, 0.f);
-            for (int q = 0; q <= 2 * pad; ++q) {
-                const float2 cw = prep_cell(own + q - pad, a.w, s);
-                const float d = (float)x - cw.x;
-                const float g = cw.y * expf(-d * d * a.k2);
-                const float2 v = c0[tr * a.wp + own + q];
-                acc = make_float2(acc.x + g * v.x, acc.y + g * v.y);
-            }
-            c1[t] = acc;
+        c1[t] = acc;
+    }
+    __syncthreads();
+    // and of each column, into the pixels
+    for (int i = tid; i < hw; i += 256) {
+        const int y = i / w, x = i - (i / w) * w;
+        const int own = y / s;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int q = 0; q <= 2 * pad; ++q) {
+            const float2 cw = prep_cell(own + q - pad, h, s);
+            const float d = (float)y - cw.x;
+            const float e = cw.y * expf(-d * d * g.k2);
+            const float2 v = c1[(own + q) * w + x];
+            acc = make_float2(acc.x + e * v.x, acc.y + e * v.y);
         }
-        __syncthreads();
-        // and of each column, into the pixels: the blurred value, combined
-        for (int i = tid; i < hw; i += 256) {
-            const int y = i / a.w, x = i - (i / a.w) * a.w;
-            const int own = y / s;
-            float2 acc = make_float2(0.f, 0.f);
-            for (int q = 0; q <= 2 * pad; ++q) {
-                const float2 cw = prep_cell(own + q - pad, a.h, s);
-                const float d = (float)y - cw.x;
-                const float g = cw.y * expf(-d * d * a.k2);
-                const float2 v = c1[(own + q) * a.w + x];
-                acc = make_float2(acc.x + g * v.x, acc.y + g * v.y);
-            }
-            img[i] = combine(img[i], acc.x / acc.y);
+        emit(i, acc.x / acc.y);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int clahe_reflect(int i, int n, int pad) {
+    if (n <= 1) return 0;
+    int x = abs(i - pad);
+    if (x >= n) x = 2 * n - 2 - x;
+    return min(max(x, 0), n - 1);
+}
+
+// One tile's table from its histogram (prep.wgsl:343-364), in place (each level read before it
+// is written): clipped at clip x the mean count, what was cut spread evenly (the remainder one
+// count each to the first levels), accumulated and scaled by the tile's pixel count.
+__device__ void clahe_table(unsigned* hist, int B, float voxels, float clip) {
+    float* cdf = reinterpret_cast<float*>(hist);
+    const float nb = (float)B;
+    const float limit = fmaxf(clip * voxels / nb, 1.f);
+    float sumc = 0.f;
+    for (int i = 0; i < B; ++i) sumc += fminf((float)hist[i], limit);
+    const float clipped = voxels - sumc;
+    const int residual = (int)(clipped - nb * floorf(clipped / nb));
+    const float redist = (clipped - (float)residual) / nb;
+    float acc = 0.f;
+    for (int i = 0; i < B; ++i) {
+        float c = fminf((float)hist[i], limit) + redist;
+        if (i < residual) c += 1.f;
+        acc += c;
+        cdf[i] = fminf(fmaxf(acc / voxels, 0.f), 1.f);
+    }
+}
+
+// The equalized value of pixel (row, col) holding v (prep.wgsl:366-397): the four nearest
+// tiles' tables [T][T][B], each read between its two nearest levels.
+__device__ float clahe_value(float v, int row, int col, int h, int w, int T, int B, const float* cdf) {
+    auto mix = [](float x, float y, float t) { return x * (1.f - t) + y * t; };
+    v = fminf(fmaxf(v, 0.f), 1.f);
+    const float gy = ((float)row + 0.5f) / (float)h * (float)(T - 1);
+    const float gx = ((float)col + 0.5f) / (float)w * (float)(T - 1);
+    const int y0 = (int)floorf(gy), x0 = (int)floorf(gx);
+    const int y1 = min(y0 + 1, T - 1), x1 = min(x0 + 1, T - 1);
+    const float fy = gy - (float)y0, fx = gx - (float)x0;
+    const float bf = v * (float)(B - 1);
+    const int b0 = (int)fminf(fmaxf(floorf(bf), 0.f), (float)(B - 2));
+    const float fb = bf - (float)b0;
+    auto at = [&](int ty, int tx) {
+        const float* c = cdf + (ty * T + tx) * B;
+        return mix(c[b0], c[b0 + 1], fb);
+    };
+    return mix(mix(at(y0, x0), at(y0, x1), fx), mix(at(y1, x0), at(y1, x1), fx), fy);
+}
+
+__device__ __forceinline__ int clahe_bin(float v, int B) { return (int)fminf(fmaxf(fminf(fmaxf(v, 0.f), 1.f) * (float)(B - 1), 0.f), (float)(B - 1)); }
+
+// CLAHE of the h x w image in LDS, in place (prep.wgsl:289-397): stretched to [0, 1] by its
+// extremes; padded by reflection at its top and left to whole tiles; each tile's histogram
+// (LDS atomics, `hist` tiles^2 bins words) turned into its table; each pixel equalized.
+__device__ void clahe_block(float* img, unsigned* hist, const PrepArg& a, float* red) {
+    const int h = a.h, w = a.w, hw = h * w;
+    const int tid = threadIdx.x;
+    const int T = a.tiles, B = a.bins;
+    float lo = 3.4e38f, hi = -3.4e38f;
+    for (int i = tid; i < hw; i += 256) {
+        lo = fminf(lo, img[i]);
+        hi = fmaxf(hi, img[i]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((tid & 63) == 0) {
+        red[tid >> 6] = lo;
+        red[4 + (tid >> 6)] = hi;
+    }
+    for (int i = tid; i < T * T * B; i += 256) hist[i] = 0u;
+    __syncthreads();
+    lo = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    hi = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
+    const float k = 1.f / fmaxf(hi - lo, 1e-20f);
+    for (int i = tid; i < hw; i += 256) img[i] = (img[i] - lo) * k;
+    __syncthreads();
+    const int hp = h + a.pad_h, wp = w + a.pad_w;
+    for (int t = tid; t < hp * wp; t += 256) {
+        const int py = t / wp, px = t - (t / wp) * wp;
+        const int bin = clahe_bin(img[clahe_reflect(py, h, a.pad_h) * w + clahe_reflect(px, w, a.pad_w)], B);
+        const int tile = (py / a.tile_h) * T + px / a.tile_w;
+        atomicAdd(&hist[tile * B + bin], 1u);
+    }
+    __syncthreads();
+    for (int t = tid; t < T * T; t += 256) clahe_table(hist + t * B, B, (float)max(a.tile_h * a.tile_w, 1), a.clip);
+    __syncthreads();
+    const float* cdf = reinterpret_cast<const float*>(hist);
+    for (int i = tid; i < hw; i += 256) {
+        const int row = i / w, col = i - (i / w) * w;
+        img[i] = clahe_value(img[i], row, col, h, w, T, B, cdf);
+    }
+    __syncthreads();
+}
+
+// CLAHE of images too large for one block's LDS (a master's sheets), in three launches over
+// global memory: the extremes and the stretch, one block an image; the tables, one block a
+// (image, tile) with its histogram in LDS; the equalized pixels.
+struct ClaheArg {
+    int n, h, w, tiles, bins;
+    float clip;
+    int pad_h, pad_w, tile_h, tile_w;
+};
+
+__global__ void __launch_bounds__(256) k_clahe_stretch(float* __restrict__ img, ClaheArg a) {
+    __shared__ float red[8];
+    float* x = img + (size_t)blockIdx.x * a.h * a.w;
+    const int hw = a.h * a.w, tid = threadIdx.x;
+    float lo = 3.4e38f, hi = -3.4e38f;
+    for (int i = tid; i < hw; i += 256) {
+        lo = fminf(lo, x[i]);
+        hi = fmaxf(hi, x[i]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((tid & 63) == 0) {
+        red[tid >> 6] = lo;
+        red[4 + (tid >> 6)] = hi;
+    }
+    __syncthreads();
+    lo = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    hi = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
+    const float k = 1.f / fmaxf(hi - lo, 1e-20f);
+    for (int i = tid; i < hw; i += 256) x[i] = (x[i] - lo) * k;
+}
+
+__global__ void __launch_bounds__(256) k_clahe_tables(const float* __restrict__ img, ClaheArg a, float* __restrict__ cdf) {
+    __shared__ unsigned hist[256];
+    const int T = a.tiles, B = a.bins;
+    const int b = blockIdx.x / (T * T), tile = blockIdx.x - b * (T * T);
+    const int ty = tile / T, tx = tile - ty * T;
+    const float* x = img + (size_t)b * a.h * a.w;
+    for (int i = threadIdx.x; i < B; i += 256) hist[i] = 0u;
+    __syncthreads();
+    for (int t = threadIdx.x; t < a.tile_h * a.tile_w; t += 256) {
+        const int py = ty * a.tile_h + t / a.tile_w, px = tx * a.tile_w + t % a.tile_w;
+        atomicAdd(&hist[clahe_bin(x[clahe_reflect(py, a.h, a.pad_h) * a.w + clahe_reflect(px, a.w, a.pad_w)], B)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) clahe_table(hist, B, (float)max(a.tile_h * a.tile_w, 1), a.clip);
+    __syncthreads();
+    for (int i = threadIdx.x; i < B; i += 256) cdf[(size_t)blockIdx.x * B + i] = reinterpret_cast<const float*>(hist)[i];
+}
+
+__global__ void __launch_bounds__(256) k_clahe_apply(float* __restrict__ img, ClaheArg a, const float* __restrict__ cdf) {
+    const long tid = (long)blockIdx.x * 256 + threadIdx.x;
+    const long hw = (long)a.h * a.w;
+    if (tid >= (long)a.n * hw) return;
+    const int b = (int)(tid / hw);
+    const int pix = (int)(tid - (long)b * hw);
+    img[tid] = clahe_value(img[tid], pix / a.w, pix % a.w, a.h, a.w, a.tiles, a.bins, cdf + (size_t)b * a.tiles * a.tiles * a.bins);
+}
+
+// The filter chain and the normalization, one block per pattern with the image in LDS: the
+// static background subtracted on the way in, the dynamic background, CLAHE, the band pass
+// (the coarse Gaussian parked in the pattern's output row), then zero mean and unit norm under
+// the mask; HBM sees the raw pattern and the result once each.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prep(const T* __restrict__ raw, const float* __restrict__ bg, PrepArg a, float* __restrict__ out) {
+    extern __shared__ float lds[];
+    __shared__ float red[8];
+    const int hw = a.h * a.w;
+    const int tid = threadIdx.x;
+    float* img = lds;
+    float* aux = lds + ((hw + 3) & ~3);
+    const T* src = raw + (size_t)blockIdx.x * hw;
+    float* dst = out + (size_t)blockIdx.x * hw;
+    for (int i = tid; i < hw; i += 256) img[i] = (float)src[i] - (bg ? bg[i] : 0.f);
+    __syncthreads();
+    if (a.dyn) {
+        const int dyn = a.dyn;
+        gauss(img, aux, a.h, a.w, a.dynb, [&](int i, float b) { img[i] = dyn == 1 ? img[i] - b : img[i] / fmaxf(b, 1e-6f); });
+    }
+    if (a.clahe) clahe_block(img, reinterpret_cast<unsigned*>(aux), a, red);
+    if (a.bp) {
+        // the coarse Gaussian into the output row (each pixel back on the thread that wrote it)
+        gauss(img, aux, a.h, a.w, a.coarse, [&](int i, float b) { dst[i] = b; });
+        if (a.bp_fine) {
+            gauss(img, aux, a.h, a.w, a.fine, [&](int i, float b) { img[i] = b - dst[i]; });
+        } else {
+            for (int i = tid; i < hw; i += 256) img[i] -= dst[i];
+            __syncthreads();
         }
-        __syncthreads();
     }
     // zero mean and unit norm under the mask (as k_normalize)
     float sm = 0.f, cn = 0.f;
@@ -272,7 +468,6 @@ __global__ void __launch_bounds__(256) k_prep(const T* __restrict__ raw, const f
     }
     ss = block_sum<256>(ss, red);
     const float inv = 1.f / sqrtf(fmaxf(ss, 1e-30f));
-    float* dst = out + (size_t)blockIdx.x * hw;
     for (int i = tid; i < hw; i += 256) dst[i] = img[i] * inv;
 }
 
@@ -358,6 +553,22 @@ __global__ void __launch_bounds__(256) k_gradient(const float* __restrict__ src,
     grad[tid] = make_float4(g1 * e1.x + g2 * e2.x, g1 * e1.y + g2 * e2.y, g1 * e1.z + g2 * e2.z, src[tid]);
 }
 
+// The mean over the master's point group (ops [n_ops][3][3], row-major): each texel the
+// average of the sheets at op . d (mastersph.wgsl:111-122).
+__global__ void __launch_bounds__(256) k_symmetrize(const float* __restrict__ src, float* __restrict__ dst, int side, const float* __restrict__ ops,
+                                                    int n_ops) {
+    const int tid = blockIdx.x * 256 + threadIdx.x;
+    if (tid >= 2 * side * side) return;
+    const float3 d = texel_dir(tid, side);
+    float acc = 0.f;
+    for (int g = 0; g < n_ops; ++g) {
+        const float* m = ops + g * 9;
+        const float3 v = make_float3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[3] * d.x + m[4] * d.y + m[5] * d.z, m[6] * d.x + m[7] * d.y + m[8] * d.z);
+        acc += sheet_bilinear<float>(src, side, normalize3(v));
+    }
+    dst[tid] = acc / (float)max(n_ops, 1);
+}
+
 }  // namespace zb
 
 using namespace zb;
@@ -394,11 +605,40 @@ extern "C" int32_t zb_normalize(const void* raw, int32_t pixel, int32_t n, int32
     return check_launch("zb_normalize");
 }
 
+// The Gaussian's parameters for deviation sigma pixels on an h x w image, and the LDS words its
+// scratch takes (beside the image).
+static size_t blur_arg(double sigma, int h, int w, BlurArg& g) {
+    const long hw = (long)h * w;
+    const int s = (int)floor(sigma / 6);
+    g = BlurArg{};
+    if (s < 2) {
+        g.box = 0;
+        g.radius = (int)fmax(1.0, ceil(3 * sigma));
+        g.k = (float)(1.0 / (2 * sigma * sigma));
+        return (size_t)((hw + 3) & ~3L) + (size_t)(g.radius + 1);
+    }
+    const double cs = sqrt(sigma * sigma - (double)s * s - ((double)s * s - 1) / 12);
+    g.box = 1;
+    g.s = s;
+    g.pad = 3;
+    g.reach = (float)ceil(4 * cs);
+    g.k0 = (float)(1.0 / (2 * cs * cs));
+    g.k2 = (float)(1.0 / (2.0 * s * s));
+    g.hs = (h + s - 1) / s;
+    g.ws = (w + s - 1) / s;
+    g.hp = g.hs + 2 * g.pad;
+    g.wp = g.ws + 2 * g.pad;
+    return (size_t)2 * g.hp * (g.wp > w ? g.wp : w) * 2;
+}
+
 extern "C" int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* o,
                            float* out, void* stream) {
     if (n < 0 || h < 2 || w < 2 || (n > 0 && (!raw || !out)) || !mask || !o || pixel < 0 || pixel > 2 || o->dynamic < 0 ||
-        o->dynamic > 2 || (o->dynamic && !(o->dynamic_sigma > 0.f))) {
-        set_error("zb_prep: bad arguments (n %d, h %d, w %d, pixel %d)", n, h, w, pixel);
+        o->dynamic > 2 || (o->dynamic && !(o->dynamic_sigma > 0.f)) ||
+        (o->clahe && (o->clahe_grid < 1 || o->clahe_grid > 16 || o->clahe_bins < 2 || o->clahe_bins > 256 || !(o->clahe_clip > 0.f))) ||
+        (o->bandpass && !(o->bp_fine >= 0.f && o->bp_coarse > o->bp_fine))) {
+        set_error("zb_prep: bad arguments (n %d, h %d, w %d, pixel %d, dynamic %d, clahe %d, bandpass %d)", n, h, w, pixel, o ? o->dynamic : 0,
+                  o ? o->clahe : 0, o ? o->bandpass : 0);
         return ZB_EINVAL;
     }
     if (n == 0) return ZB_OK;
@@ -408,30 +648,28 @@ extern "C" int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h,
     for (int i = 0; i < 3; ++i) a.mask[i] = mask[i];
     a.dyn = o->dynamic;
     const long hw = (long)h * w;
-    size_t lds = (size_t)((hw + 3) & ~3L) * 4;  // the image
-    if (a.dyn) {
-        const double sigma = o->dynamic_sigma;
-        const int s = (int)floor(sigma / 6);
-        if (s < 2) {
-            a.box = 0;
-            a.radius = (int)fmax(1.0, ceil(3 * sigma));
-            a.k = (float)(1.0 / (2 * sigma * sigma));
-            lds += (size_t)((hw + 3) & ~3L) * 4 + (size_t)(a.radius + 1) * 4;
-        } else {
-            const double cs = sqrt(sigma * sigma - (double)s * s - ((double)s * s - 1) / 12);
-            a.box = 1;
-            a.s = s;
-            a.pad = 3;
-            a.reach = (float)ceil(4 * cs);
-            a.k0 = (float)(1.0 / (2 * cs * cs));
-            a.k2 = (float)(1.0 / (2.0 * s * s));
-            a.hs = (h + s - 1) / s;
-            a.ws = (w + s - 1) / s;
-            a.hp = a.hs + 2 * a.pad;
-            a.wp = a.ws + 2 * a.pad;
-            lds += (size_t)2 * a.hp * (a.wp > w ? a.wp : w) * 8;
+    size_t scratch = 0;  // LDS words beside the image
+    if (a.dyn) scratch = std::max(scratch, blur_arg(o->dynamic_sigma, h, w, a.dynb));
+    if (o->clahe) {
+        a.clahe = 1;
+        a.tiles = o->clahe_grid;
+        a.bins = o->clahe_bins;
+        a.clip = o->clahe_clip;
+        a.pad_h = (a.tiles - h % a.tiles) % a.tiles;
+        a.pad_w = (a.tiles - w % a.tiles) % a.tiles;
+        a.tile_h = std::max(1, (h + a.pad_h) / a.tiles);
+        a.tile_w = std::max(1, (w + a.pad_w) / a.tiles);
+        scratch = std::max(scratch, (size_t)a.tiles * a.tiles * a.bins);
+    }
+    if (o->bandpass) {
+        a.bp = 1;
+        scratch = std::max(scratch, blur_arg(o->bp_coarse, h, w, a.coarse));
+        if (o->bp_fine > 0.05f) {
+            a.bp_fine = 1;
+            scratch = std::max(scratch, blur_arg(o->bp_fine, h, w, a.fine));
         }
     }
+    const size_t lds = ((size_t)((hw + 3) & ~3L) + scratch) * 4;
     int dev = 0, limit = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
         set_error("zb_prep: no device");
@@ -485,6 +723,55 @@ extern "C" int32_t zb_master_gradient(const float* sheets, int32_t side, float* 
     float h = (float)(2.5066282746310002 / (double)side);  // sqrt(2 pi) / side
     ZB_LAUNCH("k_gradient", k_gradient, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, sheets, side, h, (float4*)grad);
     return check_launch("zb_master_gradient");
+}
+
+extern "C" int64_t zb_clahe_workspace(int32_t n, int32_t grid, int32_t bins) {
+    if (n < 0 || grid < 1 || grid > 64 || bins < 2 || bins > 256) {
+        set_error("zb_clahe_workspace: bad arguments (n %d, grid %d, bins %d)", n, grid, bins);
+        return ZB_EINVAL;
+    }
+    return (int64_t)n * grid * grid * bins * 4;
+}
+
+extern "C" int32_t zb_clahe(float* img, int32_t n, int32_t h, int32_t w, int32_t grid, int32_t bins, float clip, void* work, int64_t work_bytes,
+                            void* stream) {
+    if (n < 0 || h < 2 || w < 2 || grid < 1 || grid > 64 || bins < 2 || bins > 256 || !(clip > 0.f) || (n > 0 && (!img || !work))) {
+        set_error("zb_clahe: bad arguments (n %d, h %d, w %d, grid %d, bins %d)", n, h, w, grid, bins);
+        return ZB_EINVAL;
+    }
+    if (n == 0) return ZB_OK;
+    if (work_bytes < zb_clahe_workspace(n, grid, bins)) {
+        set_error("zb_clahe: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)zb_clahe_workspace(n, grid, bins));
+        return ZB_ESPACE;
+    }
+    ClaheArg a;
+    a.n = n;
+    a.h = h;
+    a.w = w;
+    a.tiles = grid;
+    a.bins = bins;
+    a.clip = clip;
+    a.pad_h = (grid - h % grid) % grid;
+    a.pad_w = (grid - w % grid) % grid;
+    a.tile_h = std::max(1, (h + a.pad_h) / grid);
+    a.tile_w = std::max(1, (w + a.pad_w) / grid);
+    hipStream_t s = (hipStream_t)stream;
+    float* cdf = (float*)work;
+    ZB_LAUNCH("k_clahe_stretch", k_clahe_stretch, dim3(n), dim3(256), 0, s, img, a);
+    ZB_LAUNCH("k_clahe_tables", k_clahe_tables, dim3((unsigned)(n * grid * grid)), dim3(256), 0, s, (const float*)img, a, cdf);
+    const long total = (long)n * h * w;
+    ZB_LAUNCH("k_clahe_apply", k_clahe_apply, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, img, a, (const float*)cdf);
+    return check_launch("zb_clahe");
+}
+
+extern "C" int32_t zb_master_symmetrize(const float* src, float* dst, int32_t side, const float* ops, int32_t n_ops, void* stream) {
+    if (!src || !dst || src == dst || side < 2 || n_ops < 1 || !ops) {
+        set_error("zb_master_symmetrize: bad arguments");
+        return ZB_EINVAL;
+    }
+    const int total = 2 * side * side;
+    ZB_LAUNCH("k_symmetrize", k_symmetrize, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, src, dst, side, ops, n_ops);
+    return check_launch("zb_master_symmetrize");
 }
 
 // The gradient field in half precision (each float rounded to nearest even): 8 bytes a texel, so

@@ -256,11 +256,41 @@ class Engine:
     # ── masters and tables ──
 
     def bake_key(self, p):
-        return master_mod.bake_sigma(p.master.side, self.pixel_angle(), self.settings["master_smooth"])
+        """What the phase's master is filtered for (src/engine.zig:1674-1697): the smoothing of a
+        pattern pixel, and the patterns' band pass and CLAHE at their sizes on the sphere."""
+        s = self.settings
+        px = master_mod.bake_px(p.master.side, self.pixel_angle())
+        if px == 0.0:
+            return (0.0, None, None)
+        smooth = s["master_smooth"] * px
+        bp = (s["bp_fine"] * px, s["bp_coarse"] * px) if s["bandpass"] else None
+        cl = None
+        if s["clahe"]:
+            # tiles as wide on the sphere as the patterns': pattern height / grid pixels
+            tile = self.pattern_hw()[0] / s["clahe_grid"] * px
+            cl = (int(np.clip(np.round(p.master.side / tile), 1, 64)), int(s["clahe_bins"]), float(s["clahe_clip"]))
+        return (smooth if (smooth > 0.05 or bp or cl) else 0.0, bp, cl)
+
+    def _sphere_blur(self, cur, side, sigma):
+        """The master's sheets smoothed on the sphere by sigma sheet pixels (passes of doubling
+        width, src/prep.zig:215-237); a new buffer when any pass runs."""
+        lib = nat.load()
+        passes = master_mod.blur_passes(sigma)
+        if not passes:
+            return cur
+        cur = cur.clone()
+        tmp = torch.empty_like(cur)
+        px_rad = np.sqrt(2 * np.pi) / side
+        for s in passes:
+            nat.check(lib.zb_master_blur(cur.data_ptr(), tmp.data_ptr(), side, float(s * px_rad), stream_ptr()), "master blur")
+            cur, tmp = tmp, cur
+        return cur
 
     def baked(self, pi):
-        """The phase's master smoothed on the sphere for this pattern size and geometry, renormalized,
-        and its gradient field (src/engine.zig:1702-1788)."""
+        """The phase's master filtered for this pattern size and geometry (src/engine.zig:1716-1788):
+        smoothed on the sphere by what a pattern pixel averages; with CLAHE, equalized as two
+        images and averaged over the master's point group; with the band pass, the difference of
+        two smoothings on the sphere; renormalized over both sheets; and its gradient field."""
         p = self.phases[pi]
         key = self.bake_key(p)
         if key in p.baked:
@@ -271,13 +301,28 @@ class Engine:
         side = m.side
         cur = _dev(m.raw, dev).reshape(-1)
         host = m.raw
-        passes = master_mod.blur_passes(key)
-        if passes:
-            tmp = torch.empty_like(cur)
-            px_rad = np.sqrt(2 * np.pi) / side
-            for s in passes:
-                nat.check(lib.zb_master_blur(cur.data_ptr(), tmp.data_ptr(), side, float(s * px_rad), stream_ptr()), "master blur")
-                cur, tmp = tmp, cur
+        smooth, bp, cl = key
+        if smooth > 0.0:
+            fine = bp[0] if bp else 0.0
+            if cl:
+                cur = self._sphere_blur(cur, side, smooth)  # (a buffer of its own either way)
+                ws = torch.empty((int(lib.zb_clahe_workspace(2, cl[0], cl[1])),), dtype=torch.uint8, device=dev)
+                nat.check(lib.zb_clahe(cur.data_ptr(), 2, side, side, cl[0], cl[1], cl[2], ws.data_ptr(), ws.numel(), stream_ptr()), "master clahe")
+                if len(m.ops) > 1:
+                    ops = torch.from_numpy(np.asarray(m.ops, np.float32).reshape(-1)).to(dev)
+                    out = torch.empty_like(cur)
+                    nat.check(lib.zb_master_symmetrize(cur.data_ptr(), out.data_ptr(), side, ops.data_ptr(), len(m.ops), stream_ptr()),
+                              "master symmetrize")
+                    cur = out
+            else:
+                # what the pixel averages and the band pass's fine Gaussian compose into one
+                fine = float(np.sqrt(smooth * smooth + fine * fine))
+                if not bp:
+                    cur = self._sphere_blur(cur, side, smooth)
+            if bp:
+                # the difference of two smoothings on the sphere
+                low = self._sphere_blur(cur, side, bp[1])
+                cur = self._sphere_blur(cur, side, fine) - low
             host = master_mod.normalize_sheets(cur.cpu().numpy().astype(np.float64).reshape(2, side, side))
             cur = _dev(host, dev).reshape(-1)
         grad = torch.empty((2 * side * side * 4,), dtype=torch.float32, device=dev)
@@ -439,7 +484,7 @@ class Engine:
         s = self.settings
         mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
         kind = TORCH_PIXEL_KIND[raw.dtype]
-        if s["static_background"] == "none" and s["dynamic_background"] == "none":
+        if s["static_background"] == "none" and s["dynamic_background"] == "none" and not s["clahe"] and not s["bandpass"]:
             nat.check(lib.zb_normalize(raw.data_ptr(), kind, n, h, w, mask, out.data_ptr(), st), "normalize")
             return
         o = nat.PrepOpts()
@@ -447,6 +492,10 @@ class Engine:
         o.background = bg.data_ptr() if bg is not None else None
         o.dynamic = {"none": 0, "subtract": 1, "divide": 2}[s["dynamic_background"]]
         o.dynamic_sigma = float(s["dynamic_sigma"]) * w
+        o.clahe = int(bool(s["clahe"]))
+        o.clahe_grid, o.clahe_bins, o.clahe_clip = int(s["clahe_grid"]), int(s["clahe_bins"]), float(s["clahe_clip"])
+        o.bandpass = int(bool(s["bandpass"]))
+        o.bp_fine, o.bp_coarse = float(s["bp_fine"]), float(s["bp_coarse"])  # pixels of the pattern as stored (binning 1)
         nat.check(lib.zb_prep(raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), out.data_ptr(), st), "prep")
 
     def run_batch(self, raw, items, L=None, workspace=None, lane=None):

@@ -257,6 +257,15 @@ def load(path):
 
 # ── the master as filtered for a pattern size ──
 
+def bake_px(side, pixel_angle):
+    """A detector pixel `pixel_angle` radians wide in sheet pixels, quantised to 5 % steps as the
+    reference keys its baked masters (src/engine.zig:1679-1683). 0 without a pixel angle."""
+    if not (pixel_angle > 0):
+        return 0.0
+    px = pixel_angle * side / np.sqrt(2 * np.pi)
+    return float(1.05 ** int(np.round(np.log(px) / np.log(1.05))))
+
+
 def bake_sigma(side, pixel_angle, master_smooth):
     """The Gaussian's sigma in sheet pixels for a detector pixel `pixel_angle` radians wide,
     quantised to 5 % steps of the pixel's width as the reference keys its baked masters

@@ -2,8 +2,8 @@
 
 `apply` rejects unknown keys and values out of range atomically, as Settings.apply/check does
 (src/settings.zig:404-463). Keys of the reference that belong to stages outside this build
-(NLPAR, CLAHE, the band pass, dictionary and PCA search, binning, the fits) are accepted only at
-the values that leave those stages out, and say so otherwise.
+(NLPAR, dictionary and PCA search, binning, the fits) are accepted only at the values that leave
+those stages out, and say so otherwise.
 """
 import copy
 import math
@@ -17,10 +17,11 @@ DEFAULTS = {
     "scan_x_sign": -1, "scan_y_sign": -1,
     # ── resolution ──
     "binning": 1.0, "full_resolution": "none",
-    # ── averaging and pattern filters, in the order applied (NLPAR, CLAHE and the band pass are
-    # outside this build: only their "off" values) ──
+    # ── averaging and pattern filters, in the order applied (NLPAR is outside this build: only
+    # its "off" value) ──
     "nlpar": False, "static_background": "average", "static_patterns": 1024, "dynamic_background": "subtract",
-    "dynamic_sigma": 0.15, "clahe": False, "bandpass": False,
+    "dynamic_sigma": 0.15, "clahe": False, "clahe_grid": 4, "clahe_bins": 64, "clahe_clip": 40.0,
+    "bandpass": False, "bp_fine": 1.0, "bp_coarse": 8.0,
     # ── mask ──
     "mask": True, "mask_radius": 1.0,
     # ── master ──
@@ -44,8 +45,8 @@ DEFAULTS = {
 
 # values a key may take where this build has only some of the reference's
 _ONLY = {
-    "nlpar": (False,), "static_background": ("average", "file", "none"), "dynamic_background": ("subtract", "divide", "none"), "clahe": (False,),
-    "bandpass": (False,), "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",),
+    "nlpar": (False,), "static_background": ("average", "file", "none"), "dynamic_background": ("subtract", "divide", "none"),
+    "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",),
     "sweep_method": ("same", "spherical"),
 }
 
@@ -77,6 +78,8 @@ def check(s):
     bad(not (1 <= s["sweeps"] <= 6), "sweeps")
     bad(not (s["static_patterns"] >= 1), "static_patterns")
     bad(not (s["dynamic_sigma"] > 0), "dynamic_sigma")
+    bad(not (1 <= s["clahe_grid"] <= 16 and 2 <= s["clahe_bins"] <= 256 and s["clahe_clip"] > 0), "clahe_grid")
+    bad(not (s["bp_fine"] >= 0 and s["bp_coarse"] > s["bp_fine"]), "bp_fine")
     bad(not (8 <= s["sweep_bandwidth"] <= 256), "sweep_bandwidth")
     bad(s["correlation_precision"] not in ("fp32", "fp16", "bf16"), "correlation_precision")
     bad(not (0 <= s["split_degree"] <= 256), "split_degree")
