@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MASTER = os.path.join(ROOT, "tests", "golden", "Ni_master_20kV.npz")
+GAN_MASTER = os.path.join(ROOT, "tests", "golden", "GaN_master_20kV.npz")
 PC = (0.50, 0.50, 0.55)
 H = W = 60
 FP32_PEAK_TFLOPS = 157.3   # MI355X published dense FP32 (MFMA and packed VALU)
@@ -97,9 +98,11 @@ def mask_count(h, w):
 
 # ── the workload ──
 
-def make_workload(e, n, seed, counts):
-    """Render n patterns on the GPU (zb_project), Poisson noise on the GPU. Returns the raw
-    patterns [n, H, W] on the device, items [n, 4] (t, pattern index) and the true orientations."""
+def make_workload(e, n, seed, counts, p_first=1.0):
+    """Render n patterns on the GPU (zb_project), Poisson noise on the GPU. With two phases, each
+    pattern is drawn from the first with probability p_first (Bernoulli), else from the second.
+    Returns the raw patterns [n, H, W] on the device, items [n, 4] (t, pattern index), the true
+    orientations and the true phases."""
     import torch
     from zebsdsolve_amd import _native as nat, engine
     rng = np.random.default_rng(seed)
@@ -107,20 +110,25 @@ def make_workload(e, n, seed, counts):
     q /= np.linalg.norm(q, axis=1, keepdims=True)
     q[q[:, 0] < 0] *= -1
     pcs = np.asarray(PC)[None, :] + rng.uniform(-0.01, 0.01, size=(n, 3))
+    phase = (rng.random(n) >= p_first).astype(np.int64) if len(e.phases) > 1 else np.zeros(n, np.int64)
     R = e.geo.rotation()
     t = np.stack([R @ np.array([0.5 - p[1] * H, 0.5 - p[0] * W, p[2] * H]) for p in pcs]).astype(np.float32)
     poses = np.concatenate([t, np.zeros((n, 1), np.float32), q.astype(np.float32)], 1)
     lib = nat.load()
-    _, (sheets, _, _) = e.baked(0)
-    side = e.phases[0].master.side
     dev = torch.device("cuda")
     sims = torch.empty((n, H, W), dtype=torch.float32, device=dev)
-    pd = torch.from_numpy(poses).to(dev)
     fr = e.frame()
     step = 16384
-    for c0 in range(0, n, step):
-        c1 = min(n, c0 + step)
-        nat.check(lib.zb_project(sheets.data_ptr(), side, fr, pd[c0:c1].data_ptr(), c1 - c0, sims[c0:c1].data_ptr(), engine.stream_ptr()), "project")
+    for pi in range(len(e.phases)):
+        _, (sheets, _, _) = e.baked(pi)
+        side = e.phases[pi].master.side
+        idx = np.flatnonzero(phase == pi)
+        for c0 in range(0, len(idx), step):
+            sel = idx[c0:c0 + step]
+            pd = torch.from_numpy(poses[sel]).to(dev)
+            out = torch.empty((len(sel), H, W), dtype=torch.float32, device=dev)
+            nat.check(lib.zb_project(sheets.data_ptr(), side, fr, pd.data_ptr(), len(sel), out.data_ptr(), engine.stream_ptr()), "project")
+            sims[torch.from_numpy(sel).to(dev)] = out
     lo = sims.min()
     mean = sims.mean()
     gen = torch.Generator(device=dev)
@@ -128,7 +136,7 @@ def make_workload(e, n, seed, counts):
     pats = torch.poisson(counts * (sims - lo) / (mean - lo), generator=gen).float()
     del sims
     items = np.concatenate([t, np.arange(n, dtype=np.float32)[:, None]], 1)
-    return pats, items, q
+    return pats, items, q, phase
 
 
 def chunks_of(n, seed, lo=1000, hi=8000):
@@ -205,6 +213,10 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp16", "bf16"), default="fp32",
                     help="products of the correlation's GEMM: float32, or 16-bit with float32 accumulation (hi + lo for l >= 64)")
     ap.add_argument("--size", type=int, default=60, help="pattern side in pixels (60: the plain FP32 form; 120: the precision form)")
+    ap.add_argument("--phases", choices=("ni", "ni,gan"), default="ni",
+                    help="masters: nickel alone, or nickel and GaN with each pattern drawn from nickel at --p-first")
+    ap.add_argument("--p-first", type=float, default=0.7, help="with two phases, the share of patterns drawn from nickel")
+    ap.add_argument("--chunks", default="1000,8000", help="chunk sizes U[lo, hi] (the two-phase workload: 512,16384)")
     ap.add_argument("--lanes", type=int, default=1, help="streams the chunks are spread over (1: one stream)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
@@ -237,24 +249,29 @@ def main():
     e.settings.apply(static_background="none", dynamic_background="none")
     e.geo = Geometry(pc=PC, sample_tilt_deg=70.0)
     e.add_phase(MASTER)
+    two = args.phases == "ni,gan"
+    if two:
+        e.add_phase(GAN_MASTER)
     e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters, correlation_precision=args.precision)
     e.correlator = args.correlator
     e.set_patterns(np.zeros((1, 1, H, W), np.float32))
     n = args.patterns
-    pats, items, truth = make_workload(e, n, args.seed, args.counts)
+    pats, items, truth, truth_phase = make_workload(e, n, args.seed, args.counts, args.p_first if two else 1.0)
     items_d = torch.from_numpy(items).cuda()
-    chunks = chunks_of(n, args.seed)
+    clo, chi = (int(x) for x in args.chunks.split(","))
+    chunks = chunks_of(n, args.seed, clo, chi)
     # per chunk, item indices local to the chunk
     local = []
     for c0, c1 in chunks:
         it = items_d[c0:c1].clone()
         it[:, 3] = torch.arange(c1 - c0, dtype=torch.float32, device=it.device)
         local.append(it)
-    ph = e.phase_band(0, args.bandwidth)
-    if e.use_mfma(args.bandwidth):
-        e.vrden(0, args.bandwidth)
-    else:
-        e.rden(0, args.bandwidth)
+    phs = [e.phase_band(pi, args.bandwidth) for pi in range(len(e.phases))]
+    for pi in range(len(e.phases)):
+        if e.use_mfma(args.bandwidth, pi):
+            e.vrden(pi, args.bandwidth)
+        else:
+            e.rden(pi, args.bandwidth)
     torch.cuda.synchronize()
 
     lanes = e.lanes(args.lanes) if args.lanes > 1 else [None]
@@ -282,20 +299,31 @@ def main():
     el = t1 - t0
     value = args.steps * n / el
 
-    rec = torch.cat([r[0] for r in recs]).cpu().numpy()
-    rots = e.phases[0].master.rotations
-    dis = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), truth, rots))
+    allrec = torch.cat(recs, dim=1).cpu().numpy()  # [phases, n, 8]
+    # the phase of the best NMI (the reference's default phase_by; ties to the lower phase)
+    chosen = np.argmax(allrec[:, :, 5], axis=0)
+    rec = allrec[chosen, np.arange(n)]
+    dis = np.full(n, 180.0)
+    right = chosen == truth_phase
+    for pi in range(len(e.phases)):
+        sel = right & (truth_phase == pi)
+        if sel.any():
+            dis[sel] = np.degrees(sym.disorientation(rec[sel, :4].astype(np.float64), truth[sel], e.phases[pi].master.rotations))
 
     n_mask = mask_count(H, W)
-    fold, inversion = ph.host.fold, ph.host.hm.rules.inversion
-    fpp, fpp_total = flops_per_pattern(args.bandwidth, fold, inversion, args.iters, n_mask)
+    fpp, fpp_total, bpp = {}, 0.0, {}
+    for ph in phs:  # every pattern is correlated and refined against every phase
+        f1, t1 = flops_per_pattern(args.bandwidth, ph.host.fold, ph.host.hm.rules.inversion, args.iters, n_mask)
+        for k, v in f1.items():
+            fpp[k] = fpp.get(k, 0.0) + v
+        fpp_total += t1
+        for k, v in bytes_per_pattern(args.bandwidth, ph.host.fold, H, W).items():
+            bpp[k] = bpp.get(k, 0.0) + v
     roofline = None
     kernels = {}
     if prof:
         for k, (cnt, ms) in prof.items():
             kernels[k] = {"launches": cnt, "ms_total": round(ms, 3)}
-        bpp = bytes_per_pattern(args.bandwidth, fold, H, W)
-
         def roof(k):
             cnt, ms = prof[k]
             per_launch_ms = ms / cnt
@@ -322,7 +350,7 @@ def main():
         kernels["_all_algorithmic_TFLOPs"] = round(total_alg / el / 1e12, 3)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not two:
         _, (_, _, host_sheets) = e.baked(0)
         sample = pats[:64].cpu().numpy()
         cpu = cpu_baseline(sample, items[:64], host_sheets, e.phases[0].master.side, args.bandwidth, args.iters)
@@ -339,14 +367,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else f"{args.precision} GEMM products (f32 accumulate), the rest f32",
-        "data": "synthetic: Ni master projected on the GPU at seeded random orientations, PC jitter +-0.01, Poisson 400 counts",
-        "config": {"workload": f"{n} patterns {H}x{W} f32, Ni (Fm-3m, bundled ebsdsim master), L={args.bandwidth}, "
-                               f"{args.iters} Gauss-Newton iterations, chunks U[1000, 8000]",
+        "data": ("synthetic: Ni master projected on the GPU at seeded random orientations, PC jitter +-0.01, Poisson 400 counts" if not two else
+                 f"synthetic: Ni ({args.p_first:g}) and GaN masters projected on the GPU, phase Bernoulli per pattern, seeded random "
+                 "orientations, PC jitter +-0.01, Poisson 400 counts"),
+        "config": {"workload": f"{n} patterns {H}x{W} f32, {'Ni (Fm-3m' if not two else 'Ni (Fm-3m) + GaN (P6_3mc'}, bundled ebsdsim "
+                               f"master{'' if not two else 's'}), L={args.bandwidth}, {args.iters} Gauss-Newton iterations, chunks U[{clo}, {chi}]",
                    "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks),
-                   "correlator": "mfma" if e.use_mfma(args.bandwidth) else "slab", "streams": max(args.lanes, 1),
-                   "precision": args.precision},
+                   "correlator": "mfma" if e.use_mfma(args.bandwidth) else ("slab" if not e.use_mfma(args.bandwidth, 0) else "mfma+slab"),
+                   "streams": max(args.lanes, 1), "precision": args.precision, "phases": args.phases},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
-                     "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1)},
+                     "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1),
+                     "phase_right": round(float(np.mean(right)), 5)},
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,

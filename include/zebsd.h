@@ -95,7 +95,7 @@ typedef struct zb_phase {
 typedef struct zb_vtable {
     int32_t L;
     int32_t n;      /* fft_size(L), at most 256 on this path */
-    int32_t nc;     /* n / fold */
+    int32_t nc;     /* n / fold: at most 64, or 128 at n = 256 */
     int32_t l_step; /* 2 when the master has no odd l, else 1 */
     int32_t slots;  /* ceil(L / l_step) */
     int32_t cols;
@@ -105,8 +105,9 @@ typedef struct zb_vtable {
     const void* qlp;  /* the 16-bit table (lp_kind != 0), 2-byte values */
 } zb_vtable;
 
-/* The tiling of zb_vtable.q for nc points along c and `slots` degree slots: cols, S (slots per
- * wave, the first of 8, 16, 32, 48, 64 >= slots) and nblk (column blocks of 32 per wave). */
+/* The tiling of zb_vtable.q for nc points along c (up to 128) and `slots` degree slots (up to
+ * 96): cols, S (slots per wave, the first of 8, 16, 32, 48, 64, 96 >= slots) and nblk (column
+ * blocks of 32 per wave: 1 when cols = 32 or S >= 64, else 2). */
 int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* S, int32_t* nblk);
 
 /* The tiling of zb_vtable.qlp: cols, groups (of 8 slots: 2 ceil(slots / 16)) and nblk (1 when

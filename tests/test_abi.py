@@ -49,8 +49,10 @@ def test_bad_arguments_fail_with_a_message_and_launch_nothing():
     ph = nat.Phase(3, 32, 21, 2, None, None)  # fold 3 is not a power of two
     band = nat.Band(64, 128, None, None, None, None)
     assert lib.zb_so3_workspace(ctypes.byref(band), ctypes.byref(ph), 4) == -1
-    vt = nat.VTable(96, 256, 64, 1, 96, 128, 4096)  # 96 degree slots: past the MFMA tile's 64
+    vt = nat.VTable(96, 256, 64, 1, 96, 128, 4096, 1, 0, 4096)  # 96 degree slots: the 16-bit form takes 64
     assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "degree slots" in nat.last_error()
+    vt = nat.VTable(96, 256, 128, 1, 96, 256, 4096)  # GaN at L = 96: FP32 products take 96 slots, 128 columns
+    assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) > 0
     vt = nat.VTable(96, 256, 16, 2, 48, 32, 4096)  # n = 256 takes two lanes per column of 32
     assert lib.zb_vcorr_workspace(ctypes.byref(vt), 4, 0) == -1 and "nc <= 64" in nat.last_error()
     vt = nat.VTable(160, 512, 64, 2, 80, 128, None)  # n = 512: past the register transform
@@ -86,8 +88,8 @@ def test_table_tiling_rule_is_the_librarys():
     import numpy as np
     from zebsdsolve_amd import _native as nat, tables
     lib = nat.load()
-    for nc in (2, 4, 8, 16, 32, 64):
-        for slots in (1, 5, 8, 9, 16, 17, 32, 33, 64):
+    for nc in (2, 4, 8, 16, 32, 64, 128):
+        for slots in (1, 5, 8, 9, 16, 17, 32, 33, 48, 49, 64, 65, 96):
             c, S, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
             assert lib.zb_vtable_shape(nc, slots, ctypes.byref(c), ctypes.byref(S), ctypes.byref(b)) == 0
             assert (c.value, S.value, b.value) == tables.vgemm_shape(nc, slots)

@@ -76,3 +76,53 @@ def test_l96_correlation_against_the_oracle_and_recovery():
             # the score to float32 accuracy, the orientation within 1/100 of a grid step (1.4 deg)
             assert abs(pk[b, 4] - v0) < 2e-4 * abs(v0), (b, pk[b, 4], v0)
             assert np.degrees(sym.disorientation(pk[b, :4].astype(np.float64), q, rots)) < 0.014, b
+
+
+def test_gan_l96_matrix_cores_match_the_slab_path():
+    """GaN at L = 96 (fold 2, every l: 96 degree slots, nc = 128): the matrix-core correlation
+    with its 2.4 GB table and the denominators taken from the slab path in natural order,
+    against the VALU slab path on the same coefficients (scores to float32 rounding of the two
+    summation orders, the same cell and sub-cell peak), and the whole path against the truth."""
+    import torch
+    from tests.conftest import GAN
+    from tests.test_gpu_vcorr import _coefficients
+    from zebsdsolve_amd import _native as nat, engine, sym
+    from zebsdsolve_amd.engine import stream_ptr
+    from zebsdsolve_amd.geometry import Geometry
+    print("GaN L=96 matrix cores", flush=True)
+    n = 8
+    e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(GAN)
+    e.settings.apply(bandwidth=L)
+    e.set_patterns(np.zeros((1, n, 60, 60), np.float32))
+    _, (_, _, host) = e.baked(0)
+    side = e.phases[0].master.side
+    pats, truth = cases.simulate(host, side, n, seed=17)
+    e.set_patterns(pats.reshape(1, n, 60, 60))
+    assert e.use_mfma(L)
+    ph = e.phase_band(0, L)
+    assert ph.big_tables() and ph.host.nc == 128
+    alm, _ = _coefficients(e, L, pats)
+    lib = nat.load()
+    band = e.band(L)
+    vt, _ = ph.vtable_of(0)
+    p1 = torch.empty((n, 8), device="cuda")
+    ws = torch.empty((lib.zb_vcorr_workspace(vt, n, 0),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), alm.data_ptr(), n, e.vrden(0, L).data_ptr(), n, ws.data_ptr(), ws.numel(),
+                                     p1.data_ptr(), stream_ptr()), "vcorr")
+    p2 = torch.empty((n, 8), device="cuda")
+    ws2 = torch.empty((lib.zb_so3_workspace(band.c, ph.c, n),), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_so3_correlate(band.c, ph.c, alm.data_ptr(), n, e.rden(0, L).data_ptr(), n, ws2.data_ptr(), ws2.numel(), p2.data_ptr(),
+                                   stream_ptr()), "so3")
+    a, b = p1.cpu().numpy(), p2.cpu().numpy()
+    rots = e.phases[0].master.rotations
+    # the scores to float32 rounding of the two summation orders, the same peak
+    assert np.all(np.abs(a[:, 4] - b[:, 4]) <= 2e-5 * np.abs(b[:, 4])), np.abs(a[:, 4] - b[:, 4]).max()
+    d = np.degrees(sym.disorientation(a[:, :4].astype(np.float64), b[:, :4].astype(np.float64), rots))
+    assert d.max() < 0.01, d
+    # the whole path against the truth (the reference's thresholds)
+    rec = e.index_cells(np.arange(n), L)[0]
+    dt = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), truth, rots))
+    assert np.median(dt) < 0.15 and dt.max() < 1.0, dt

@@ -957,19 +957,20 @@ static int ilog2v(int v) {
 }
 
 static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
-    if (!t || t->L < 2 || ilog2v(t->n) < 4 || t->n < 2 * t->L - 1 || ilog2v(t->nc) < 1 || t->nc > 64 || t->n / t->nc > 4 ||
+    if (!t || t->L < 2 || ilog2v(t->n) < 4 || t->n < 2 * t->L - 1 || ilog2v(t->nc) < 1 || t->nc > (t->n == 256 ? 128 : 64) || t->n / t->nc > 4 ||
         !(t->l_step == 1 || t->l_step == 2) || t->slots != (t->L + t->l_step - 1) / t->l_step || t->cols != 32 * ((2 * t->nc + 31) / 32) ||
         t->n / 2 > 128 || (t->n == 256 && t->nc < 32) || !t->q) {
-        set_error("%s: bad table (L %d, n %d, nc %d, slots %d, cols %d; this path takes n <= 256, nc <= 64)", who, t ? t->L : 0, t ? t->n : 0,
+        set_error("%s: bad table (L %d, n %d, nc %d, slots %d, cols %d; this path takes n <= 256, nc <= 64, or nc <= 128 at n = 256)", who,
+                  t ? t->L : 0, t ? t->n : 0,
                   t ? t->nc : 0, t ? t->slots : 0, t ? t->cols : 0);
         return ZB_EINVAL;
     }
-    if (t->slots > 64) {
-        set_error("%s: %d degree slots: at most 64", who, t->slots);
+    if (t->slots > (t->n == 256 && !t->lp_kind ? 96 : 64)) {
+        set_error("%s: %d degree slots: at most 64 (96 at n = 256 with FP32 products)", who, t->slots);
         return ZB_EINVAL;
     }
-    if (t->lp_kind && (t->lp_kind < 1 || t->lp_kind > 2 || !t->qlp || t->lp_split < 0 || t->lp_split % 16 || t->nc < 32)) {
-        set_error("%s: bad 16-bit table (kind %d, split slot %d, nc %d; kinds 1 FP16, 2 BF16, the split a multiple of 16, nc >= 32)", who,
+    if (t->lp_kind && (t->lp_kind < 1 || t->lp_kind > 2 || !t->qlp || t->lp_split < 0 || t->lp_split % 16 || t->nc < 32 || t->nc > 64)) {
+        set_error("%s: bad 16-bit table (kind %d, split slot %d, nc %d; kinds 1 FP16, 2 BF16, the split a multiple of 16, 32 <= nc <= 64)", who,
                   t->lp_kind, t->lp_split, t->nc);
         return ZB_EINVAL;
     }
@@ -1090,8 +1091,8 @@ extern "C" int32_t zb_vtable16_shape(int32_t nc, int32_t slots, int32_t* cols, i
 // The choice of (S, NBLK) fixes the table's tiling (zb_vtable_shape).
 static void vgemm_shape(int nc, int slots, int& cols, int& S, int& nblk) {
     cols = 32 * ((2 * nc + 31) / 32);
-    S = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : slots <= 48 ? 48 : 64;
-    nblk = (cols == 32 || S == 64) ? 1 : 2;
+    S = slots <= 8 ? 8 : slots <= 16 ? 16 : slots <= 32 ? 32 : slots <= 48 ? 48 : slots <= 64 ? 64 : 96;
+    nblk = (cols == 32 || S >= 64) ? 1 : 2;
 }
 
 static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* v, hipStream_t s) {
@@ -1100,7 +1101,9 @@ static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* 
     // one tile of 32 items per wave: the coefficients (S registers), two sets of accumulators
     // (2 NBLK 16) and the staged table rows leave room for two or three waves per SIMD
 #define ZB_VG(NB, SS) launch_vgemm_t<1, NB, SS>(a, q, alm, v, s)
-    if (S == 64) {
+    if (S == 96) {
+        ZB_VG(1, 96);
+    } else if (S == 64) {
         ZB_VG(1, 64);
     } else if (nblk == 1) {
         if (S == 8) ZB_VG(1, 8); else if (S == 16) ZB_VG(1, 16); else if (S == 32) ZB_VG(1, 32); else ZB_VG(1, 48);
@@ -1112,7 +1115,7 @@ static int launch_vgemm(const VArg& a, const float* q, const float* alm, float* 
 }
 
 extern "C" int32_t zb_vtable_shape(int32_t nc, int32_t slots, int32_t* cols, int32_t* S, int32_t* nblk) {
-    if (nc < 2 || nc > 64 || (nc & (nc - 1)) || slots < 1 || slots > 64 || !cols || !S || !nblk) {
+    if (nc < 2 || nc > 128 || (nc & (nc - 1)) || slots < 1 || slots > 96 || !cols || !S || !nblk) {
         set_error("zb_vtable_shape: nc %d, slots %d", nc, slots);
         return ZB_EINVAL;
     }

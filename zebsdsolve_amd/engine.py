@@ -70,29 +70,40 @@ class DevicePhase:
         self.rden = {}
         self.band = band
         self.dev = dev
-        self._vt = None
+        self._vt = [None, None]
+        self._vth = [None, None]
         self._vt16 = {}
         self.vrden = {}
 
+    def vtable_of(self, k):
+        """The GEMM table (zb_vtable, device buffer) of the master (k = 0) or of its square
+        (k = 1, for the denominators), built on first use."""
+        if self._vt[k] is None:
+            t = tables_mod.VTableHost(self.band.host, self.host, (self.host.hm.f, self.host.hm.f2)[k])
+            q = _dev(t.tiled, self.dev)
+            t.tiled = None  # on the device now
+            self._vt[k] = (nat.VTable(t.L, t.n, t.nc, t.l_step, t.slots, t.cols, q.data_ptr()), q)
+            self._vth[k] = t
+        return self._vt[k]
+
     def vtables(self):
-        """The GEMM tables of the master and of its square (zb_vtable), built on first use."""
-        if self._vt is None:
-            out = []
-            self._vth = []
-            for coef in (self.host.hm.f, self.host.hm.f2):
-                t = tables_mod.VTableHost(self.band.host, self.host, coef)
-                q = _dev(t.tiled, self.dev)
-                out.append((nat.VTable(t.L, t.n, t.nc, t.l_step, t.slots, t.cols, q.data_ptr()), q))
-                self._vth.append(t)
-            self._vt = out
-        return self._vt
+        """The GEMM tables of the master and of its square."""
+        return [self.vtable_of(0), self.vtable_of(1)]
+
+    def big_tables(self):
+        """Whether the FP32 table passes 1 GB (GaN at L = 96: 2.4 GB), so that the denominators
+        come from the slab path instead of a second table of the master's square."""
+        L, n, nc = self.band.host.L, self.band.host.n, self.host.nc
+        slots = (L + self.host.l_step - 1) // self.host.l_step
+        cols, S, _ = tables_mod.vgemm_shape(nc, slots)
+        return (n // 2 + 1) * L * S * 2 * cols * 4 > (1 << 30)
 
     def vtable16(self, kind, split_degree):
         """The master's GEMM table with 16-bit products (kind 1 FP16, 2 BF16; hi + lo products for
         degrees from split_degree on), built on first use."""
         key = (kind, split_degree)
         if key not in self._vt16:
-            (vt, q), _ = self.vtables()
+            vt, q = self.vtable_of(0)
             t = self._vth[0]
             split = tables_mod.split_slot(split_degree, t.l_step)
             q16 = torch.from_numpy(t.table16(kind, split).view(np.int16)).to(self.dev)
@@ -115,13 +126,13 @@ def lattice(rows, cols, want):
 
 def mfma_correlation(L, n, nc=None, slots=None):
     """Whether the matrix-core correlation takes this shape: its transform along a is held in
-    registers (n <= 128 one lane per column, n = 256 two lanes per column with at least 32
-    columns), the table's columns 2 nc <= 128 and its degree slots <= 64."""
-    if nc is not None and (nc > 64 or (n == 256 and nc < 32)):
+    registers (n <= 128 one lane per column with nc <= 64 and at most 64 degree slots; n = 256
+    two lanes per column with 32 <= nc <= 128 and at most 96 slots)."""
+    if n > 256:
         return False
-    if slots is not None and slots > 64:
-        return False
-    return n <= 256
+    if n == 256:
+        return (nc is None or 32 <= nc <= 128) and (slots is None or slots <= 96)
+    return (nc is None or nc <= 64) and (slots is None or slots <= 64)
 
 
 class Lane:
@@ -400,11 +411,21 @@ class Engine:
         key = self._geo_key()
         if key in ph.vrden:
             return ph.vrden[key]
+        n, nc = band.host.n, ph.host.nc
+        if ph.big_tables():
+            # the slab path's volume (no table of the master's square), put in natural order:
+            # slab position (p, cq, k) holds a = 2 bitrev(p) + k at c = bitrev(cq)
+            nb, h = n // 2 + 1, n // 2
+            slab = self.rden(pi, L).view(nb, h, nc, 2).permute(0, 2, 1, 3)
+            bc = torch.from_numpy(tables_mod.bitrev(np.arange(nc), int(np.log2(nc)))).to(slab.device)
+            bh = torch.from_numpy(tables_mod.bitrev(np.arange(h), int(np.log2(h)))).to(slab.device)
+            rden = slab[:, bc][:, :, bh].reshape(-1).contiguous()
+            ph.vrden[key] = rden
+            return rden
         w_alm, w_stats = self._window(band)
         dev = self._device()
         (tf, _), (tf2, _) = ph.vtables()
-        n = band.host.n
-        rden = torch.empty((n // 2 + 1) * n * ph.host.nc, dtype=torch.float32, device=dev)
+        rden = torch.empty((n // 2 + 1) * n * nc, dtype=torch.float32, device=dev)
         ws = torch.empty((nat.check(lib.zb_vcorr_workspace(tf, 1, 1), "vcorr workspace"),), dtype=torch.uint8, device=dev)
         nat.check(lib.zb_vcorr_denominators(tf, tf2, w_alm.data_ptr(), w_stats.data_ptr(), 1, ws.data_ptr(), ws.numel(), rden.data_ptr(),
                                             stream_ptr()), "vcorr denominators")
@@ -421,9 +442,11 @@ class Engine:
     def corr_vtable(self, ph):
         """The GEMM table the correlation multiplies with, at the precision the settings name."""
         kind = {"fp32": 0, "fp16": 1, "bf16": 2}[self.settings["correlation_precision"]]
-        # the 16-bit form pairs 32 complex columns a group: below nc = 32 the FP32 form
-        if kind == 0 or ph.host.nc < 32:
-            return ph.vtables()[0][0]
+        # the 16-bit form pairs 32 complex columns a group, for up to 64 degree slots and 64
+        # columns: otherwise the FP32 form
+        slots = (ph.host.hm.L + ph.host.l_step - 1) // ph.host.l_step
+        if kind == 0 or not (32 <= ph.host.nc <= 64 and slots <= 64):
+            return ph.vtable_of(0)[0]
         return ph.vtable16(kind, int(self.settings["split_degree"]))
 
     def use_mfma(self, L, pi=None):

@@ -117,11 +117,11 @@ def d_beta(L, fold, wigner, betas):
 def vgemm_shape(nc, slots):
     """(cols, S, nblk) of the matrix-core correlation for nc points along c and `slots` degree
     slots, by the rule its kernel launch follows (zb_vtable_shape in the C ABI): cols = 32
-    ceil(2 nc / 32) table columns, S = the first of 8, 16, 32, 48, 64 >= slots slots per wave,
-    nblk column blocks of 32 per wave (1 when cols = 32 or S = 64, else 2)."""
+    ceil(2 nc / 32) table columns, S = the first of 8, 16, 32, 48, 64, 96 >= slots slots per
+    wave, nblk column blocks of 32 per wave (1 when cols = 32 or S >= 64, else 2)."""
     cols = 32 * ((2 * nc + 31) // 32)
-    S = next(x for x in (8, 16, 32, 48, 64) if x >= slots)
-    nblk = 1 if (cols == 32 or S == 64) else 2
+    S = next(x for x in (8, 16, 32, 48, 64, 96) if x >= slots)
+    nblk = 1 if (cols == 32 or S >= 64) else 2
     return cols, S, nblk
 
 
