@@ -58,6 +58,12 @@ __device__ __forceinline__ float2 buf_f32x2(__amdgpu_buffer_rsrc_t r, unsigned v
     const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
     return make_float2(__uint_as_float(u.x), __uint_as_float(u.y));
 }
+
+// Cache policy of V's stores and loads: non-temporal (nt). V is written once by the GEMM and
+// read once by the transform along a, gigabytes apart, so no cache level would hold it for
+// the other: streaming it past them measured 3.5 % faster for the GEMM and 8.5 % for the
+// transform along a (nickel, L = 64).
+constexpr int kStream = 2;
 __device__ __forceinline__ void buf_st_f32(float x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, 0);
 }
@@ -78,9 +84,10 @@ struct VEl {
     static constexpr unsigned bytes = VT ? 4u : 8u;
     static __device__ __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
         if constexpr (VT == 0) {
-            return buf_f32x2(r, voff, soff);
+            const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, kStream);
+            return make_float2(__uint_as_float(u.x), __uint_as_float(u.y));
         } else {
-            return unpack(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+            return unpack(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, kStream));
         }
     }
     static __device__ __forceinline__ float2 at(const void* v, size_t i) {
@@ -228,7 +235,8 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
                     const floatx16& x = acc[mb][nb][0];
-                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x[r]), vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w + nb * 32) * 4), 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x[r]), vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w + nb * 32) * 4),
+                                                          kStream);
                 }
         stage_store(nxt);
         __syncthreads();
@@ -424,7 +432,7 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) {
                 const unsigned u = VEl<BF ? 2 : 1>::pack(acc[mb][0][0][r], acc[mb][1][0][r]);
-                __builtin_amdgcn_raw_buffer_store_b32(u, vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w) * 4), 0);
+                __builtin_amdgcn_raw_buffer_store_b32(u, vrs, (int)loff[mb], (int)((((r >> 2) * 8 + (r & 3)) * w) * 4), kStream);
             }
         stage_store(nxt);
         __syncthreads();
