@@ -234,7 +234,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by patterns with no "
-              "exchange in the hot loop; see DESIGN.md)", file=sys.stderr)
+              "exchange in the hot loop: one process per GPU indexing its own chunks)", file=sys.stderr)
         sys.exit(2)
 
     global H, W
