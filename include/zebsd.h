@@ -219,6 +219,22 @@ int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, co
 int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
                 int32_t flags, float* sums, void* stream);
 
+/* The six-direction fit to convergence, one Levenberg-Marquardt step at a time, in FP64 on the
+ * device (one thread an item): zb_fit6_propose turns each item's sums (zb_sums at its pose) into
+ * the NCC, gradient and Gauss-Newton matrix (planefit.zig:177-201 localOf), damps the matrix by
+ * lambda on its diagonal plus 1e-9 of its trace (planefit.zig:229-247 damped; the last
+ * `unknowns` of the six directions: 3 the rotation alone, 6 the shift too), solves it by
+ * elimination with partial pivoting (planefit.zig:203-227 solve), limits the step to max_shift
+ * pixels and max_rot radians, and writes the moved poses to trial ([n][8], as poses) and
+ * (NCC, step length) to cur ([n][2] f64). zb_fit6_accept, given zb_sums at the trial poses,
+ * keeps a trial whose NCC rose (pose and sums copied, lambda / 10) or the old pose
+ * (lambda x 10); state [n] i32: 0 fitting, 1 converged (the step was below tol), 2 flat (no
+ * NCC; the pose is left alone). lambda: [n] f64, set by the caller before the first step. */
+int32_t zb_fit6_propose(const float* poses, const float* sums, const double* lambda, int32_t n, int32_t unknowns, double max_rot,
+                        double max_shift, float* trial, double* cur, int32_t* state, void* stream);
+int32_t zb_fit6_accept(const float* trial, const float* trial_sums, int32_t n, double tol, const double* cur, float* poses, float* sums,
+                       double* lambda, int32_t* state, void* stream);
+
 /* The pattern filters before the path. background: [h][w] f32 subtracted from every pattern
  * (the static background, e.g. the mean of patterns over the map), or NULL; dynamic: 0 none,
  * 1 subtract, 2 divide by a Gaussian blur of each pattern of deviation dynamic_sigma pixels

@@ -645,6 +645,43 @@ def local6(s):
     return True, ncc, g, H
 
 
+def fit6(grad_field, side, fr, pat, t0, q0, iters=20, unknowns=6, lambda0=1e-3, max_rot=np.radians(2.0), max_shift=2.0):
+    """The six-direction fit by Levenberg-Marquardt steps (planefit.zig:177-247 localOf, damped,
+    solve): the Newton step of the NCC model over the last `unknowns` directions with the matrix
+    damped by lambda on its diagonal plus 1e-9 of its trace (numpy's solver), limited to
+    max_shift pixels and max_rot radians, kept when the NCC rises (lambda / 10) and otherwise
+    refused (lambda x 10). Returns (t, q, NCC)."""
+    t = np.asarray(t0, np.float64).copy()
+    q = np.asarray(q0, np.float64).copy()
+    lam = lambda0
+    ok, ncc, g, H = local6(sums6(grad_field, side, fr, pat, t, q))
+    if not ok:
+        return t, q, 0.0
+    o = 6 - unknowns
+    for _ in range(iters):
+        A = H[o:, o:].copy()
+        tr = np.trace(A)
+        A[np.diag_indices(unknowns)] = A[np.diag_indices(unknowns)] * (1 + lam) + 1e-9 * tr + 1e-30
+        x = np.zeros(6)
+        x[o:] = np.linalg.solve(A, g[o:])
+        sh, th = np.linalg.norm(x[:3]), np.linalg.norm(x[3:])
+        if sh > max_shift:
+            x[:3] *= max_shift / sh
+        if th > max_rot:
+            x[3:] *= max_rot / th
+        t2 = t + x[:3]
+        q2 = qmul(qexp(x[3:]), q)
+        q2 = q2 / np.linalg.norm(q2) * (1 if q2[0] >= 0 else -1)
+        # (the rotation block of g and H does not depend on the shift directions' sums)
+        ok2, n2, g2, H2 = local6(sums6(grad_field, side, fr, pat, t2, q2))
+        if ok2 and n2 > ncc:
+            t, q, ncc, g, H = t2, q2, n2, g2, H2
+            lam = max(lam * 0.1, 1e-9)
+        else:
+            lam = min(lam * 10, 1e9)
+    return t, q, ncc
+
+
 def refine(grad_field, side, fr, pat, t, q0, iters=8, max_step=np.radians(2.0), scale0=1.0, scale_min=0.02, scale_max=1e4):
     """Gauss-Newton on the orientation with the curvature rescaled from each gain against its
     promise, accept/reject against the best point (detector.wgsl:529-659). Returns

@@ -102,3 +102,69 @@ def test_differentiable_ncc_backward_is_the_oracle_gradient(fitcase):
         assert np.abs(fd[:3] - gd[b, :3]).max() < 0.1 * np.abs(fd[:3]).max(), (b, fd, gd[b])
         assert np.abs(fd[3:] - gd[b, 3:]).max() < 0.1 * np.abs(fd[3:]).max(), (b, fd, gd[b])
     assert moved.shape == (N, 8)
+
+
+def _qangle(a, b):
+    """Degrees between unit quaternions (either sign), from the vector part of a* b (well
+    conditioned near zero, unlike the arccos of a dot product of float32 quaternions)."""
+    a = np.asarray(a, np.float64) / np.linalg.norm(a, axis=-1, keepdims=True)
+    b = np.asarray(b, np.float64) / np.linalg.norm(b, axis=-1, keepdims=True)
+    w = np.sum(a * b, -1)
+    v = a[..., :1] * b[..., 1:] - b[..., :1] * a[..., 1:] - np.cross(a[..., 1:], b[..., 1:])
+    return np.degrees(2 * np.arctan2(np.linalg.norm(v, axis=-1), np.abs(w)))
+
+
+def test_fit6_matches_the_oracles_fit(fitcase):
+    """The device fit (zb_fit6_propose / zb_sums / zb_fit6_accept) and the oracle's, from the
+    same poses on the same noisy patterns: both climb to the same NCC maximum (the NCC to 1e-5,
+    the orientation to 0.01 deg, the detector position to 0.02 px)."""
+    op, fit, field = fitcase["op"], fitcase["fit"], fitcase["field"]
+    print("fit6 vs oracle", flush=True)
+    poses, ncc, state = fit.fit6(field, fitcase["poses"], iters=20)
+    got, ncc = poses.cpu().numpy().astype(np.float64), ncc.cpu().numpy()
+    for b in range(3):
+        p = fitcase["poses"][b].astype(np.float64)
+        t, q, n = op.fit6(fitcase["grad"], fitcase["side"], fitcase["fr"], fitcase["pats_n"][b].astype(np.float64), p[:3], p[4:8], iters=20)
+        d = _qangle(q, got[b, 4:8])
+        assert abs(ncc[b] - n) < 1e-5 and d < 0.01 and np.abs(got[b, :3] - t).max() < 0.02, (b, ncc[b] - n, d, got[b, :3] - t)
+    assert (state.cpu().numpy() != 2).all()
+
+
+def test_fit6_to_convergence_from_config4_starts():
+    """The differentiation form's starts (80 x 80 patterns; the crystal 0.5 - 2 deg off, uniform
+    angle and axis; the pattern centre up to 0.02 off per coordinate): on noise-free patterns the
+    NCC maximum is the truth, and the fit reaches it (NCC > 0.999, orientation within 0.02 deg,
+    the detector position within 0.05 px) for every pattern."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import engine, fit
+    from zebsdsolve_amd.geometry import Geometry
+    print("fit6 config 4 starts", flush=True)
+    n, S = 16, 80
+    e = engine.Engine()
+    e.settings.apply(static_background="none", dynamic_background="none")
+    e.geo = Geometry(pc=cases.PC, sample_tilt_deg=70.0)
+    e.add_phase(NI)
+    e.set_patterns(np.zeros((1, n, S, S), np.float32))
+    _, (_, _, host) = e.baked(0)
+    side = e.phases[0].master.side
+    pats, truth = cases.simulate(host, side, n, h=S, w=S, seed=29)
+    e.set_patterns(pats.reshape(1, n, S, S))
+    _, t = cases.oracle_frame(S, S)
+    field = fit.PoseField.for_cells(e, 0, np.arange(n))
+    rng = np.random.default_rng(6)
+    ang = np.radians(rng.uniform(0.5, 2.0, n))
+    ax = rng.normal(size=(n, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    q0 = np.stack([op.qmul(op.qexp(ax[b] * ang[b]), truth[b]) for b in range(n)])
+    dpc = rng.uniform(-0.02, 0.02, size=(n, 3))
+    R = op.bruker_R(70.0, 0.0)
+    t0 = np.stack([R @ np.array([-dpc[b, 1] * S, -dpc[b, 0] * S, dpc[b, 2] * S]) for b in range(n)]) + t[None, :]
+    poses0 = np.concatenate([t0, np.arange(n)[:, None], q0], 1).astype(np.float32)
+    poses, ncc, state = fit.fit6(field, poses0, iters=40)
+    torch.cuda.synchronize()
+    got, ncc = poses.cpu().numpy().astype(np.float64), ncc.cpu().numpy()
+    d = _qangle(got[:, 4:8], truth)
+    dt = np.abs(got[:, :3] - t[None, :]).max(1)
+    print(f"fit6: NCC min {ncc.min():.5f}, orientation max {d.max():.4f} deg, position max {dt.max():.3f} px", flush=True)
+    assert ncc.min() > 0.999 and d.max() < 0.02 and dt.max() < 0.05, (ncc, d, dt)

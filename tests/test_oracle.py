@@ -426,3 +426,33 @@ def test_bandpass_of_a_constant_is_zero_and_a_difference_of_blurs():
     assert np.allclose(bp, op.blur_direct(x, 1.0) - op.blur_box(x, 14.0))
     bp0 = op.prep(x, None, bandpass=(0.0, 8.0), norm=False)
     assert np.allclose(bp0, x - op.blur_direct(x, 8.0))
+
+
+def test_six_direction_fit_climbs_to_the_truth():
+    """The oracle's Levenberg-Marquardt fit on a noise-free nickel pattern, started half a pixel
+    and half a degree off: the NCC maximum is the truth (the pattern is the master there, NCC 1),
+    so the fit ends on it; with lambda 0 its first step is the Newton step H^-1 g."""
+    from oracle import pipeline as op
+    from tests import cases
+    from zebsdsolve_amd import master as mm
+    from tests.conftest import NI
+    m = mm.load(NI)
+    side = m.side
+    sheets = np.asarray(m.raw, np.float64).reshape(2, side, side)
+    grad = op.gradient(sheets, side)
+    pats, truth = cases.simulate(sheets, side, 1, seed=4)
+    fr, t = cases.oracle_frame(60, 60)
+    pat = op.normalize(pats[0].astype(np.float64), fr["mask"])
+    rng = np.random.default_rng(8)
+    t0 = t + rng.uniform(-0.5, 0.5, 3)
+    q0 = op.qmul(op.qexp(rng.normal(size=3) * np.radians(0.3)), truth[0])
+    tf, qf, ncc = op.fit6(grad.reshape(-1, 4), side, fr, pat, t0, q0, iters=12)
+    d = np.degrees(2 * np.arccos(min(1.0, abs(float(np.dot(qf, truth[0]))))))
+    assert ncc > 0.999 and d < 0.02 and np.abs(tf - t).max() < 0.1, (ncc, d, tf - t)
+    ok, n0, g, H = op.local6(op.sums6(grad.reshape(-1, 4), side, fr, pat, t0, q0))
+    assert ok and ncc > n0
+    # lambda 0, one step: the Newton step (the 1e-9 trace term aside) moves the shift by H^-1 g
+    x = np.linalg.solve(H + 1e-9 * np.trace(H) * np.eye(6), g)
+    assert np.linalg.norm(x[:3]) < 2.0 and np.linalg.norm(x[3:]) < np.radians(2.0)
+    t1, q1, n1 = op.fit6(grad.reshape(-1, 4), side, fr, pat, t0, q0, iters=1, lambda0=0.0)
+    assert n1 > n0 and np.allclose(t1, t0 + x[:3], atol=1e-9)

@@ -66,6 +66,8 @@ _SIGS = {
     "zb_clahe_workspace": ([i32, i32, i32], i64),
     "zb_clahe": ([vp, i32, i32, i32, i32, i32, ctypes.c_float, vp, i64, vp], i32),
     "zb_master_symmetrize": ([vp, vp, i32, vp, i32, vp], i32),
+    "zb_fit6_propose": ([vp, vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp], i32),
+    "zb_fit6_accept": ([vp, vp, i32, ctypes.c_double, vp, vp, vp, vp, vp, vp], i32),
     "zb_master_gradient": ([vp, i32, vp, vp], i32),
     "zb_master_pack": ([vp, i32, vp, vp], i32),
     "zb_profile": ([i32], i32),

@@ -126,3 +126,34 @@ class PoseNcc(torch.autograd.Function):
     def backward(ctx, grad_out):
         (gd,) = ctx.saved_tensors
         return grad_out[:, None] * gd, None, None
+
+
+def fit6(field, poses0, iters=20, unknowns=6, lambda0=1e-3, max_rot=np.radians(2.0), max_shift=2.0, tol=1e-7, with_ncc=True):
+    """Each pose fitted to its pattern by Levenberg-Marquardt steps over the six directions (or
+    the rotation alone, unknowns = 3), on the device: per step the sums at the poses give each
+    item's damped Newton step (zb_fit6_propose, FP64), the sums at the moved poses decide which
+    are kept (zb_fit6_accept). Returns (poses [n, 8] float32, NCC [n] float64 (None without
+    with_ncc), state [n]: 0 still moving after `iters` steps, 1 converged, 2 flat), all on the
+    device."""
+    from .engine import stream_ptr
+    lib = nat.load()
+    dev = field.pats.device
+    poses = torch.as_tensor(poses0, dtype=torch.float32, device=dev).clone().contiguous()
+    n = poses.shape[0]
+    flags = ROTATION_ONLY if unknowns == 3 else 0
+    sums = field.sums(poses, flags)
+    lam = torch.full((n,), float(lambda0), dtype=torch.float64, device=dev)
+    cur = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    state = torch.zeros((n,), dtype=torch.int32, device=dev)
+    trial = torch.empty_like(poses)
+    st = stream_ptr()
+    for _ in range(iters):
+        nat.check(lib.zb_fit6_propose(poses.data_ptr(), sums.data_ptr(), lam.data_ptr(), n, unknowns, float(max_rot), float(max_shift),
+                                      trial.data_ptr(), cur.data_ptr(), state.data_ptr(), st), "fit6 propose")
+        tsums = field.sums(trial, flags)
+        nat.check(lib.zb_fit6_accept(trial.data_ptr(), tsums.data_ptr(), n, float(tol), cur.data_ptr(), poses.data_ptr(), sums.data_ptr(),
+                                     lam.data_ptr(), state.data_ptr(), st), "fit6 accept")
+    if not with_ncc:
+        return poses, None, state
+    ok, ncc, _, _ = local_of(sums.cpu().numpy())
+    return poses, torch.from_numpy(np.where(ok, ncc, 0.0)).to(dev), state
