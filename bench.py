@@ -221,6 +221,8 @@ def main():
                     help="the differentiation form instead: every pattern's pose (crystal 0.5-2 deg off, pattern centre up to 0.02 "
                          "off) fitted over the six directions by --fit-iters Levenberg-Marquardt steps (use with --size 80)")
     ap.add_argument("--fit-iters", type=int, default=20)
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay each chunk size's batch as a HIP graph captured once (Engine.graph; pays off with equal chunks)")
     ap.add_argument("--lanes", type=int, default=1, help="streams the chunks are spread over (1: one stream)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
@@ -281,10 +283,18 @@ def main():
     torch.cuda.synchronize()
 
     lanes = e.lanes(args.lanes) if args.lanes > 1 else [None]
+    graphs = {}
+    if args.graphs:
+        for c0, c1 in chunks:
+            if c1 - c0 not in graphs:
+                graphs[c1 - c0] = e.graph(c1 - c0, H, W, pats.dtype, args.bandwidth)
 
     def step():
         recs = []
         for i, ((c0, c1), it) in enumerate(zip(chunks, local)):
+            if args.graphs:
+                recs.append(graphs[c1 - c0](pats[c0:c1], it).clone())
+                continue
             r, _ = e.run_batch(pats[c0:c1], it, args.bandwidth, lane=lanes[i % len(lanes)])
             recs.append(r)
         return recs
@@ -380,7 +390,7 @@ def main():
                                f"master{'' if not two else 's'}), L={args.bandwidth}, {args.iters} Gauss-Newton iterations, chunks U[{clo}, {chi}]",
                    "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks),
                    "correlator": "mfma" if e.use_mfma(args.bandwidth) else ("slab" if not e.use_mfma(args.bandwidth, 0) else "mfma+slab"),
-                   "streams": max(args.lanes, 1), "precision": args.precision, "phases": args.phases},
+                   "streams": max(args.lanes, 1), "precision": args.precision, "phases": args.phases, "graphs": bool(args.graphs)},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
                      "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1),
                      "phase_right": round(float(np.mean(right)), 5)},

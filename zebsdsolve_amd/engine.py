@@ -600,6 +600,12 @@ class Engine:
                                         it.data_ptr(), peaks.data_ptr(), 8, nbt, opts, records[pi].data_ptr(), b0, st), "refine")
         return records, opts.stamp
 
+    def graph(self, n, h, w, dtype=torch.float32, L=None):
+        """run_batch for batches of n patterns h x w captured once as a HIP graph (GraphBatch):
+        a replay launches the whole batch (prep, coefficients, correlation, refinement, NMI,
+        records) with one call."""
+        return GraphBatch(self, n, h, w, dtype, L)
+
     def _raw_for(self, cells):
         dev = self._device()
         rows, cols, h, w = self.shape
@@ -667,3 +673,32 @@ class Engine:
         nat.check(lib.zb_project(sheets.data_ptr(), self.phases[pi].master.side, self.frame(), pd.data_ptr(), len(cells), out.data_ptr(),
                                  stream_ptr()), "project")
         return out.cpu().numpy()
+
+
+class GraphBatch:
+    """One batch shape of Engine.run_batch as a HIP graph: the inputs are copied into the graph's
+    own buffers, the captured launches replay on the current stream, the records land in the
+    graph's own buffer (valid until the next replay). Built by one eager run (which makes every
+    table and volume the batch needs) and one capture; the library's entry points allocate,
+    clear (k_vclear is a kernel, no memset node) and synchronize nothing, so all of the batch is
+    in the graph. The records carry the stamp of the capture: a replay's records are checked
+    against it (Engine.index_cells' guard does not apply here)."""
+
+    def __init__(self, engine, n, h, w, dtype=torch.float32, L=None):
+        dev = engine._device()
+        self.raw = torch.zeros((n, h, w), dtype=dtype, device=dev)
+        self.items = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+        self.items[:, 3] = torch.arange(n, dtype=torch.float32, device=dev)
+        engine.run_batch(self.raw, self.items, L)  # tables, volumes, workspaces
+        torch.cuda.synchronize(dev)
+        self.g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.g):
+            self.records, self.stamp = engine.run_batch(self.raw, self.items, L)
+        torch.cuda.synchronize(dev)
+
+    def __call__(self, raw, items):
+        """records [phases, n, 8] for raw [n, h, w] and items [n, 4] (device tensors)."""
+        self.raw.copy_(raw)
+        self.items.copy_(items)
+        self.g.replay()
+        return self.records
