@@ -447,7 +447,8 @@ def bench_fit(args, e, pats, items, truth):
         ppl = args.steps * n * (args.fit_iters + 1) / cnt
         tf = 120.0 * n_mask * ppl / (ms / cnt * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
-                    "traffic": None, "kernel": "k_sums6", "launch_ms": round(ms / cnt, 4), "patterns_per_launch": round(ppl, 1)}
+                    "traffic": traffic_from_profiles("k_sums6", ppl), "kernel": "k_sums6", "launch_ms": round(ms / cnt, 4),
+                    "patterns_per_launch": round(ppl, 1)}
     out = {
         "metric": "patterns/s fitted to convergence (six-direction Levenberg-Marquardt); % of roofline",
         "value": round(args.steps * n / el, 1), "unit": "patterns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
