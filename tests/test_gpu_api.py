@@ -155,3 +155,21 @@ def test_graph_replays_match_eager_batches():
         g, w = got.cpu().numpy(), want.cpu().numpy()
         assert np.array_equal(g[..., :7], w[..., :7]), seed
         assert np.all(g[..., 7] == np.float32(gb.stamp)) and np.all(w[..., 7] == np.float32(stamp))
+
+
+def test_tune_climbs_the_knobs_and_keeps_the_results():
+    """Solver.tune (python/zebsdsolve/__init__.py tune): it times a sample, leaves knobs within
+    their ranges, reports before/after and a summary, stops when progress says so, and the map
+    indexes to the same orientations afterwards (the knobs cut the work, nothing else)."""
+    s = _solver()
+    s.add_phase(NI)
+    s.synthesize(toy_rows=16, toy_cols=24, toy_grains=96, toy_counts=400, toy_seed=5)
+    r0 = s.index()
+    calls = []
+    out = s.tune(lambda k: calls.append(k) or len(calls) < 5)
+    assert set(out["before"]) == set(out["after"]) == {"step_cells", "spec_budget_mb"}
+    assert len(calls) <= 5 and "tuned on" in out["summary"]
+    assert 256 <= out["after"]["step_cells"] <= 1 << 20 and 64 <= out["after"]["spec_budget_mb"] <= 1 << 16
+    r1 = s.index()
+    q0, q1 = r0["quaternions"].reshape(-1, 4), r1["quaternions"].reshape(-1, 4)
+    assert np.allclose(np.abs(np.sum(q0 * q1, 1)), 1.0, atol=1e-6)

@@ -66,6 +66,7 @@ class Solver:
         self._chance_key = None
         self._res = None
         self._idx = None
+        self._step_cells = 16384  # cells index_step takes at least (tune climbs it)
 
     def close(self):
         self._e = None
@@ -372,7 +373,7 @@ class Solver:
             return 0
         s = self.settings
         n = min(left, max(s["batch"], 1) * max(2, min(16, 4096 // max(s["batch"], 1))) if s["batch"] < 4096 else left)
-        n = max(n, min(left, 16384))
+        n = max(n, min(left, self._step_cells))
         cells = self._todo[self._done:self._done + n]
         out = self.index_cells(cells, bandwidth=self._sweep_bandwidth(self._sweep))
         self._land(cells, out)
@@ -468,6 +469,71 @@ class Solver:
         return _sym.disorientation(a, b, self._e.phases[phase].master.rotations)
 
     # ── timing ──
+
+    def tune(self, progress=None):
+        """Fit how the work is cut up to this machine and this scan (python/zebsdsolve/__init__.py
+        tune): a sample of the region is indexed and timed, and each of this build's knobs in
+        turn climbs from the value it has, doubled while that is faster, else halved while that
+        is: the cells indexed a step (`index_step`, 16 384) and the correlation's workspace a
+        launch may take (the engine's V budget, 4 GiB). The sample is sized by the bandwidth
+        (8 192 patterns up to 32, 4 096 up to 64, 1 024 up to 128, 128 beyond) and timed the
+        way index_step runs it. Neither the orientations found nor which matches are accepted
+        depend on these knobs. The reference's own knobs (batch, refine_pixels, wg_refine,
+        wg_fft) cut up its WebGPU dispatches and mean nothing here. Returns the knobs before
+        and after and a summary; `progress(trials_done)` is called after each timed trial,
+        and returning False from it stops (the best so far stays)."""
+        import time
+
+        import torch
+
+        if not self._e.phases:
+            raise Error("tune: no phases")
+        e = self._e
+        L = int(self.settings["bandwidth"])
+        rows, cols = e.shape[0], e.shape[1]
+        want = 8192 if L <= 32 else 4096 if L <= 64 else 1024 if L <= 128 else 128
+        cells = np.arange(min(want, rows * cols), dtype=np.int64)
+        knobs = {"step_cells": (lambda: self._step_cells, lambda v: setattr(self, "_step_cells", v), 256, 1 << 20),
+                 "spec_budget_mb": (lambda: e.spec_budget >> 20, lambda v: setattr(e, "spec_budget", v << 20), 64, 1 << 16)}
+        before = {k: g() for k, (g, _, _, _) in knobs.items()}
+        done = 0
+
+        def trial():
+            nonlocal done
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for c0 in range(0, len(cells), self._step_cells):
+                e.index_cells(cells[c0:c0 + self._step_cells], L)
+            torch.cuda.synchronize()
+            done += 1
+            go = progress(done) if progress is not None else True
+            return time.perf_counter() - t0, go is not False
+
+        trial()  # tables, volumes and workspaces, outside the timings
+        best, go = trial()
+        for name, (get, put, lo, hi) in knobs.items():
+            if not go:
+                break
+            for factor in (2, 0.5):
+                moved = False
+                while go:
+                    cur = get()
+                    nxt = int(cur * factor)
+                    if not (lo <= nxt <= hi) or nxt == cur:
+                        break
+                    put(nxt)
+                    t, go = trial()
+                    if t < best:
+                        best, moved = t, True
+                    else:
+                        put(cur)
+                        break
+                if moved:
+                    break
+        after = {k: g() for k, (g, _, _, _) in knobs.items()}
+        summary = f"tuned on {len(cells)} patterns at L={L}: {len(cells) / best:.0f} patterns/s in {done} trials; " + \
+                  ", ".join(f"{k} {before[k]} -> {after[k]}" for k in knobs)
+        return {"before": before, "after": after, "summary": summary}
 
     def profile(self, on=True):
         _native.profile(on)

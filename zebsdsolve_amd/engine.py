@@ -176,6 +176,9 @@ class Engine:
         # "mfma": the correlation as a GEMM on the matrix cores (n <= 256, at most 64 degree slots);
         # "slab": the spectrum + slab transforms (every bandwidth, and the fallback where the GEMM does not fit)
         self.correlator = "mfma"
+        # bytes of the correlation's workspace (V) a launch may take: it cuts a batch into
+        # correlation chunks (Solver.tune climbs it)
+        self.spec_budget = SPEC_BUDGET
         self._lanes = []
         self._bg = None           # (key, static background on the device)
         self.file_background = None  # [h, w] the scan file's static background (static_background "file")
@@ -555,7 +558,7 @@ class Engine:
         # path pads items); the coefficients and the refinement take the whole batch at once, so
         # that an uneven batch is one launch of each, not one per correlation chunk
         per_item = max(f(4096) for f in corr_ws) // 4096 + 1
-        chunk = max(1, min(n, SPEC_BUDGET // max(per_item, 1)))
+        chunk = max(1, min(n, self.spec_budget // max(per_item, 1)))
         big = max(chunk, min(n, BATCH_ITEMS))
         pairs = band.host.pairs
         need = max(int(lib.zb_sh_workspace(band.c, big)), max(f(chunk) for f in corr_ws))
