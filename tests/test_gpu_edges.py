@@ -1,0 +1,97 @@
+"""Edge cases of the indexing path on the GPU.
+- Batches of every size across the kernels' tile boundaries (32 items a correlation tile, 128 a
+  GEMM block, 256 a launch block of the per-item kernels) give each cell the record it gets in
+  one large batch, in both product precisions of the correlation (the reference's batch-size
+  independence, tests/test_api.py:511-513).
+- Degenerate patterns (constant, all zero, one bright pixel) among good ones give finite
+  records, are not accepted (their NMI is at or near 0), and leave their neighbours' records
+  unchanged.
+- Zero-count calls through the C ABI return success and write nothing."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.conftest import NI, gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+def _solver(**cfg):
+    import zebsdsolve_amd as ze
+    s = ze.Solver()
+    s.set_geometry(pc=(0.44, 0.24, 0.53), sample_tilt_deg=70, detector_tilt_deg=0, px_um=70 * 8, step_x_um=20, step_y_um=20)
+    s.configure(static_background="none", dynamic_background="none", **cfg)
+    s.add_phase(NI)
+    return s
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_ragged_batches_match_one_batch(precision):
+    s = _solver(correlation_precision=precision)
+    s.synthesize(toy_rows=16, toy_cols=24, toy_grains=200, toy_counts=400, toy_seed=11)
+    n = 16 * 24
+    one = s.index_cells(np.arange(n))
+    start = 0
+    for size in (1, 31, 33, 127, 129, 63):
+        cells = np.arange(start, start + size)
+        part = s.index_cells(cells)
+        print(f"{precision}: batch of {size} from cell {start}", flush=True)
+        for k in ("quaternions", "ncc", "nmi"):
+            assert np.allclose(part[k][0], one[k][0, cells], atol=1e-6, rtol=0), (size, k, np.abs(part[k][0] - one[k][0, cells]).max())
+        start += size
+    # a batch of scattered cells in reverse order
+    cells = np.arange(n - 1, -1, -3)
+    part = s.index_cells(cells)
+    assert np.allclose(part["quaternions"][0], one["quaternions"][0, cells], atol=1e-6, rtol=0)
+    assert s.index_cells(np.arange(0))["quaternions"].shape == (1, 0, 4)
+
+
+def test_degenerate_patterns_are_finite_and_not_accepted():
+    s = _solver()
+    s.synthesize(toy_rows=8, toy_cols=12, toy_grains=96, toy_counts=400, toy_seed=5)
+    pats = s._e.patterns.reshape(96, 60, 60).copy()
+    base = s.index_cells(np.arange(96))
+    odd = {3: np.full((60, 60), 400.0, np.float32),  # constant
+           17: np.zeros((60, 60), np.float32),      # all zero
+           40: np.zeros((60, 60), np.float32)}      # one bright pixel
+    odd[40][30, 30] = 1000.0
+    for c, p in odd.items():
+        pats[c] = p
+    s.set_patterns(pats.reshape(8, 12, 60, 60))
+    r = s.index()
+    got = s.index_cells(np.arange(96))
+    bad = np.array(sorted(odd))
+    good = np.setdiff1d(np.arange(96), bad)
+    for k in ("quaternions", "ncc", "nmi", "step"):
+        assert np.isfinite(got[k]).all(), k
+    print("degenerate ncc", got["ncc"][0, bad], "nmi", got["nmi"][0, bad], "z", got["z"][0, bad], flush=True)
+    # the zero pattern has nothing to correlate, and one value in its histogram
+    assert abs(got["ncc"][0, 17]) < 1e-6 and abs(got["nmi"][0, 17]) < 1e-6
+    assert (got["nmi"][0, bad] < 0.05).all()
+    assert (r["sweep"].reshape(-1)[bad] < 0).all()
+    assert (r["sweep"].reshape(-1)[good] >= 0).all()
+    for k in ("quaternions", "ncc", "nmi"):
+        assert np.allclose(got[k][0, good], base[k][0, good], atol=1e-6, rtol=0), k
+
+
+def test_zero_count_calls_return_success():
+    import torch
+
+    from zebsdsolve_amd import _native as nat
+    from zebsdsolve_amd.engine import stream_ptr
+    lib = nat.load()
+    st = stream_ptr()
+    sentinel = torch.full((64,), 7.0, device="cuda")
+    p = sentinel.data_ptr()
+    mask = (ctypes.c_float * 3)(0.0, 0.0, 0.0)
+    opts = nat.PrepOpts()
+    assert lib.zb_normalize(p, 2, 0, 8, 8, mask, p, st) == 0
+    assert lib.zb_prep(p, 2, 0, 8, 8, mask, ctypes.byref(opts), p, st) == 0
+    assert lib.zb_fit6_propose(p, p, p, 0, 6, 0.03, 2.0, p, p, p, st) == 0
+    assert lib.zb_fit6_accept(p, p, 0, 1e-7, p, p, p, p, p, st) == 0
+    torch.cuda.synchronize()
+    assert (sentinel == 7.0).all()
+    # and a negative count is an error, with a message
+    assert lib.zb_fit6_propose(p, p, p, -1, 6, 0.03, 2.0, p, p, p, st) != 0
+    assert "bad arguments" in nat.last_error()
