@@ -95,3 +95,19 @@ def test_zero_count_calls_return_success():
     # and a negative count is an error, with a message
     assert lib.zb_fit6_propose(p, p, p, -1, 6, 0.03, 2.0, p, p, p, st) != 0
     assert "bad arguments" in nat.last_error()
+
+
+def test_index_cells_over_two_streams_matches_one():
+    """A request large enough to be cut over two streams (Engine.index_lanes) gives every cell
+    the record one stream gives it."""
+    s = _solver()
+    s.synthesize(toy_rows=48, toy_cols=96, toy_grains=400, toy_counts=400, toy_seed=13)
+    n = 48 * 96
+    s._e.index_lanes = 1
+    one = s.index_cells(np.arange(n))
+    s._e.index_lanes = 2
+    two = s.index_cells(np.arange(n))
+    s._e.index_lanes = 1
+    assert len(s._e._lanes) >= 2
+    for k in ("quaternions", "ncc", "nmi", "step"):
+        assert np.array_equal(one[k], two[k]), (k, np.abs(one[k] - two[k]).max())

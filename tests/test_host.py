@@ -209,6 +209,29 @@ def test_symmetry_groups_fundamental_zone_and_disorientation():
     assert np.all(sym.disorientation(q, r, ops) <= np.radians(62.81) + 1e-6)
 
 
+def test_to_fz_is_the_largest_real_part_among_the_equivalents():
+    """For every group: to_fz(q) is one of q's equivalents g . q (sign fixed, unit), its real
+    part is at least every equivalent's (the smallest rotation angle), and the first operator
+    reaching the maximum wins a tie; any leading shape, and none."""
+    rng = np.random.default_rng(11)
+    for name, ops in sym.GROUPS.items():
+        q = rng.normal(size=(3, 17, 4)) * rng.uniform(0.5, 2.0, size=(3, 17, 1))
+        f = sym.to_fz(q, ops)
+        assert f.shape == q.shape
+        cands = np.stack([rot.qstd(q)] + [rot.qstd(rot.qmul(np.broadcast_to(g, q.shape), q)) for g in ops])
+        assert (f[..., 0] >= cands[..., 0].max(0) - 1e-12).all(), name
+        assert (np.abs(cands - f[None]).max(-1).min(0) < 1e-12).all(), name
+        assert np.allclose(np.linalg.norm(f, axis=-1), 1.0)
+    ops = sym.GROUPS["m-3m"]
+    # the identity and its equivalents tie at real part 1: the identity stays
+    assert np.allclose(sym.to_fz(np.array([1.0, 0, 0, 0]), ops), [1, 0, 0, 0])
+    # a quarter turn about z ties with the identity's equivalents too: the first operator that
+    # reaches real part 1 (the identity) maps it
+    qz = np.array([np.cos(np.pi / 4), 0, 0, np.sin(np.pi / 4)])
+    assert np.isclose(sym.to_fz(qz, ops)[0], 1.0)
+    assert sym.to_fz(np.zeros((0, 4)), ops).shape == (0, 4)
+
+
 def test_rotations_of_measured_operators():
     ops = np.array([rot.to_mat(q) for q in sym.GROUPS["6/m"]] + [-rot.to_mat(q) for q in sym.GROUPS["6/m"]])
     r = sym.rotations_of(ops)

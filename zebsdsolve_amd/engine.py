@@ -21,6 +21,8 @@ SPEC_BUDGET = 4 << 30
 # items whose coefficients and refinement go in one launch (the correlation takes them in
 # SPEC_BUDGET chunks)
 BATCH_ITEMS = 16384
+# the fewest items a stream of index_cells takes
+LANE_MIN_ITEMS = 2048
 
 
 def device():
@@ -180,6 +182,9 @@ class Engine:
         # correlation chunks (Solver.tune climbs it)
         self.spec_budget = SPEC_BUDGET
         self._lanes = []
+        # streams a large index_cells request is spread over (1: one; more measured no faster
+        # through Solver.index, whose steps each wait for their records)
+        self.index_lanes = 1
         self._bg = None           # (key, static background on the device)
         self.file_background = None  # [h, w] the scan file's static background (static_background "file")
 
@@ -614,9 +619,13 @@ class Engine:
         rows, cols, h, w = self.shape
         flat = self.patterns.reshape(rows * cols, h, w)
         cells = np.asarray(cells, np.int64)
+        # a run of consecutive cells (the usual request) is a slice: no gather copy
+        run = len(cells) > 0 and cells[-1] - cells[0] == len(cells) - 1 and bool(np.all(np.diff(cells) == 1))
         if isinstance(flat, torch.Tensor):
-            return flat[torch.as_tensor(cells, device=flat.device)].to(dev).contiguous()
-        return torch.from_numpy(np.ascontiguousarray(flat[cells])).to(dev)
+            part = flat[int(cells[0]):int(cells[-1]) + 1] if run else flat[torch.as_tensor(cells, device=flat.device)]
+            return part.to(dev).contiguous()
+        part = flat[cells[0]:cells[-1] + 1] if run else flat[cells]
+        return torch.from_numpy(np.ascontiguousarray(part)).to(dev)
 
     def index_cells(self, cells, L=None):
         """Records [n_phases, n, 8] (q, ncc, nmi, step, stamp) for the given map cells; the
@@ -628,13 +637,29 @@ class Engine:
             return np.zeros((len(self.phases), 0, RECORD), np.float32)
         raw = self._raw_for(cells)
         t = self.positions(cells)
-        items = np.concatenate([t, np.arange(len(cells), dtype=np.float32)[:, None]], 1)
-        items_d = torch.from_numpy(items).to(self._device())
-        rec, stamp = self.run_batch(raw, items_d, L)
-        out = rec.cpu().numpy()
-        if not np.all(out[..., 7] == np.float32(stamp)):
-            raise RuntimeError("DeviceSkippedBatch: records without this batch's stamp")
-        return out
+        dev = self._device()
+        n = len(cells)
+        # a large request is cut into consecutive parts on streams of their own, so that one
+        # part's correlation runs beside another's refinement (each cell's record does not depend
+        # on the batch it is in)
+        k = max(1, min(self.index_lanes, n // LANE_MIN_ITEMS))
+        lanes = self.lanes(k) if k > 1 else [None]
+        bounds = np.linspace(0, n, k + 1).astype(np.int64)
+        parts = []
+        for i in range(k):
+            a, b = int(bounds[i]), int(bounds[i + 1])
+            items = np.concatenate([t[a:b], np.arange(b - a, dtype=np.float32)[:, None]], 1)
+            items_d = torch.from_numpy(items).to(dev)
+            parts.append(self.run_batch(raw[a:b], items_d, L, lane=lanes[i]))
+        if k > 1:
+            torch.cuda.synchronize(dev)
+        outs = []
+        for rec, stamp in parts:
+            o = rec.cpu().numpy()
+            if not np.all(o[..., 7] == np.float32(stamp)):
+                raise RuntimeError("DeviceSkippedBatch: records without this batch's stamp")
+            outs.append(o)
+        return outs[0] if k == 1 else np.concatenate(outs, axis=1)
 
     def nmi_at(self, pi, cells, quats):
         """NMI of each cell's pattern with phase pi's master at the given orientation (no
