@@ -173,3 +173,31 @@ def test_tune_climbs_the_knobs_and_keeps_the_results():
     r1 = s.index()
     q0, q1 = r0["quaternions"].reshape(-1, 4), r1["quaternions"].reshape(-1, 4)
     assert np.allclose(np.abs(np.sum(q0 * q1, 1)), 1.0, atol=1e-6)
+
+
+def test_steps_launched_ahead_match_one_step_and_follow_configure():
+    """index_step launches the sweep's next step before landing the current one: many small
+    steps give what one step over the whole map gives, and a configure between steps drops the
+    step launched ahead (the next step follows the new settings)."""
+    s = _solver()
+    s.add_phase(NI)
+    s.synthesize(toy_rows=16, toy_cols=24, toy_grains=200, toy_counts=400, toy_seed=9)
+    s._step_cells = 1 << 20
+    one = s.index()
+    s._step_cells = 64
+    s.configure(batch=1)  # steps of 64 cells: six of them
+    many = s.index()
+    for k in ("quaternions", "ncc", "nmi", "phase", "sweep"):
+        assert np.array_equal(one[k], many[k]), k
+    s.index_begin()
+    s.index_step()
+    assert s._pending is not None
+    s.configure(refine=False)
+    assert s._pending is None
+    s.index_step()
+    got = s._res["ncc"][64:128]
+    want = s.index_cells(np.arange(64, 128))["ncc"][0]
+    assert np.array_equal(got, want)
+    s.configure(refine=True)
+    with_refine = s.index_cells(np.arange(64, 128))["ncc"][0]
+    assert not np.array_equal(got, with_refine)

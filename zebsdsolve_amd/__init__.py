@@ -65,6 +65,9 @@ class Solver:
         self._chance = None
         self._chance_key = None
         self._res = None
+        # a step's batch launched ahead of the step (index_step), and the state it belongs to
+        self._pending = None
+        self._gen = 0
         self._idx = None
         self._step_cells = 16384  # cells index_step takes at least (tune climbs it)
 
@@ -81,6 +84,7 @@ class Solver:
 
     def configure(self, **settings):
         """Set options by name (zebsdsolve_amd/settings.py lists them with the reference's names)."""
+        self._changed()
         try:
             self._e.settings.apply(**settings)
         except SettingError as ex:
@@ -98,6 +102,7 @@ class Solver:
     def set_geometry(self, **fields):
         """Set geometry fields: pc (3), sample_tilt_deg, detector_tilt_deg, rot (3), px_um,
         step_x_um, step_y_um."""
+        self._changed()
         g = self._e.geo
         for k, v in fields.items():
             if k in ("pc", "rot"):
@@ -116,6 +121,7 @@ class Solver:
     def set_patterns(self, patterns):
         """Patterns (rows, cols, h, w) or (n, h, w): uint8, uint16 or float32, a numpy array or a
         device tensor (kept on the device)."""
+        self._changed()
         self._opened = None
         self._e.file_background = None
         try:
@@ -130,6 +136,7 @@ class Solver:
         shape, steps and tilt, the pattern centre (the mean of the stored per-pattern ones), the
         detector pixel (its size times the binning) and the stored static background (used when
         `static_background` is "file"). Other formats are refused."""
+        self._changed()
         from . import h5
         try:
             with h5.File(str(path)) as f:
@@ -211,12 +218,14 @@ class Solver:
 
     def add_phase(self, path):
         """Load a master pattern (ebsdsim .npz); returns its index."""
+        self._changed()
         try:
             return self._e.add_phase(str(path))
         except (OSError, ValueError) as ex:
             raise Error(f"add_phase: {ex}") from None
 
     def clear_phases(self):
+        self._changed()
         self._e.phases = []
 
     @property
@@ -250,6 +259,7 @@ class Solver:
         background, and Poisson-counted at toy_counts (src/engine.zig:2771-2898). Returns the
         answer: quaternions (rows, cols, 4), phase (rows, cols; -1 blank), grain, geometry.
         The random streams are numpy's (the reference draws from Zig's)."""
+        self._changed()
         if settings:
             self.configure(**settings)
         s = self.settings
@@ -341,15 +351,23 @@ class Solver:
             raise Error(f"index_cells: method {method!r} is outside this build (spherical only)")
         if not self._e.phases:
             raise Error("index_cells: no phases")
-        rec = self._e.index_cells(np.asarray(cells, np.int64).reshape(-1), bandwidth)
+        return self._out_of(self._e.index_cells(np.asarray(cells, np.int64).reshape(-1), bandwidth))
+
+    def _out_of(self, rec):
         med, spread = self._calibrate()
         z = (rec[..., 5] - med[:, None]) / spread[:, None]
         return {"quaternions": rec[..., :4].copy(), "ncc": rec[..., 4].copy(), "nmi": rec[..., 5].copy(), "step": rec[..., 6].copy(), "z": z}
+
+    def _changed(self):
+        """Settings, geometry, patterns or phases changed: a batch launched ahead is stale."""
+        self._gen = getattr(self, "_gen", 0) + 1
+        self._pending = None
 
     def _sweep_bandwidth(self, i):
         return self._e.settings.sweep_bandwidth_of(i)
 
     def index_begin(self):
+        self._changed()
         if not self._e.phases:
             raise Error("index: no phases")
         rows, cols = self._e.shape[0], self._e.shape[1]
@@ -372,10 +390,21 @@ class Solver:
         if left == 0:
             return 0
         s = self.settings
-        n = min(left, max(s["batch"], 1) * max(2, min(16, 4096 // max(s["batch"], 1))) if s["batch"] < 4096 else left)
-        n = max(n, min(left, self._step_cells))
+        n = self._step_size(left)
         cells = self._todo[self._done:self._done + n]
-        out = self.index_cells(cells, bandwidth=self._sweep_bandwidth(self._sweep))
+        L = self._sweep_bandwidth(self._sweep)
+        pend, self._pending = self._pending, None
+        if pend is not None and pend[0] == (self._gen, self._sweep, self._done, n):
+            handle = pend[1]
+        else:
+            handle = self._e.index_cells_begin(cells, L)
+        # the sweep's next step goes to the GPU before this one is read back and landed on the
+        # host, so the device does not wait for the host between steps
+        nd = self._done + n
+        if nd < len(self._todo):
+            n2 = self._step_size(len(self._todo) - nd)
+            self._pending = ((self._gen, self._sweep, nd, n2), self._e.index_cells_begin(self._todo[nd:nd + n2], L))
+        out = self._out_of(self._e.index_cells_finish(handle))
         self._land(cells, out)
         self._done += n
         if self._done == len(self._todo) and self._sweep + 1 < s["sweeps"]:
@@ -386,6 +415,11 @@ class Solver:
                 self._done = 0
                 self._sweep += 1
         return len(self._todo) - self._done
+
+    def _step_size(self, left):
+        s = self.settings
+        n = min(left, max(s["batch"], 1) * max(2, min(16, 4096 // max(s["batch"], 1))) if s["batch"] < 4096 else left)
+        return max(n, min(left, self._step_cells))
 
     def _land(self, cells, out):
         """Each cell keeps the better match, the accepted ones their sweep (src/engine.zig:2390-2419)."""

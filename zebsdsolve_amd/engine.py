@@ -630,15 +630,19 @@ class Engine:
     def index_cells(self, cells, L=None):
         """Records [n_phases, n, 8] (q, ncc, nmi, step, stamp) for the given map cells; the
         stamp is checked (src/engine.zig:2127-2137)."""
+        return self.index_cells_finish(self.index_cells_begin(cells, L))
+
+    def index_cells_begin(self, cells, L=None):
+        """index_cells' launches, without waiting for them: a handle for index_cells_finish."""
         if not self.phases:
             raise ValueError("no phases")
         cells = np.asarray(cells, np.int64).reshape(-1)
-        if len(cells) == 0:
-            return np.zeros((len(self.phases), 0, RECORD), np.float32)
+        n = len(cells)
+        if n == 0:
+            return [], 0
         raw = self._raw_for(cells)
         t = self.positions(cells)
         dev = self._device()
-        n = len(cells)
         # a large request is cut into consecutive parts on streams of their own, so that one
         # part's correlation runs beside another's refinement (each cell's record does not depend
         # on the batch it is in)
@@ -651,8 +655,15 @@ class Engine:
             items = np.concatenate([t[a:b], np.arange(b - a, dtype=np.float32)[:, None]], 1)
             items_d = torch.from_numpy(items).to(dev)
             parts.append(self.run_batch(raw[a:b], items_d, L, lane=lanes[i]))
+        return parts, k
+
+    def index_cells_finish(self, handle):
+        """The records of an index_cells_begin, read back and stamp-checked."""
+        parts, k = handle
+        if k == 0:
+            return np.zeros((len(self.phases), 0, RECORD), np.float32)
         if k > 1:
-            torch.cuda.synchronize(dev)
+            torch.cuda.synchronize(self._device())
         outs = []
         for rec, stamp in parts:
             o = rec.cpu().numpy()
