@@ -639,7 +639,7 @@ class Engine:
         cells = np.asarray(cells, np.int64).reshape(-1)
         n = len(cells)
         if n == 0:
-            return [], 0
+            return [], 0, None
         raw = self._raw_for(cells)
         t = self.positions(cells)
         dev = self._device()
@@ -653,20 +653,55 @@ class Engine:
         for i in range(k):
             a, b = int(bounds[i]), int(bounds[i + 1])
             items = np.concatenate([t[a:b], np.arange(b - a, dtype=np.float32)[:, None]], 1)
-            items_d = torch.from_numpy(items).to(dev)
+            items_d = self._items_to_device(items, dev)
             parts.append(self.run_batch(raw[a:b], items_d, L, lane=lanes[i]))
-        return parts, k
+        # the point on this stream where the request's records are complete: the read-back
+        # waits for it alone, not for what is launched after it
+        done = torch.cuda.Event()
+        done.record()
+        return parts, k, done
+
+    def _items_to_device(self, items, dev):
+        """items [n, 4] float32 on the device without the host waiting for the stream's earlier
+        kernels (a pageable upload would): through one of two pinned buffers allocated once and
+        used in turn, each refilled only after its previous upload has completed."""
+        if torch.device(dev).type != "cuda":
+            return torch.from_numpy(items).to(dev)
+        self._stage_k = 1 - getattr(self, "_stage_k", 0)
+        stage = getattr(self, "_stage", None) or [[None, None], [None, None]]
+        self._stage = stage
+        buf, ev = stage[self._stage_k]
+        flat = np.ascontiguousarray(items, np.float32).reshape(-1)
+        if ev is not None:
+            ev.synchronize()
+        if buf is None or buf.numel() < flat.size:
+            buf = torch.empty((max(flat.size, 1 << 16),), dtype=torch.float32, pin_memory=True)
+        buf[:flat.size].copy_(torch.from_numpy(flat))
+        out = torch.empty(items.shape, dtype=torch.float32, device=dev)
+        out.view(-1).copy_(buf[:flat.size], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        stage[self._stage_k] = [buf, ev]
+        return out
 
     def index_cells_finish(self, handle):
         """The records of an index_cells_begin, read back and stamp-checked."""
-        parts, k = handle
+        parts, k, done = handle
         if k == 0:
             return np.zeros((len(self.phases), 0, RECORD), np.float32)
         if k > 1:
             torch.cuda.synchronize(self._device())
+        # read back on a stream of its own, after the request's event: a request launched
+        # since (Solver.index_step launches the next step first) keeps running meanwhile
+        rb = getattr(self, "_readback", None)
+        if rb is None:
+            rb = self._readback = torch.cuda.Stream(device=self._device())
+        rb.wait_event(done)
         outs = []
         for rec, stamp in parts:
-            o = rec.cpu().numpy()
+            rec.record_stream(rb)
+            with torch.cuda.stream(rb):
+                o = rec.cpu().numpy()
             if not np.all(o[..., 7] == np.float32(stamp)):
                 raise RuntimeError("DeviceSkippedBatch: records without this batch's stamp")
             outs.append(o)
