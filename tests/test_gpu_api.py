@@ -167,9 +167,10 @@ def test_tune_climbs_the_knobs_and_keeps_the_results():
     r0 = s.index()
     calls = []
     out = s.tune(lambda k: calls.append(k) or len(calls) < 5)
-    assert set(out["before"]) == set(out["after"]) == {"step_cells", "spec_budget_mb"}
+    assert set(out["before"]) == set(out["after"]) == {"step_cells", "spec_budget_mb", "index_lanes"}
     assert len(calls) <= 5 and "tuned on" in out["summary"]
     assert 256 <= out["after"]["step_cells"] <= 1 << 20 and 64 <= out["after"]["spec_budget_mb"] <= 1 << 16
+    assert 1 <= out["after"]["index_lanes"] <= 4
     r1 = s.index()
     q0, q1 = r0["quaternions"].reshape(-1, 4), r1["quaternions"].reshape(-1, 4)
     assert np.allclose(np.abs(np.sum(q0 * q1, 1)), 1.0, atol=1e-6)

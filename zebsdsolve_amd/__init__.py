@@ -508,8 +508,9 @@ class Solver:
         """Fit how the work is cut up to this machine and this scan (python/zebsdsolve/__init__.py
         tune): a sample of the region is indexed and timed, and each of this build's knobs in
         turn climbs from the value it has, doubled while that is faster, else halved while that
-        is: the cells indexed a step (`index_step`, 16 384) and the correlation's workspace a
-        launch may take (the engine's V budget, 4 GiB). The sample is sized by the bandwidth
+        is: the cells indexed a step (`index_step`, 16 384), the correlation's workspace a
+        launch may take (the engine's V budget, 4 GiB) and the streams a step is spread over
+        (`Engine.index_lanes`, 1 to 4). The sample is sized by the bandwidth
         (8 192 patterns up to 32, 4 096 up to 64, 1 024 up to 128, 128 beyond) and timed the
         way index_step runs it. Neither the orientations found nor which matches are accepted
         depend on these knobs. The reference's own knobs (batch, refine_pixels, wg_refine,
@@ -528,7 +529,8 @@ class Solver:
         want = 8192 if L <= 32 else 4096 if L <= 64 else 1024 if L <= 128 else 128
         cells = np.arange(min(want, rows * cols), dtype=np.int64)
         knobs = {"step_cells": (lambda: self._step_cells, lambda v: setattr(self, "_step_cells", v), 256, 1 << 20),
-                 "spec_budget_mb": (lambda: e.spec_budget >> 20, lambda v: setattr(e, "spec_budget", v << 20), 64, 1 << 16)}
+                 "spec_budget_mb": (lambda: e.spec_budget >> 20, lambda v: setattr(e, "spec_budget", v << 20), 64, 1 << 16),
+                 "index_lanes": (lambda: e.index_lanes, lambda v: setattr(e, "index_lanes", v), 1, 4)}
         before = {k: g() for k, (g, _, _, _) in knobs.items()}
         done = 0
 
