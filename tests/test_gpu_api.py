@@ -202,3 +202,37 @@ def test_steps_launched_ahead_match_one_step_and_follow_configure():
     s.configure(refine=True)
     with_refine = s.index_cells(np.arange(64, 128))["ncc"][0]
     assert not np.array_equal(got, with_refine)
+
+
+def test_chance_is_measured_again_after_what_it_depends_on_changes():
+    """The chance calibration (median and spread of NMI at random orientations) is dropped by
+    every setter: a filter toggled, or a phase swapped for another, gives a new measurement (the
+    reference keys it on every setting, the geometry and the scan's and phases' serials,
+    src/engine.zig:2202-2240). Index progress and the chance arrive as events; profile_report is
+    the reference's text."""
+    s = _solver()
+    s.add_phase(NI)
+    s.synthesize(toy_rows=6, toy_cols=8, toy_grains=12, toy_counts=400, toy_seed=2)
+    s.events()
+    s.profile(True)
+    s.index()
+    ev = s.events()
+    assert any(e["kind"] == "progress" and e["stage"] == "index" and e["done"] == e["total"] == 48 for e in ev)
+    assert any(e["kind"] == "log" and e["text"].startswith("chance, phase 0:") for e in ev)
+    rep = s.profile_report()
+    s.profile(False)
+    lines = rep.splitlines()
+    assert lines[0].endswith("dispatches  total") and any("k_refine" in ln for ln in lines[1:])
+    ms = [float(ln.split()[0]) for ln in lines[1:]]
+    assert ms == sorted(ms, reverse=True)
+    c0 = s.chance(0)
+    s.configure(dynamic_background="subtract")
+    assert s._chance is None
+    c1 = s.chance(0)
+    assert c1 != c0
+    s.configure(dynamic_background="none")
+    assert s.chance(0) == c0
+    s.clear_phases()
+    s.add_phase(GAN)
+    c2 = s.chance(0)
+    assert c2 != c0

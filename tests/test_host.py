@@ -362,3 +362,42 @@ def test_left_jacobian_of_exp():
             q = q * np.sign(q[0])
             # the rotation vector of exp(phi + h e_i) exp(phi)^-1 is J h e_i to first order
             assert np.abs(2 * q[1:] / h - J[:, i]).max() < 1e-5
+
+
+def test_solver_surface_returns_the_reference_shapes():
+    """The Solver calls a drop-in caller unpacks (python/zebsdsolve/__init__.py): reduce returns
+    (quaternions (n, 4) float32, euler (n, 3) float64), each quaternion the largest-w equivalent
+    with its Bunge angles; sweep_plan lists (bandwidth, dictionary grid, method) a sweep with
+    equal ratios (src/settings.zig:465-483); events() drains the queue; cancel() makes the next
+    index_step return 0 and index_begin clears it."""
+    import zebsdsolve_amd as ze
+    from zebsdsolve_amd import rot
+    s = ze.Solver()
+    assert s.add_phase(NI) == 0
+    ev = s.events()
+    assert ev and ev[0]["kind"] == "log" and ev[0]["text"].startswith("phase 0:")
+    assert s.events() == []
+    rng = np.random.default_rng(1)
+    q = rot.qstd(rng.normal(size=(5, 4)))
+    fz, eu = s.reduce(q)
+    assert fz.dtype == np.float32 and fz.shape == (5, 4) and eu.dtype == np.float64 and eu.shape == (5, 3)
+    rots = s._e.phases[0].master.rotations
+    for k in range(5):
+        # the representative is an equivalent of q with the largest real part
+        cands = rot.qmul(np.asarray(rots, np.float64), np.repeat(q[k:k + 1], len(rots), 0))
+        assert abs(abs(fz[k, 0]) - np.abs(cands[:, 0]).max()) < 1e-6
+        assert np.allclose(rot.from_bunge(eu[k:k + 1])[0] * np.sign(fz[k, 0]), fz[k], atol=1e-5)
+    q1, e1 = s.reduce(q[0])
+    assert q1.shape == (1, 4) and e1.shape == (1, 3)
+    s.configure(sweeps=3, sweep_bandwidth=32, bandwidth=64)
+    assert s.sweep_plan == [(32, 0, "spherical"), (45, 0, "spherical"), (64, 0, "spherical")]
+    s.configure(sweeps=1)
+    assert s.sweep_plan == [(64, 0, "spherical")]
+    s.set_patterns(np.zeros((2, 3, 60, 60), np.float32))
+    s.index_begin()
+    s.cancel()
+    assert s.index_step() == 0 and s.index_status["done"] == 0
+    s.index_begin()
+    assert not s._cancel
+    with pytest.raises(ze.Error):
+        s.reduce(q, phase=3)

@@ -10,6 +10,7 @@ import scipy.special
 
 from oracle import pipeline as op
 from oracle import sh as osh
+from tests.conftest import NI
 
 
 def rand_coef(rng, L, scale=1.0):
@@ -456,3 +457,127 @@ def test_six_direction_fit_climbs_to_the_truth():
     assert np.linalg.norm(x[:3]) < 2.0 and np.linalg.norm(x[3:]) < np.radians(2.0)
     t1, q1, n1 = op.fit6(grad.reshape(-1, 4), side, fr, pat, t0, q0, iters=1, lambda0=0.0)
     assert n1 > n0 and np.allclose(t1, t0 + x[:3], atol=1e-9)
+
+
+def _real_field(coef, L, v):
+    """The real field sum over signed m of a_lm Y_lm at directions v, by scipy (no oracle code)."""
+    ms, ls = osh.pairs(L)
+    th = np.arccos(np.clip(v[..., 2], -1, 1))
+    ph = np.arctan2(v[..., 1], v[..., 0])
+    out = np.zeros(v.shape[:-1])
+    for c, m, l in zip(coef, ms, ls):
+        if c != 0:
+            out += (1.0 if m == 0 else 2.0) * np.real(c * scipy.special.sph_harm_y(l, m, th, ph))
+    return out
+
+
+def _gl_dirs(nt, nphi):
+    x, w = np.polynomial.legendre.leggauss(nt)
+    phi = 2 * np.pi * np.arange(nphi) / nphi
+    st = np.sqrt(1 - x * x)
+    d = np.stack(np.broadcast_arrays(st[:, None] * np.cos(phi), st[:, None] * np.sin(phi), x[:, None]), -1)
+    return d, w[:, None] * (2 * np.pi / nphi) * np.ones((1, nphi))
+
+
+def test_sphere_blur_scales_a_degree_l_field_by_the_heat_kernel_of_its_lattice():
+    """sphere_blur (mastersph.wgsl:87-109; src/prep.zig:215-237: a 9 x 9 lattice of step
+    0.6 sigma in the tangent plane, weights exp(-(a^2 + b^2) / 2 sigma^2)) is, to first order, a
+    rotation-invariant smoothing, so a degree-l harmonic comes back scaled by one number:
+    (1) that number, the lattice's zonal mean of P_l(cos theta) (tan theta = the lattice radius),
+    is the heat kernel exp(-l(l+1) s^2 / 2) of the lattice's own per-axis variance s^2, taken from
+    the 1-D lattice alone (0.947 sigma^2: a step of 0.5 sigma, a weight exp(-x^2 / sigma^2) or a
+    wider lattice would each move it); (2) the oracle's blur of Re Y_lm (l = 2, 6, 12) sampled on
+    the sheets is that number times the field at every texel tested, within the sheets' bilinear
+    error. A misread tangent basis, step, weight or normalization fails (2) by 2e-2 or more."""
+    side, sigma = 201, 0.04
+    d = op.texel_dirs(side)
+    rng = np.random.default_rng(0)
+    sel = rng.choice(len(d), 3000, replace=False)
+    x1 = 0.6 * np.arange(-4, 5)
+    w1 = np.exp(-x1 * x1 / 2)
+    s2 = np.sum(w1 * x1 * x1) / w1.sum() * sigma ** 2
+    assert abs(s2 / sigma ** 2 - 0.947) < 1e-3
+    i, j = np.meshgrid(np.arange(-4, 5), np.arange(-4, 5), indexing="ij")
+    r = 0.6 * sigma * np.hypot(i, j)
+    wl = np.exp(-(r * r) / (2 * sigma ** 2))
+    th, ph = np.arccos(np.clip(d[:, 2], -1, 1)), np.arctan2(d[:, 1], d[:, 0])
+    for l, m in ((2, 1), (6, 3), (12, 5)):
+        lam = np.sum(wl * scipy.special.eval_legendre(l, 1 / np.sqrt(1 + r * r))) / wl.sum()
+        heat = np.exp(-l * (l + 1) * s2 / 2)
+        assert abs(lam - heat) < 2e-4, (l, lam, heat)
+        Y = np.real(scipy.special.sph_harm_y(l, m, th, ph))
+        got = op.blur(Y, side, sigma, texels=sel)
+        err = np.abs(got - lam * Y[sel]).max() / np.abs(Y).max()
+        assert err < 4e-3, (l, err, 1 - lam)
+
+
+def test_square_of_the_master_obeys_parseval_and_brute_force_quadrature():
+    """harmonics' f2 (src/master.zig:680-710: the band-limited master synthesized on 2L x 4L,
+    squared, analysed again): its f2_00 is Parseval's sum k |f_lm|^2 / sqrt(4 pi) (k = 1 for
+    m = 0, 2 above), and every f2_lm equals the projection of the squared field onto Y_lm by a
+    different quadrature (48 x 96 points, scipy's harmonics), on the nickel master at L = 8."""
+    from zebsdsolve_amd import master
+    L = 8
+    m = master.load(NI)
+    f, f2, fold, inversion = op.harmonics(m.raw, m.side, L)
+    ms, ls = osh.pairs(L)
+    k = np.where(ms == 0, 1.0, 2.0)
+    assert abs(f2[0].real - np.sum(k * np.abs(f) ** 2) / np.sqrt(4 * np.pi)) < 1e-12 * np.abs(f2[0])
+    d, wt = _gl_dirs(48, 96)
+    F = _real_field(f, L, d)
+    th, ph = np.arccos(np.clip(d[..., 2], -1, 1)), np.arctan2(d[..., 1], d[..., 0])
+    want = np.array([np.sum(wt * F * F * np.conj(scipy.special.sph_harm_y(l, mm, th, ph))) for mm, l in zip(ms, ls)])
+    assert np.abs(f2 - want).max() < 1e-10 * np.abs(want).max()
+    # and the square is not the f of a master squared texel by texel (what a misread would take)
+    assert np.abs(f2[1:]).max() > 1e-3 * np.abs(f2[0])
+
+
+def test_normalized_correlation_of_a_window_limited_match_is_one():
+    """So3.rden (so3.wgsl:614-626: 1 / sqrt(<w,f^2> - <w,f>^2 / area)) with So3.volume: for a
+    pattern that is the master rotated by S, seen through a window w (a spherical cap) with its
+    window mean removed, volume x rden is its NCC with the master at every cell. At S it is 1,
+    the maximum; at other cells it equals the NCC by brute-force quadrature of the rotated master
+    (scipy's harmonics at the rotated directions). The master is band-limited to l < 4, so its
+    square lies below L = 8 and every inner product is exact on the quadrature grid: a dropped
+    area, a missing square root or a transposed volume fails by far more than the 1e-10 here."""
+    L, lf = 8, 4
+    rng = np.random.default_rng(3)
+    ms, ls = osh.pairs(L)
+    f = (rng.normal(size=len(ms)) + 1j * rng.normal(size=len(ms))) * ((ls < lf) & (ls > 0))
+    f = np.where(ms == 0, f.real + 0j, f)
+    so = op.So3(L, 1)
+    n = so.n
+    nt, nphi = 24, 48
+    x, wq = osh.gl(nt)
+    phi = 2 * np.pi * np.arange(nphi) / nphi
+    st = np.sqrt(1 - x * x)
+    D = np.stack(np.broadcast_arrays(st[:, None] * np.cos(phi), st[:, None] * np.sin(phi), x[:, None]), -1)
+    wt = wq[:, None] * (2 * np.pi / nphi) * np.ones((1, nphi))
+    cap = np.array([0.3, -0.2, 0.93])
+    win = (D @ (cap / np.linalg.norm(cap)) > np.cos(0.7)).astype(float)
+    area = np.sum(win * wt)
+
+    def rotated_master(q):
+        # f(S^-1 x) at the grid directions
+        qi = op.qconj(q)
+        u = np.asarray(qi[1:])
+        t = 2 * np.cross(u, D)
+        return _real_field(f, L, D + qi[0] * t + np.cross(u, t))
+
+    ia, jb, ic = 3, 5, 11
+    fS = rotated_master(op.zyz(2 * np.pi * ia / n, 2 * np.pi * jb / n, 2 * np.pi * ic / so.nc))
+    g = win * (fS - np.sum(win * fS * wt) / area)
+    gnorm = np.sqrt(np.sum(g * g * wt))
+    x2, w2 = osh.gl(2 * L)
+    s2 = osh.synthesize(f, L, x2, 4 * L)
+    f2 = osh.analyze(s2 * s2, L, x2, w2)
+    w_lm = osh.analyze(win, L, x, wq)
+    score = so.volume(f, osh.analyze(g, L, x, wq)) * so.rden(so.volume(f, w_lm), so.volume(f2, w_lm), area) / gnorm
+    assert abs(score[jb, ic, ia] - 1) < 1e-10
+    assert np.unravel_index(np.argmax(score), score.shape) == (jb, ic, ia)
+    for _ in range(4):
+        jj, cc, aa = int(rng.integers(0, so.nb)), int(rng.integers(0, so.nc)), int(rng.integers(0, n))
+        fq = rotated_master(op.zyz(2 * np.pi * aa / n, 2 * np.pi * jj / n, 2 * np.pi * cc / so.nc))
+        mq = np.sum(win * fq * wt) / area
+        ncc = np.sum(g * fq * wt) / np.sqrt(np.sum(win * (fq - mq) ** 2 * wt)) / gnorm
+        assert abs(score[jj, cc, aa] - ncc) < 1e-10, (jj, cc, aa)

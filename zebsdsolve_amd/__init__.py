@@ -70,6 +70,8 @@ class Solver:
         self._gen = 0
         self._idx = None
         self._step_cells = 16384  # cells index_step takes at least (tune climbs it)
+        self._events = []
+        self._cancel = False
 
     def close(self):
         self._e = None
@@ -220,9 +222,13 @@ class Solver:
         """Load a master pattern (ebsdsim .npz); returns its index."""
         self._changed()
         try:
-            return self._e.add_phase(str(path))
+            k = self._e.add_phase(str(path))
         except (OSError, ValueError) as ex:
             raise Error(f"add_phase: {ex}") from None
+        m = self._e.phases[k].master
+        self._log(f"phase {k}: {m.name} ({m.side} px, point group {m.point_group}); orientations reduced by its "
+                  f"{len(m.rotations)} rotations")
+        return k
 
     def clear_phases(self):
         self._changed()
@@ -285,11 +291,17 @@ class Solver:
         if s["toy_blank"] > 0:
             phase = np.where(r.random(n) < s["toy_blank"], -1, phase).astype(np.int32)
         data = np.zeros((n, h, w), np.float64)
+        self._cancel = False
+        made = 0
         for p in range(len(self._e.phases)):
             sel = np.nonzero(phase == p)[0]
             for c0 in range(0, len(sel), 16384):
+                if self._cancel:
+                    raise Error("synthesize: cancelled")
                 cs = sel[c0:c0 + 16384]
                 data[cs] = self._e.simulate(p, cs, q[cs].astype(np.float32))
+                made += len(cs)
+                self._progress("synthesize", made, int((phase >= 0).sum()))
         crystals = phase >= 0
         mean = data[crystals].mean() if crystals.any() else 0.0
         low = data[crystals].min() if crystals.any() else 0.0
@@ -306,6 +318,9 @@ class Solver:
         out = rn.poisson(s["toy_counts"] * inten).astype(np.float32) if s["toy_counts"] > 0 else inten.astype(np.float32)
         self._e.set_patterns(out.reshape(rows, cols, h, w))
         self._res = None
+        self._log(f"toy scan: {rows} × {cols} patterns of {h} × {w}, {ng} grains of {len(self._e.phases)} phase(s), "
+                  f"{int((phase < 0).sum())} blank cells, contrast {s['toy_contrast']:.2f}, {s['toy_counts']:.4f} electrons a pixel, "
+                  f"seed {s['toy_seed']}")
         return {"quaternions": q.astype(np.float32).reshape(rows, cols, 4), "phase": phase.reshape(rows, cols),
                 "grain": grain.astype(np.int32).reshape(rows, cols), "geometry": self.geometry}
 
@@ -315,8 +330,10 @@ class Solver:
         """What NMI chance gives per phase: about 256 patterns of the map on a lattice, each at 8
         random orientations, median and 1.4826 x median deviation (src/engine.zig:2202-2292)."""
         e = self._e
-        key = (e.shape, tuple(e.geo.as_dict()["pc"]), e.geo.sample_tilt_deg, e.geo.detector_tilt_deg, tuple(e.geo.rot), e.geo.px_um,
-               len(e.phases), e.settings["nmi_bins"], e.settings["mask"], e.settings["mask_radius"], e.settings["master_smooth"], id(e.patterns))
+        # every setter drops the calibration (_changed); the key also guards changes made to the
+        # engine underneath this surface
+        key = (e.shape, repr(sorted(e.settings.as_dict().items())), repr(sorted(e.geo.as_dict().items())),
+               tuple(id(p) for p in e.phases), id(e.patterns))
         if self._chance is not None and self._chance_key == key:
             return self._chance
         rows, cols = e.shape[0], e.shape[1]
@@ -333,6 +350,9 @@ class Solver:
             dev = np.sort(np.abs(c - m))
             med.append(float(m))
             spread.append(max(1.4826 * float(dev[len(c) // 2]), 1e-6))
+            az = e.settings["accept_z"]
+            self._log(f"chance, phase {pi}: NMI {med[-1]:.4f} ± {spread[-1]:.4f} over {n} patterns at random orientations; "
+                      f"z {az:.0f} is NMI {med[-1] + az * spread[-1]:.4f}")
         self._chance = (np.array(med, np.float32), np.array(spread, np.float32))
         self._chance_key = key
         return self._chance
@@ -359,15 +379,47 @@ class Solver:
         return {"quaternions": rec[..., :4].copy(), "ncc": rec[..., 4].copy(), "nmi": rec[..., 5].copy(), "step": rec[..., 6].copy(), "z": z}
 
     def _changed(self):
-        """Settings, geometry, patterns or phases changed: a batch launched ahead is stale."""
+        """Settings, geometry, patterns or phases changed: a batch launched ahead is stale, and so
+        is the chance calibration (every score depends on them; the reference keys its
+        calibration on every setting, the geometry and the scan's and phases' serials,
+        src/engine.zig:2202-2240)."""
+        self._drop_pending()
+        self._chance = None
+        self._chance_key = None
+
+    def _drop_pending(self):
         self._gen = getattr(self, "_gen", 0) + 1
         self._pending = None
+
+    # ── events and cancelling ──
+
+    def _log(self, text):
+        self._events.append({"kind": "log", "text": text})
+
+    def _progress(self, stage, done, total, **more):
+        self._events.append({"kind": "progress", "stage": stage, "done": int(done), "total": int(total), **more})
+
+    def events(self):
+        """Progress events queued since the last call, as a list of dicts: {"kind": "log",
+        "text": ...} and {"kind": "progress", "stage": ..., "done": ..., "total": ...}
+        (python/zebsdsolve/__init__.py:114-123; src/events.zig:1-4)."""
+        out, self._events = self._events, []
+        return out
+
+    def cancel(self):
+        """Ask a running indexing, tuning or toy-scan synthesis to stop at its next step: index
+        keeps what it has (index_step returns 0), tune and synthesize raise Error
+        (src/exports.zig:62-65; checked between steps as src/engine.zig:2429)."""
+        self._cancel = True
 
     def _sweep_bandwidth(self, i):
         return self._e.settings.sweep_bandwidth_of(i)
 
     def index_begin(self):
-        self._changed()
+        """Start indexing the region; index_step then does a few batches at a time. Clears a
+        cancel() asked for before (src/engine.zig:2335-2348)."""
+        self._drop_pending()
+        self._cancel = False
         if not self._e.phases:
             raise Error("index: no phases")
         rows, cols = self._e.shape[0], self._e.shape[1]
@@ -387,7 +439,7 @@ class Solver:
         if self._res is None:
             self.index_begin()
         left = len(self._todo) - self._done
-        if left == 0:
+        if left == 0 or self._cancel:
             return 0
         s = self.settings
         n = self._step_size(left)
@@ -407,9 +459,12 @@ class Solver:
         out = self._out_of(self._e.index_cells_finish(handle))
         self._land(cells, out)
         self._done += n
+        self._progress("index", self._done, len(self._todo), sweep=self._sweep, sweeps=s["sweeps"])
         if self._done == len(self._todo) and self._sweep + 1 < s["sweeps"]:
             # the next sweep takes what this one left
             keep = self._todo[self._res["sweep"][self._todo] < 0]
+            self._log(f"sweep {self._sweep + 1} of {s['sweeps']} (spherical, bandwidth {L}): {len(self._todo) - len(keep)} of "
+                      f"{len(self._todo)} patterns accepted, {len(keep)} left")
             if len(keep):
                 self._todo = keep
                 self._done = 0
@@ -464,6 +519,13 @@ class Solver:
                 "done": getattr(self, "_done", 0), "total": total, "region": region, "accepted": getattr(self, "_accepted", 0),
                 "method": "spherical"}
 
+    @property
+    def sweep_plan(self):
+        """What each sweep searches with, in order: a list of (bandwidth, dictionary grid,
+        method) (python/zebsdsolve/__init__.py:748-755; src/settings.zig:465-483). The
+        dictionary methods are outside this build, so the grid is 0 and the method spherical."""
+        return [(int(self._sweep_bandwidth(i)), 0, "spherical") for i in range(int(self.settings["sweeps"]))]
+
     def index(self, progress=None):
         """Index the whole map a few batches at a time; progress(status) after each step, False
         from it stops. Returns results()."""
@@ -492,9 +554,14 @@ class Solver:
     # ── symmetry ──
 
     def reduce(self, quaternions, phase=0):
-        """Each orientation's fundamental-zone representative under the phase's rotations."""
-        q = np.asarray(quaternions, np.float64)
-        return _sym.to_fz(q.reshape(-1, 4), self._e.phases[phase].master.rotations).reshape(q.shape).astype(np.float32)
+        """Orientations as the results store them: (quaternions (n, 4) float32, euler (n, 3)
+        float64), each quaternion's representative in the phase's fundamental zone and its Bunge
+        angles in radians (python/zebsdsolve/__init__.py:907-916)."""
+        if not 0 <= int(phase) < len(self._e.phases):
+            raise Error(f"reduce: no phase {phase}")
+        q = np.asarray(quaternions, np.float32).reshape(-1, 4).astype(np.float64)
+        fz = _sym.to_fz(q, self._e.phases[int(phase)].master.rotations)
+        return fz.astype(np.float32), _rot.to_bunge(fz)
 
     def disorientation(self, a, b, phase=0):
         """Disorientation angles (radians) between orientations a and b of one phase."""
@@ -512,42 +579,74 @@ class Solver:
         launch may take (the engine's V budget, 4 GiB) and the streams a step is spread over
         (`Engine.index_lanes`, 1 to 4). The sample is sized by the bandwidth
         (8 192 patterns up to 32, 4 096 up to 64, 1 024 up to 128, 128 beyond) and timed the
-        way index_step runs it. Neither the orientations found nor which matches are accepted
-        depend on these knobs. The reference's own knobs (batch, refine_pixels, wg_refine,
-        wg_fft) cut up its WebGPU dispatches and mean nothing here. Returns the knobs before
-        and after and a summary; `progress(trials_done)` is called after each timed trial,
-        and returning False from it stops (the best so far stays)."""
+        way index_step runs it. A move is tried only when it changes the sample's work (a step
+        of at least the sample, or a budget that leaves the chunks as they are, is the same
+        work), is kept only when it is faster by more than 3 % (timing noise), and a trial that
+        fails (out of memory, say) is a loss, its value put back (the reference scores a failed
+        trial the same way, python/zebsdsolve/__init__.py:705-710). Neither the orientations
+        found nor which matches are accepted depend on these knobs. The reference's own knobs
+        (batch, refine_pixels, wg_refine, wg_fft) cut up its WebGPU dispatches and mean nothing
+        here. Returns the knobs before and after and a summary; `progress(trials_done)` is
+        called after each timed trial, and returning False from it stops (the best so far
+        stays); cancel() stops it with an Error."""
         import time
 
         import torch
 
+        from .engine import LANE_MIN_ITEMS
+
         if not self._e.phases:
             raise Error("tune: no phases")
+        self._cancel = False
         e = self._e
         L = int(self.settings["bandwidth"])
         rows, cols = e.shape[0], e.shape[1]
         want = 8192 if L <= 32 else 4096 if L <= 64 else 1024 if L <= 128 else 128
         cells = np.arange(min(want, rows * cols), dtype=np.int64)
-        knobs = {"step_cells": (lambda: self._step_cells, lambda v: setattr(self, "_step_cells", v), 256, 1 << 20),
-                 "spec_budget_mb": (lambda: e.spec_budget >> 20, lambda v: setattr(e, "spec_budget", v << 20), 64, 1 << 16),
-                 "index_lanes": (lambda: e.index_lanes, lambda v: setattr(e, "index_lanes", v), 1, 4)}
-        before = {k: g() for k, (g, _, _, _) in knobs.items()}
+        ns = len(cells)
+
+        def steps_of(v):
+            # the step sizes index_step would cut the sample into
+            return tuple(min(v, ns - c0) for c0 in range(0, ns, v))
+
+        def chunks_of(mb):
+            return tuple(e.corr_chunk(k, L, mb << 20) for k in steps_of(self._step_cells))
+
+        def lanes_of(v):
+            return tuple(max(1, min(v, k // LANE_MIN_ITEMS)) for k in steps_of(self._step_cells))
+
+        # name: (get, put, lowest, highest, the work a value gives the sample)
+        knobs = {"step_cells": (lambda: self._step_cells, lambda v: setattr(self, "_step_cells", v), 256, 1 << 20, steps_of),
+                 "spec_budget_mb": (lambda: e.spec_budget >> 20, lambda v: setattr(e, "spec_budget", v << 20), 64, 1 << 16, chunks_of),
+                 "index_lanes": (lambda: e.index_lanes, lambda v: setattr(e, "index_lanes", v), 1, 4, lanes_of)}
+        before = {k: g() for k, (g, _, _, _, _) in knobs.items()}
         done = 0
 
         def trial():
             nonlocal done
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for c0 in range(0, len(cells), self._step_cells):
-                e.index_cells(cells[c0:c0 + self._step_cells], L)
-            torch.cuda.synchronize()
+            if self._cancel:
+                raise Error("tune: cancelled")
+            try:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for c0 in range(0, ns, self._step_cells):
+                    e.index_cells(cells[c0:c0 + self._step_cells], L)
+                torch.cuda.synchronize()
+                t = time.perf_counter() - t0
+            except (RuntimeError, _native.NativeError):
+                # a value this machine cannot run loses
+                torch.cuda.synchronize()
+                t = float("inf")
             done += 1
+            self._progress("tune", done, 12)
             go = progress(done) if progress is not None else True
-            return time.perf_counter() - t0, go is not False
+            return t, go is not False
 
         trial()  # tables, volumes and workspaces, outside the timings
         best, go = trial()
-        for name, (get, put, lo, hi) in knobs.items():
+        if not np.isfinite(best):
+            raise Error("tune: the sample does not run with the present settings")
+        for name, (get, put, lo, hi, work) in knobs.items():
             if not go:
                 break
             for factor in (2, 0.5):
@@ -555,28 +654,42 @@ class Solver:
                 while go:
                     cur = get()
                     nxt = int(cur * factor)
+                    # values past the range, or that give the sample the same work, are skipped
+                    while lo <= nxt <= hi and nxt != cur and work(nxt) == work(cur):
+                        nxt = int(nxt * factor)
                     if not (lo <= nxt <= hi) or nxt == cur:
                         break
                     put(nxt)
                     t, go = trial()
-                    if t < best:
+                    if t < 0.97 * best:
                         best, moved = t, True
                     else:
                         put(cur)
                         break
                 if moved:
                     break
-        after = {k: g() for k, (g, _, _, _) in knobs.items()}
-        summary = f"tuned on {len(cells)} patterns at L={L}: {len(cells) / best:.0f} patterns/s in {done} trials; " + \
+        after = {k: g() for k, (g, _, _, _, _) in knobs.items()}
+        summary = f"tuned on {ns} patterns at L={L}: {ns / best:.0f} patterns/s in {done} trials; " + \
                   ", ".join(f"{k} {before[k]} -> {after[k]}" for k in knobs)
+        self._log(summary)
         return {"before": before, "after": after, "summary": summary}
 
     def profile(self, on=True):
+        """Time every kernel launch on the GPU from now on (events around each); returns True
+        (python/zebsdsolve/__init__.py profile)."""
         _native.profile(on)
+        return True
 
     def profile_report(self):
-        """{kernel: (launches, total ms)} since profile(True)."""
-        return _native.profile_report()
+        """GPU time per kernel since profile(True), slowest first, as text: a total line
+        "ms count dispatches  total", then "ms count  kernel" per kernel (the reference's
+        layout, src/gpu/gpu.zig:291-308)."""
+        rows = sorted(_native.profile_report().items(), key=lambda kv: -kv[1][1])
+        total = sum(ms for _, (_, ms) in rows)
+        calls = sum(c for _, (c, _) in rows)
+        out = [f"{total:9.3f} ms {calls:6d} dispatches  total"]
+        out += [f"{ms:9.3f} ms {c:6d}  {name}" for name, (c, ms) in rows]
+        return "\n".join(out) + "\n"
 
 
 __all__ = ["Solver", "Result", "index", "Error", "decoys", "__version__"]

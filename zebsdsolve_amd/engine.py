@@ -469,6 +469,23 @@ class Engine:
                 return False
         return mfma_correlation(L, n)
 
+    def corr_chunk(self, n, L=None, budget=None):
+        """Items per correlation launch for a batch of n at L under a workspace budget (bytes;
+        the engine's spec_budget by default), as run_batch cuts it."""
+        lib = nat.load()
+        L = int(L or self.settings["bandwidth"])
+        band = self.band(L)
+        per = 1
+        for pi in range(len(self.phases)):
+            ph = self.phase_band(pi, L)
+            if self.use_mfma(L, pi):
+                ws = int(lib.zb_vcorr_workspace(self.corr_vtable(ph), 4096, 0))
+            else:
+                ws = int(lib.zb_so3_workspace(band.c, ph.c, 4096))
+            per = max(per, ws // 4096 + 1)
+        budget = self.spec_budget if budget is None else budget
+        return max(1, min(int(n), int(budget) // per))
+
     # ── the batch pipeline ──
 
     def refine_opts(self):
@@ -655,11 +672,17 @@ class Engine:
             items = np.concatenate([t[a:b], np.arange(b - a, dtype=np.float32)[:, None]], 1)
             items_d = self._items_to_device(items, dev)
             parts.append(self.run_batch(raw[a:b], items_d, L, lane=lanes[i]))
-        # the point on this stream where the request's records are complete: the read-back
-        # waits for it alone, not for what is launched after it
+        # the points where the request's records are complete (this stream's, and each lane's
+        # stream's): the read-back waits for them alone, not for what is launched after them
         done = torch.cuda.Event()
         done.record()
-        return parts, k, done
+        events = [done]
+        for ln in lanes:
+            if ln is not None:
+                ev = torch.cuda.Event()
+                ev.record(ln.stream)
+                events.append(ev)
+        return parts, k, events
 
     def _items_to_device(self, items, dev):
         """items [n, 4] float32 on the device without the host waiting for the stream's earlier
@@ -686,17 +709,16 @@ class Engine:
 
     def index_cells_finish(self, handle):
         """The records of an index_cells_begin, read back and stamp-checked."""
-        parts, k, done = handle
+        parts, k, events = handle
         if k == 0:
             return np.zeros((len(self.phases), 0, RECORD), np.float32)
-        if k > 1:
-            torch.cuda.synchronize(self._device())
-        # read back on a stream of its own, after the request's event: a request launched
+        # read back on a stream of its own, after the request's events: a request launched
         # since (Solver.index_step launches the next step first) keeps running meanwhile
         rb = getattr(self, "_readback", None)
         if rb is None:
             rb = self._readback = torch.cuda.Stream(device=self._device())
-        rb.wait_event(done)
+        for ev in events:
+            rb.wait_event(ev)
         outs = []
         for rec, stamp in parts:
             rec.record_stream(rb)
