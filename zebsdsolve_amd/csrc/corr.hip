@@ -153,17 +153,20 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
     // at slot group S0 / 4 and are F4 consecutive 16-byte pieces for each j
     const size_t blk = (size_t)S * NBLK * 64;
     const size_t first = ((size_t)mp * ncg + cg) * blk + (size_t)(S0 / 4) * NBLK * 64 * 4;
-    const size_t total = (size_t)a.nb * a.L * ncg * blk;
-    const auto trs = buf(tab + first, (long long)(total - first) * 4);
-    const unsigned jstride = (unsigned)((size_t)a.L * ncg * blk * 4);
+    // one j's rows of this block are F4 pieces at tab + first + j jstride floats; the resource
+    // is made per j over exactly them, so the hardware's bounds (voffset against the record
+    // count) cover every access and no offset passes 32 bits, whatever the table's size (GaN's
+    // is 2.4 GB at L = 96)
+    const size_t jstride = (size_t)a.L * ncg * blk;
     floatx4 pre[PER];
     // every thread loads (a clamped piece when the rows are fewer than the threads) so that
     // the waits count the same loads on every path
     auto stage_load = [&](int j) {
+        const auto trs = buf(tab + first + (size_t)j * jstride, (long long)F4 * 16);
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
             const int idx = min(tid + 64 * kWaves * p, F4 - 1);
-            pre[p] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(trs, idx * 16, (int)((unsigned)j * jstride), 0));
+            pre[p] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(trs, idx * 16, 0, 0));
         }
     };
     // pieces past the rows go to a spare slot after them (no predicate, so the loads above
@@ -359,15 +362,16 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
     const int ncg = a.cols / (NBLK * 32);
     const size_t blk = (size_t)off(G8);
     const size_t first = ((size_t)mp * ncg + cg) * blk + (size_t)off(G0);
-    const size_t total = (size_t)a.nb * a.L * ncg * blk;
-    const auto trs = buf(tab + first, (long long)(total - first) * 16);
-    const unsigned jstride = (unsigned)((size_t)a.L * ncg * blk * 16);
+    // one j's rows of this block: F4 16-byte pieces at tab + first + j jstride, a resource per j
+    // over exactly them (bounds on every access, offsets within 32 bits at any table size)
+    const size_t jstride = (size_t)a.L * ncg * blk;
     floatx4 pre[PER];
     auto stage_load = [&](int j) {
+        const auto trs = buf(tab + first + (size_t)j * jstride, (long long)F4 * 16);
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
             const int idx = min(tid + 64 * kWaves * p, F4 - 1);
-            pre[p] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(trs, idx * 16, (int)((unsigned)j * jstride), 0));
+            pre[p] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(trs, idx * 16, 0, 0));
         }
     };
     auto stage_store = [&](floatx4* dst) {
