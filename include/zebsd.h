@@ -150,6 +150,18 @@ int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame, const flo
                            int32_t n_items, int32_t flags, void* work, int64_t work_bytes, float* alm, float* stats,
                            void* stream);
 
+/* zb_sh_coefficients without a workspace argument, for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Its workspace is one static device array the library holds on
+ * each device it runs on until the process ends: ZB_SH_SCRATCH_BYTES (64 MiB), enough for
+ * 1 023 items at L = 64 (65 536 bytes an item and 8 more), 4 094 at L = 32 and 341 at L = 96; more items
+ * fail with ZB_ESPACE and the limit in the message. It allocates nothing and clears nothing (every
+ * scratch byte it reads it has written in the same call). Calls that overlap on one device share
+ * that array and race: serialize them (one stream, or events between streams). */
+#define ZB_SH_SCRATCH_BYTES (64LL * 1024 * 1024)
+int32_t zb_sh_coefficients_noalloc(const zb_band* band, const zb_frame* frame, const float* pats, const float* items, int32_t n_items,
+                                   int32_t flags, float* alm, float* stats, void* stream);
+
 /* Bytes of workspace zb_so3_correlate / zb_so3_denominators need for n_items. */
 int64_t zb_so3_workspace(const zb_band* band, const zb_phase* phase, int32_t n_items);
 

@@ -51,6 +51,7 @@ _SIGS = {
     "zb_prep": ([vp, i32, i32, i32, i32, f32p, ctypes.POINTER(PrepOpts), vp, vp], i32),
     "zb_sh_workspace": ([ctypes.POINTER(Band), i32], i64),
     "zb_sh_coefficients": ([ctypes.POINTER(Band), ctypes.POINTER(Frame), vp, vp, i32, i32, vp, i64, vp, vp, vp], i32),
+    "zb_sh_coefficients_noalloc": ([ctypes.POINTER(Band), ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp, vp], i32),
     "zb_so3_workspace": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), i32], i64),
     "zb_so3_correlate": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_so3_denominators": ([ctypes.POINTER(Band), ctypes.POINTER(Phase), vp, vp, i32, vp, i64, vp, vp], i32),

@@ -10,6 +10,8 @@
 #define ZB_TAU 6.28318530717959f
 // twiddle table length: tw[t] = exp(-2 pi i t / ZB_TW)
 #define ZB_TW 1024
+// devices a per-device table of the library (the no-alloc scratch addresses) has room for
+#define ZB_MAX_DEVICES 64
 
 // Launch a kernel, bracketed by timing events when profiling is on.
 #define ZB_LAUNCH(name, kern, grid, block, lds, stream, ...)                      \

@@ -14,6 +14,8 @@
 //   k_norm     unit harmonic norm and the item's stats
 // Workspace: per item the window sums (float2) and the ring spectra [L][n/2] float4 (m-major,
 // so that k_legendre reads them contiguously).
+#include <mutex>
+
 #include "common.h"
 
 namespace zb {
@@ -267,24 +269,88 @@ static int ilog2(int v) {
     return (1 << r) == v ? r : -1;
 }
 
+// The arguments zb_sh_coefficients and zb_sh_coefficients_noalloc share, checked; 1 when there
+// is nothing to do.
+static int sh_check(const char* who, const zb_band* band, const zb_frame* frame, const float* pats, const float* items, int32_t n_items,
+                    int32_t flags, float* alm, float* stats) {
+    if (!band || !frame || n_items < 0 || band->L < 2 || ilog2(band->n) < 4 || band->n < 2 * band->L - 1 || 2 * band->n > ZB_TW ||
+        frame->pat_h < 2 || frame->pat_w < 2) {
+        set_error("%s: bad band or frame", who);
+        return ZB_EINVAL;
+    }
+    if (n_items == 0) return 1;
+    if (!items || !alm || !stats || !band->ring || !band->wplm || !band->twiddle || (!(flags & ZB_FLAG_WINDOW) && !pats)) {
+        set_error("%s: null pointer", who);
+        return ZB_EINVAL;
+    }
+    return ZB_OK;
+}
+
+// The four launches over a workspace of sh_ws bytes at `work` (a device address).
+static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* pats, const float* items, int32_t n_items, int32_t flags,
+                      char* work, float* alm, float* stats, hipStream_t s);
+
 extern "C" int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame, const float* pats, const float* items,
                                       int32_t n_items, int32_t flags, void* work, int64_t work_bytes, float* alm, float* stats,
                                       void* stream) {
-    if (!band || !frame || n_items < 0 || band->L < 2 || ilog2(band->n) < 4 || band->n < 2 * band->L - 1 || 2 * band->n > ZB_TW ||
-        frame->pat_h < 2 || frame->pat_w < 2) {
-        set_error("zb_sh_coefficients: bad band or frame");
-        return ZB_EINVAL;
-    }
-    if (n_items == 0) return ZB_OK;
-    if (!items || !alm || !stats || !work || !band->ring || !band->wplm || !band->twiddle || (!(flags & ZB_FLAG_WINDOW) && !pats)) {
+    const int rc = sh_check("zb_sh_coefficients", band, frame, pats, items, n_items, flags, alm, stats);
+    if (rc) return rc < 0 ? rc : ZB_OK;
+    if (!work) {
         set_error("zb_sh_coefficients: null pointer");
         return ZB_EINVAL;
     }
-    const int L = band->L, n = band->n;
-    if (work_bytes < sh_ws(L, n, n_items)) {
-        set_error("zb_sh_coefficients: workspace %lld bytes, need %lld", (long long)work_bytes, (long long)sh_ws(L, n, n_items));
+    if (work_bytes < sh_ws(band->L, band->n, n_items)) {
+        set_error("zb_sh_coefficients: workspace %lld bytes, need %lld", (long long)work_bytes,
+                  (long long)sh_ws(band->L, band->n, n_items));
         return ZB_ESPACE;
     }
+    return sh_run(band, frame, pats, items, n_items, flags, (char*)work, alm, stats, (hipStream_t)stream);
+}
+
+// zb_sh_coefficients_noalloc's workspace: one array per device for the life of the process.
+// Host code never passes `scratch` itself to a kernel (on the host it names a stand-in, not the
+// device copy): it uses the device address hipGetSymbolAddress returns, kept per device.
+static __device__ __align__(256) char scratch[ZB_SH_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[ZB_MAX_DEVICES];
+
+extern "C" int32_t zb_sh_coefficients_noalloc(const zb_band* band, const zb_frame* frame, const float* pats, const float* items,
+                                              int32_t n_items, int32_t flags, float* alm, float* stats, void* stream) {
+    const int rc = sh_check("zb_sh_coefficients_noalloc", band, frame, pats, items, n_items, flags, alm, stats);
+    if (rc) return rc < 0 ? rc : ZB_OK;
+    const int64_t need = sh_ws(band->L, band->n, n_items);
+    if (need > (int64_t)ZB_SH_SCRATCH_BYTES) {
+        int64_t most = ZB_SH_SCRATCH_BYTES / ((int64_t)band->L * (band->n / 2) * 16 + 8);
+        while (most > 0 && sh_ws(band->L, band->n, (int)most) > (int64_t)ZB_SH_SCRATCH_BYTES) --most;
+        set_error("zb_sh_coefficients_noalloc: %d items need %lld bytes of scratch, the library holds %lld (at most %lld items at L = %d)",
+                  n_items, (long long)need, (long long)ZB_SH_SCRATCH_BYTES, (long long)most, band->L);
+        return ZB_ESPACE;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= ZB_MAX_DEVICES) {
+        set_error("zb_sh_coefficients_noalloc: no current device (or device %d past %d)", dev, ZB_MAX_DEVICES);
+        return ZB_EINVAL;
+    }
+    void* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                set_error("zb_sh_coefficients_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+                return ZB_ELAUNCH;
+            }
+            g_scratch_ptr[dev] = p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    return sh_run(band, frame, pats, items, n_items, flags, (char*)ptr, alm, stats, (hipStream_t)stream);
+}
+
+static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* pats, const float* items, int32_t n_items, int32_t flags,
+                      char* work, float* alm, float* stats, hipStream_t s) {
+    const int L = band->L, n = band->n;
     SphArg a;
     for (int i = 0; i < 3; ++i) {
         a.c0[i] = frame->c0[i];
@@ -299,8 +365,7 @@ extern "C" int32_t zb_sh_coefficients(const zb_band* band, const zb_frame* frame
     a.log2nphi = ilog2(2 * n);
     a.pairs = L * (L + 1) / 2;
     a.flags = flags;
-    hipStream_t s = (hipStream_t)stream;
-    char* ws = (char*)work;
+    char* ws = work;
     float2* sums = (float2*)ws;
     float4* spectra = (float4*)(ws + ((int64_t)n_items * 8 + 255) / 256 * 256);
     const float2* tw = (const float2*)band->twiddle;
