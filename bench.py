@@ -27,8 +27,13 @@ MASTER = os.path.join(ROOT, "tests", "golden", "Ni_master_20kV.npz")
 GAN_MASTER = os.path.join(ROOT, "tests", "golden", "GaN_master_20kV.npz")
 PC = (0.50, 0.50, 0.55)
 H = W = 60
-FP32_PEAK_TFLOPS = 157.3   # MI355X published dense FP32 (MFMA and packed VALU)
+FP32_PEAK_TFLOPS = 157.3   # MI355X published dense FP32 on the matrix cores
+VALU_PEAK_TFLOPS = 78.6    # the vector units' FP32 FMA rate with packed FP32 off (half the packed 157.3)
+LP_PEAK_TFLOPS = 2500.0    # dense FP16 / BF16 on the matrix cores
 HBM_PEAK_GBS = 8000.0
+# the unit each kernel's arithmetic runs on: the matrix cores (the correlation's GEMM, the
+# Legendre sums) or the vector units (FFTs, sampling, refinement)
+MFMA_KERNELS = ("k_vgemm", "k_legendre")
 
 
 # ── algorithmic work per pattern, on the reference's algorithm ──
@@ -49,13 +54,16 @@ def spectrum_terms(L, fold, inversion):
     return total
 
 
-def flops_per_pattern(L, fold, inversion, iters, n_mask):
+def flops_per_pattern(L, fold, inversion, iters, n_mask, b_half=False):
     """Algorithmic FLOP per pattern on the reference's algorithm (complex multiply-accumulate =
     8, radix-2 FFT = 5 N log2 N, refinement 120 per masked pixel per pass: ray 6, normalize 9,
     qrot 18, Lambert map 20, bilinear vec4 24, cross 6, 19 sums 38), attributed to the kernels
     that carry each term. The spectrum and its transforms along b and c are one GEMM on the
     matrix-core path (k_vgemm), the transform along a is k_apeak; on the slab path they are
-    k_spectrum (spectrum + b) and k_slab0 (c + a). Returns (per kernel, total)."""
+    k_spectrum (spectrum + b) and k_slab0 (c + a). With b_half (a master with a 2-fold axis
+    along y) the GEMM computes the rows j <= n/4 + 1 of the n/2 + 1 and the peak searches
+    j <= n/4: their work is counted at those shares, the rows not computed not counted.
+    Returns (per kernel, total)."""
     n = 16
     while n < 2 * L - 1:
         n *= 2
@@ -70,24 +78,35 @@ def flops_per_pattern(L, fold, inversion, iters, n_mask):
     f_rings = 5 * (2 * n) * lg(2 * n) * (n / 2)
     f_leg = 4 * pairs * (n / 2)
     f_ref = 120 * (iters + 1) * n_mask + 120 * n_mask
-    per = {"k_vgemm": f_spec + f_bfft + f_cfft, "k_apeak": f_afft, "k_spectrum": f_spec + f_bfft, "k_slab0": f_cfft + f_afft,
-           "k_rings": f_rings, "k_legendre": f_leg, "k_refine": f_ref}
-    return per, f_spec + f_bfft + f_cfft + f_afft + f_rings + f_leg + f_ref
+    g_share = (n // 4 + 2) / nb if b_half else 1.0
+    p_share = (n // 4 + 1) / nb if b_half else 1.0
+    per = {"k_vgemm": (f_spec + f_bfft + f_cfft) * g_share, "k_apeak": f_afft * p_share, "k_spectrum": f_spec + f_bfft,
+           "k_slab0": f_cfft + f_afft, "k_rings": f_rings, "k_legendre": f_leg, "k_refine": f_ref}
+    return per, per["k_vgemm"] + per["k_apeak"] + f_rings + f_leg + f_ref
 
 
-def bytes_per_pattern(L, fold, h, w):
-    """Algorithmic HBM bytes per pattern of the kernels whose data are per pattern: the spectrum
-    after the transforms along b and c (the reference's T is the same size: (n/2 + 1) L nc
-    complex64) written by k_vgemm and read by k_apeak, with the coefficients k_vgemm reads; the
-    pattern, item and record of k_refine."""
+def osh_n(L):
+    """fft_size(L): the orientation grid's points per turn."""
     n = 16
     while n < 2 * L - 1:
         n *= 2
-    nc = n // fold
-    nb = n // 2 + 1
-    spectrum = nb * L * nc * 8
-    pairs = L * (L + 1) // 2
-    return {"k_vgemm": spectrum + pairs * 8, "k_apeak": spectrum, "k_refine": 4 * h * w + 16 + 32}
+    return n
+
+
+def compulsory_bytes(h, w):
+    """HBM bytes a pattern must cross: the float32 pattern in, its item (t, index) in, its 8-float
+    record out. The per-phase tables (tens of MB) are read once per launch from the caches."""
+    return 4 * h * w + 16 + 32
+
+
+def intermediate_bytes(L, fold):
+    """The correlation's one large intermediate per pattern: the spectrum after the transforms
+    along b and c, (n/2 + 1) L nc complex64 (the reference's T is the same size), which the GEMM
+    writes and the transform along a reads."""
+    n = 16
+    while n < 2 * L - 1:
+        n *= 2
+    return (n // 2 + 1) * L * (n // fold) * 8
 
 
 def mask_count(h, w):
@@ -150,34 +169,61 @@ def chunks_of(n, seed, lo=1000, hi=8000):
     return out
 
 
-# ── the CPU baseline: the oracle's float64 restatement, one thread ──
+# ── the CPU baseline: the C++ / OpenMP float32 restatement of the reference engine ──
 
-def cpu_baseline(pats_host, items, sheets, side, L, iters, budget_s=20.0, max_patterns=64):
-    from threadpoolctl import threadpool_limits
-    from oracle import pipeline as op
-    R = op.bruker_R(70.0, 0.0)
-    fr = op.frame(R, H, W)
-    t0 = op.translation(R, PC, float(H), float(W))
-    f, f2, fold, _ = op.harmonics(sheets, side, L)
-    so = op.So3(L, fold)
-    wa, wst = op.coefficients(None, fr, t0, L, center=False, norm=False, window=True)
-    rden = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])
-    grad = op.gradient(sheets, side)
-    done = 0
-    with threadpool_limits(limits=1):
-        start = time.perf_counter()
-        while done < max_patterns and time.perf_counter() - start < budget_s:
-            pn = op.normalize(pats_host[done].astype(np.float64), fr["mask"])
-            t = items[done, :3].astype(np.float64)
-            alm, _ = op.coefficients(pn, fr, t, L)
-            q, v0, _, _ = so.peak(so.volume(f, alm), rden)
-            qr, ncc, _ = op.refine(grad, side, fr, pn, t, q, iters=iters)
-            op.nmi(sheets, side, fr, pn, t, qr)
-            done += 1
-        el = time.perf_counter() - start
-    return {"value": done / el, "unit": "patterns/s", "cores": 1, "kind": "port",
-            "sample": f"{done} of the same patterns (60x60, L={L}, {iters} iterations) through oracle/ (float64 numpy, "
-                      f"one thread) in {el:.1f} s; tables built beforehand, as on the GPU"}
+def cpu_baseline(e, pats_host, items, gpu_rec, truth, L, iters, slice_n=10_000, one_core_n=400, cfg1_n=2_500):
+    """The path on the host cores (oracle/cpu/zbcpu.cpp: float32, the reference's algorithm and
+    loop order, one OpenMP thread a pattern), timed on: the first `slice_n` of the workload's
+    patterns at its settings on every OpenMP thread, the first `one_core_n` on one thread, and
+    the CPU-runnable form (the first `cfg1_n` patterns at L = 32 with 8 iterations) on every
+    thread. Its records are compared with the GPU's on the same patterns and with the truth."""
+    from oracle import cpu_port
+    from zebsdsolve_amd import sym
+    from zebsdsolve_amd.geometry import frame_of
+    _, (sheets_d, grad_d, _) = e.baked(0)
+    side = e.phases[0].master.side
+    sheets = sheets_d.cpu().numpy()
+    grad = grad_d.cpu().numpy()
+    fr = frame_of(e.geo, H, W, True, 1.0)
+    t0 = e.translation()
+    rots = e.phases[0].master.rotations
+
+    def port_at(Lb, it):
+        b = e.band(Lb).host
+        p = e.phase_band(0, Lb).host
+        return cpu_port.Port(b, p, fr, sheets, grad, side, iters=it)
+
+    port = port_at(L, iters)
+    rden = port.rden(t0)
+    threads = cpu_port.threads()
+    n1 = min(slice_n, len(items))
+    start = time.perf_counter()
+    rec, _ = port.index(pats_host[:n1], items[:n1], rden)
+    el_all = time.perf_counter() - start
+    n2 = min(one_core_n, len(items))
+    start = time.perf_counter()
+    port.index(pats_host[:n2], items[:n2], rden, threads=1)
+    el_one = time.perf_counter() - start
+    port1 = port_at(32, 8)
+    rden1 = port1.rden(t0)
+    n3 = min(cfg1_n, len(items))
+    start = time.perf_counter()
+    rec1, _ = port1.index(pats_host[:n3], items[:n3], rden1)
+    el_1 = time.perf_counter() - start
+    d_gpu = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), gpu_rec[:n1, :4].astype(np.float64), rots))
+    d_true = np.degrees(sym.disorientation(rec[:, :4].astype(np.float64), truth[:n1], rots))
+    d1 = np.degrees(sym.disorientation(rec1[:, :4].astype(np.float64), truth[:n3], rots))
+    return {"value": round(n1 / el_all, 1), "unit": "patterns/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement of the reference engine (not the reference binary)",
+            "sample": f"the first {n1} of the same patterns ({H}x{W}, L={L}, {iters} iterations) through oracle/cpu/zbcpu.cpp "
+                      f"(float32 C++17 / OpenMP, one pattern a thread) on {threads} threads in {el_all:.1f} s; the window's "
+                      f"denominators and the tables built beforehand, as on the GPU",
+            "one_core": {"value": round(n2 / el_one, 1), "patterns": n2, "seconds": round(el_one, 2)},
+            "cpu_runnable_form": {"value": round(n3 / el_1, 1), "patterns": n3, "bandwidth": 32, "refine_iters": 8, "cores": threads,
+                                  "seconds": round(el_1, 2), "median_misorientation_deg": round(float(np.median(d1)), 4)},
+            "agreement": {"median_deg_to_gpu": round(float(np.median(d_gpu)), 5), "p99_deg_to_gpu": round(float(np.percentile(d_gpu, 99)), 4),
+                          "ncc_max_abs_diff_to_gpu": round(float(np.max(np.abs(rec[:, 4] - gpu_rec[:n1, 4]))), 6),
+                          "median_deg_to_truth": round(float(np.median(d_true)), 4)}}
 
 
 def traffic_from_profiles(kernel, patterns_per_launch):
@@ -231,6 +277,7 @@ def main():
     ap.add_argument("--pixel", choices=["f32", "u16"], default="f32",
                     help="the patterns' stored type (u16: the Poisson counts as detector integers)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--full-b", action="store_true", help="search every row of b (correlation_symmetry off)")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device three times (the known-size kernel the HBM counters are scaled by)")
@@ -264,7 +311,8 @@ def main():
     two = args.phases == "ni,gan"
     if two:
         e.add_phase(GAN_MASTER)
-    e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters, correlation_precision=args.precision)
+    e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters, correlation_precision=args.precision,
+                     correlation_symmetry=not args.full_b)
     e.correlator = args.correlator
     e.set_patterns(np.zeros((1, 1, H, W), np.float32))
     n = args.patterns
@@ -357,49 +405,63 @@ def main():
             dis[sel] = np.degrees(sym.disorientation(rec[sel, :4].astype(np.float64), truth[sel], e.phases[pi].master.rotations))
 
     n_mask = mask_count(H, W)
-    fpp, fpp_total, bpp = {}, 0.0, {}
-    for ph in phs:  # every pattern is correlated and refined against every phase
-        f1, t1 = flops_per_pattern(args.bandwidth, ph.host.fold, ph.host.hm.rules.inversion, args.iters, n_mask)
+    fpp, fpp_total, inter = {}, 0.0, 0.0
+    for pi, ph in enumerate(phs):  # every pattern is correlated and refined against every phase
+        half = bool(e.use_mfma(args.bandwidth, pi) and e.corr_vtable(ph).b_half)
+        f1, t1 = flops_per_pattern(args.bandwidth, ph.host.fold, ph.host.hm.rules.inversion, args.iters, n_mask, half)
         for k, v in f1.items():
             fpp[k] = fpp.get(k, 0.0) + v
         fpp_total += t1
-        for k, v in bytes_per_pattern(args.bandwidth, ph.host.fold, H, W).items():
-            bpp[k] = bpp.get(k, 0.0) + v
+        inter += intermediate_bytes(args.bandwidth, ph.host.fold) * (((osh_n(args.bandwidth) // 4 + 2) / (osh_n(args.bandwidth) // 2 + 1))
+                                                                      if half else 1.0)
+    lp = args.precision != "fp32"
     roofline = None
     kernels = {}
     if prof:
         for k, (cnt, ms) in prof.items():
             kernels[k] = {"launches": cnt, "ms_total": round(ms, 3)}
+
         def roof(k):
+            # algorithmic FLOP per launch (the reference's algorithm, its zero structure
+            # exploited) over the launch's average duration, against the peak of the unit the
+            # kernel's arithmetic runs on
             cnt, ms = prof[k]
             per_launch_ms = ms / cnt
             ppl = args.steps * n / cnt
             sec = per_launch_ms * 1e-3
             tf = fpp.get(k, 0.0) * ppl / sec / 1e12
-            gbs = bpp.get(k, 0.0) * ppl / sec / 1e9
-            # the roof that binds: the larger fraction of its peak (FP32 compute 157.3 TFLOP/s,
-            # matrix cores or VALU; HBM 8 TB/s)
-            if gbs / HBM_PEAK_GBS > tf / FP32_PEAK_TFLOPS:
-                r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            if k in MFMA_KERNELS:
+                peak, unit = (LP_PEAK_TFLOPS, "mfma-16bit") if (lp and k == "k_vgemm") else (FP32_PEAK_TFLOPS, "mfma")
             else:
-                r = {"bound": "mfma", "achieved": round(tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4)}
-            r.update({"traffic": traffic_from_profiles(k, ppl), "kernel": k, "launch_ms": round(per_launch_ms, 4),
-                      "patterns_per_launch": round(ppl, 1), "tflops": round(tf, 3), "gbs": round(gbs, 1)})
-            return r
+                peak, unit = VALU_PEAK_TFLOPS, "valu"
+            return {"bound": "mfma", "unit_kind": unit, "achieved": round(tf, 3), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(tf / peak, 4), "traffic": traffic_from_profiles(k, ppl), "kernel": k,
+                    "launch_ms": round(per_launch_ms, 4), "patterns_per_launch": round(ppl, 1)}
 
         dom = max((k for k in prof if k in fpp), key=lambda k: prof[k][1])
         roofline = roof(dom)
+        # the dominant kernel's bound key is the contract's "mfma" (FLOP-bound on the unit it
+        # uses: the matrix cores for the GEMM); a vector-unit kernel says so in unit_kind
         for k in prof:
             if k in fpp:
-                kernels[k]["roofline"] = {x: roof(k)[x] for x in ("bound", "frac", "tflops", "gbs")}
+                r = roof(k)
+                kernels[k]["roofline"] = {"unit": r["unit_kind"], "frac": r["frac"], "tflops": r["achieved"], "peak": r["peak"]}
+        # the intermediate V between the GEMM and the transform along a, per pattern
+        for k in ("k_vgemm", "k_apeak"):
+            if k in prof:
+                cnt, ms = prof[k]
+                vb = inter * (0.5 if lp else 1.0)
+                kernels[k]["intermediate"] = {"bytes_per_pattern": round(vb), "gbs": round(vb * args.steps * n / (ms * 1e-3) / 1e9, 1)}
         total_alg = fpp_total * args.steps * n
         kernels["_all_algorithmic_TFLOPs"] = round(total_alg / el / 1e12, 3)
+        kernels["_all_frac_of_fp32_matrix_peak"] = round(total_alg / el / 1e12 / FP32_PEAK_TFLOPS, 4)
+    comp = compulsory_bytes(H, W)
+    hbm_comp = {"bytes_per_pattern": comp, "gbs": round(comp * value / 1e9, 2), "frac": round(comp * value / 1e9 / HBM_PEAK_GBS, 5)}
 
     cpu = None
-    if not args.no_cpu and not two:
-        _, (_, _, host_sheets) = e.baked(0)
-        sample = pats[:64].cpu().numpy()
-        cpu = cpu_baseline(sample, items[:64], host_sheets, e.phases[0].master.side, args.bandwidth, args.iters)
+    if not args.no_cpu and not two and not args.from_host and args.pixel == "f32" and H == 60:
+        k = min(10_000, n)
+        cpu = cpu_baseline(e, pats[:k].cpu().numpy(), items[:k], rec[:k], truth[:k], args.bandwidth, args.iters)
 
     out = {
         "metric": "indexed patterns/s at <=0.1 deg median misorientation; % of HBM/MFMA roofline",
@@ -420,6 +482,7 @@ def main():
                                f"master{'' if not two else 's'}), L={args.bandwidth}, {args.iters} Gauss-Newton iterations, chunks U[{clo}, {chi}]",
                    "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks),
                    "correlator": "mfma" if e.use_mfma(args.bandwidth) else ("slab" if not e.use_mfma(args.bandwidth, 0) else "mfma+slab"),
+                   "b_half": [bool(e.use_mfma(args.bandwidth, pi) and e.corr_vtable(ph).b_half) for pi, ph in enumerate(phs)],
                    "streams": max(args.lanes, 1), "precision": args.precision, "phases": args.phases, "graphs": bool(args.graphs),
                    "input": "pinned host memory, uploaded per chunk in the timed region" if args.from_host else "resident in HBM",
                    "pixel": args.pixel},
@@ -427,6 +490,7 @@ def main():
                      "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1),
                      "phase_right": round(float(np.mean(right)), 5)},
         "roofline": roofline,
+        "hbm_compulsory": hbm_comp,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
@@ -478,7 +542,8 @@ def bench_fit(args, e, pats, items, truth):
         cnt, ms = prof["k_sums6"]
         ppl = args.steps * n * (args.fit_iters + 1) / cnt
         tf = 120.0 * n_mask * ppl / (ms / cnt * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": round(tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+        roofline = {"bound": "mfma", "unit_kind": "valu", "achieved": round(tf, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tf / VALU_PEAK_TFLOPS, 4),
                     "traffic": traffic_from_profiles("k_sums6", ppl), "kernel": "k_sums6", "launch_ms": round(ms / cnt, 4),
                     "patterns_per_launch": round(ppl, 1)}
     out = {

@@ -103,6 +103,13 @@ typedef struct zb_vtable {
     int32_t lp_kind;  /* 0 FP32 products, 1 FP16, 2 BF16 */
     int32_t lp_split; /* first slot (a multiple of 8) of the groups with hi + lo products */
     const void* qlp;  /* the 16-bit table (lp_kind != 0), 2-byte values */
+    /* 1: the master has a 2-fold axis along y, so its correlation obeys C(a, b, c) =
+     * C(a + pi, pi - b, pi - c) and every orientation has a representative with b <= pi / 2:
+     * zb_vcorr_correlate searches the rows j <= n / 4 only (and computes one more, the peak's
+     * neighbour along b). Ties keep the lowest index, as over the whole range: a cell past
+     * n / 4 has its partner at a lower j. 0: every row j <= n / 2. zb_vcorr_denominators ignores
+     * it (the window has no such symmetry). */
+    int32_t b_half;
 } zb_vtable;
 
 /* The tiling of zb_vtable.q for nc points along c (up to 128) and `slots` degree slots (up to

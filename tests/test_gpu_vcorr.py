@@ -166,3 +166,64 @@ def test_vcorr_matches_slab_path_over_many_tiles(L, N):
     # the same orientation, except where two cells score within rounding of each other
     d = np.degrees(sym.disorientation(a[:, :4].astype(np.float64), b[:, :4].astype(np.float64), e.phases[0].master.rotations))
     assert np.mean(d < 0.01) > 0.98, np.sort(d)[-5:]
+
+
+@pytest.mark.parametrize("L,precision", [(48, "fp32"), (64, "fp32"), (64, "fp16"), (96, "fp32")])
+def test_half_b_search_matches_the_whole_range(L, precision):
+    """Nickel has a 2-fold axis along y, so its correlation obeys C(a, b, c) = C(a + pi, pi - b,
+    pi - c): searching j <= n / 4 (b_half) finds, for every pattern, the whole range's peak or
+    its partner under that axis. On 48 simulated patterns (some of them with their peak past
+    b = pi / 2 over the whole range): the same orientation up to the master's symmetry
+    within 1/200 of a grid step, the same score to float32 rounding, every peak at b <= pi / 2
+    plus half a step; and the oracle's peak, as for the whole range.""" 
+    import torch
+    from zebsdsolve_amd import _native as nat, sym
+    from zebsdsolve_amd.engine import stream_ptr
+    print(f"half b L={L} {precision}", flush=True)
+    N = 48
+    e, op, host, side, pats, truth, fr, t = _setup(NI, L, N)
+    e.settings.apply(correlation_precision=precision)
+    lib = nat.load()
+    ph = e.phase_band(0, L)
+    band = e.band(L)
+    assert ph.y2
+    alm, pats_n = _coefficients(e, L, pats)
+    out = {}
+    for half in (0, 1):
+        e.settings.apply(correlation_symmetry=bool(half))
+        vt = e.corr_vtable(ph)
+        assert vt.b_half == half
+        pk = torch.empty((N, 8), dtype=torch.float32, device="cuda")
+        ws = torch.empty((lib.zb_vcorr_workspace(vt, N, 0),), dtype=torch.uint8, device="cuda")
+        nat.check(lib.zb_vcorr_correlate(vt, band.tw.data_ptr(), alm.data_ptr(), N, e.vrden(0, L).data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                         pk.data_ptr(), stream_ptr()), "vcorr")
+        out[half] = pk.cpu().numpy()
+    full, half = out[0], out[1]
+    rots = e.phases[0].master.rotations
+    n = band.host.n
+    step = 360.0 / n
+    # over the whole range the lowest index wins a tie, so most peaks already sit at b <= pi / 2;
+    # those the whole range puts past it (its float32 rounding favoured the partner) are here
+    past = full[:, 6] > np.pi / 2 + 0.5 * 2 * np.pi / n
+    if precision == "fp32":
+        assert past.sum() >= 2, past.sum()
+    assert (half[:, 6] <= np.pi / 2 + 0.5 * 2 * np.pi / n + 1e-6).all()
+    d = np.degrees(sym.disorientation(half[:, :4].astype(np.float64), full[:, :4].astype(np.float64), rots))
+    if precision == "fp32":
+        assert d.max() < 0.005 * step, d.max()
+        assert np.all(np.abs(half[:, 4] - full[:, 4]) <= 2e-5 * np.abs(full[:, 4])), np.abs(half[:, 4] - full[:, 4]).max()
+    else:
+        # a partner cell's score comes from other rows of the 16-bit V (rounded apart by up to
+        # ~1e-4), which can reorder a near-tie
+        assert np.mean(d < 0.005 * step) >= 0.95, d
+        assert np.all(np.abs(half[:, 4] - full[:, 4]) <= 3e-4 * np.abs(full[:, 4])), np.abs(half[:, 4] - full[:, 4]).max()
+    if precision == "fp32" and L <= 64:
+        f, f2, fold, _ = op.harmonics(host, side, L)
+        so = op.So3(L, fold)
+        wa, wst = op.coefficients(None, fr, t, L, center=False, norm=False, window=True)
+        rden_ref = so.rden(so.volume(f, wa), so.volume(f2, wa), wst[0])
+        for b in np.flatnonzero(past)[:3]:
+            ref_alm, _ = op.coefficients(pats_n[b].astype(np.float64), fr, t, L)
+            q, v0, _, _ = so.peak(so.volume(f, ref_alm), rden_ref)
+            assert abs(half[b, 4] - v0) < 2e-4 * abs(v0), (b, half[b, 4], v0)
+            assert np.degrees(sym.disorientation(half[b, :4].astype(np.float64), q, rots)) < 0.01 * step

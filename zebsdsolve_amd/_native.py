@@ -30,7 +30,7 @@ class Phase(ctypes.Structure):
 
 class VTable(ctypes.Structure):
     _fields_ = [("L", i32), ("n", i32), ("nc", i32), ("l_step", i32), ("slots", i32), ("cols", i32), ("q", vp),
-                ("lp_kind", i32), ("lp_split", i32), ("qlp", vp)]
+                ("lp_kind", i32), ("lp_split", i32), ("qlp", vp), ("b_half", i32)]
 
 
 class RefineOpts(ctypes.Structure):

@@ -74,6 +74,9 @@ constexpr int kItemPad = 128;
 
 struct VArg {
     int L, n, log2n, nc, log2nc, nb, slots, cols, l_step, pairs, tiles, n_items, ni_pad;
+    // rows of b the GEMM computes (jg) and the peak searches (jp): all n/2 + 1, or with b_half
+    // j <= n/4 searched and one row more computed for the peak's neighbour along b
+    int jg, jp;
     int lp_gs;  // 16-bit products: the first group of 8 slots that takes the hi + lo product
     int vk;     // V's elements: 0 FP32 pairs (cols floats a row), 1 FP16 / 2 BF16 pairs (nc a row)
 };
@@ -195,7 +198,7 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
     // never wait for the stores of j to read their registers.
     auto step = [&](int j, Acc& acc, Acc& other, const floatx4* cur, floatx4* nxt) {
         // the last j stages its own rows again: no branch
-        stage_load(min(j + 1, a.nb - 1));
+        stage_load(min(j + 1, a.jg - 1));
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
@@ -255,9 +258,9 @@ __device__ __forceinline__ void vgemm_run(const VArg& a, int mp, int tile, int c
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc1[mb][nb][0][r] = 0.f;
     // an odd count repeats the last j (the same values to the same place)
-    for (int j = 0; j < a.nb; j += 2) {
+    for (int j = 0; j < a.jg; j += 2) {
         step(j, acc0, acc1, lds0, lds1);
-        step(min(j + 1, a.nb - 1), acc1, acc0, lds1, lds0);
+        step(min(j + 1, a.jg - 1), acc1, acc0, lds1, lds0);
     }
 }
 
@@ -389,7 +392,7 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
     for (int mb = 0; mb < MB; ++mb) loff[mb] = 4u * (unsigned)(((tile * MB + mb) * 32 + half * 4) * w + cg * 32 + r32);
     typedef floatx16 Acc[MB][NBLK][1];
     auto step = [&](int j, Acc& acc, Acc& other, const floatx4* cur, floatx4* nxt) {
-        stage_load(min(j + 1, a.nb - 1));
+        stage_load(min(j + 1, a.jg - 1));
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
@@ -451,9 +454,9 @@ __device__ __forceinline__ void vgemm16_run(const VArg& a, int mp, int tile, int
         for (int nb = 0; nb < NBLK; ++nb)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc1[mb][nb][0][r] = 0.f;
-    for (int j = 0; j < a.nb; j += 2) {
+    for (int j = 0; j < a.jg; j += 2) {
         step(j, acc0, acc1, lds0, lds1);
-        step(min(j + 1, a.nb - 1), acc1, acc0, lds1, lds0);
+        step(min(j + 1, a.jg - 1), acc1, acc0, lds1, lds0);
     }
 }
 
@@ -994,6 +997,12 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
     a.nc = t->nc;
     a.log2nc = ilog2v(t->nc);
     a.nb = t->n / 2 + 1;
+    if (t->b_half < 0 || t->b_half > 1) {
+        set_error("%s: b_half %d (0 or 1)", who, t->b_half);
+        return ZB_EINVAL;
+    }
+    a.jp = t->b_half ? t->n / 4 + 1 : a.nb;
+    a.jg = t->b_half ? t->n / 4 + 2 : a.nb;
     a.slots = t->slots;
     a.cols = t->cols;
     a.l_step = t->l_step;
@@ -1016,7 +1025,7 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
 
 static int64_t al256(int64_t v) { return (v + 255) / 256 * 256; }
 // V for the padded items: cols floats per item row (complex pairs, then the table's zero columns)
-static int64_t v_bytes(const VArg& a) { return (int64_t)a.ni_pad * a.nb * a.L * (a.vk ? 2 * a.nc : a.cols) * (a.vk ? 2 : 4); }
+static int64_t v_bytes(const VArg& a) { return (int64_t)a.ni_pad * a.jg * a.L * (a.vk ? 2 * a.nc : a.cols) * (a.vk ? 2 : 4); }
 // V and the per-item best; the denominators add two volumes
 static int64_t vcorr_ws(const VArg& a, int volumes) {
     return al256(v_bytes(a)) + al256((int64_t)a.n_items * 8) + (volumes ? 2 * al256((int64_t)a.n_items * a.nb * a.n * a.nc * 4) : 0);
@@ -1029,6 +1038,7 @@ extern "C" int64_t zb_vcorr_workspace(const zb_vtable* t, int32_t n_items, int32
         return ZB_EINVAL;
     }
     if (v_arg(t, n_items, a, "zb_vcorr_workspace")) return ZB_EINVAL;
+    if (denominators) a.jg = a.jp = a.nb;  // the window's volumes take every row of b
     return vcorr_ws(a, denominators != 0);
 }
 
@@ -1148,7 +1158,7 @@ static int launch_apeak(const VArg& a, const void* v, const float* rden, int rde
     if (a.n == 256) {
         // two lanes per column: one wave per (j, 32 columns, item), four items a block
         const bool onevol = MODE == 0 && (rden_div >= a.n_items || rden_div % 4 == 0);
-        const dim3 grid((unsigned)((long)a.nb * (a.nc / 32) * ((a.n_items + 3) / 4)));
+        const dim3 grid((unsigned)((long)a.jp * (a.nc / 32) * ((a.n_items + 3) / 4)));
         const size_t lds = onevol ? (size_t)32 * 256 * sizeof(float) : 0;
         if (onevol)
             ZB_LAUNCH("k_apeak", (k_apeak128<MODE, true, VT>), grid, dim3(256), lds, s, a, v, rden, rden_div, best, (float2*)vol);
@@ -1160,7 +1170,7 @@ static int launch_apeak(const VArg& a, const void* v, const float* rden, int rde
     // up to four waves a block, at most 32 slabs (ni_pad is a multiple of 32)
     const int waves = per_wave >= 32 ? 1 : (32 / per_wave < 4 ? 32 / per_wave : 4);
     const int per_block = per_wave * waves;
-    const long slabs = (long)a.ni_pad * a.nb;
+    const long slabs = (long)a.ni_pad * a.jp;
     const dim3 grid((unsigned)(slabs / per_block));
     const int H = a.n / 2;
     // blocks start at multiples of per_block, so volumes of a multiple of per_block items never
@@ -1226,6 +1236,8 @@ extern "C" int32_t zb_vcorr_denominators(const zb_vtable* t_f, const zb_vtable* 
     if (int rc = v_arg(t_f, n_windows, a, "zb_vcorr_denominators")) return rc;
     if (int rc = v_arg(t_f2, n_windows, a2, "zb_vcorr_denominators")) return rc;
     a.vk = a2.vk = 0;  // the window's volumes from the FP32 tables q
+    a.jg = a.jp = a.nb;  // and over every row of b: the window has no symmetry
+    a2.jg = a2.jp = a2.nb;
     if (a.L != a2.L || a.n != a2.n || a.nc != a2.nc || a.l_step != a2.l_step || n_windows < 0 ||
         (n_windows > 0 && (!w_alm || !w_stats || !work || !rden))) {
         set_error("zb_vcorr_denominators: bad arguments");

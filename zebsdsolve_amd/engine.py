@@ -63,8 +63,11 @@ class DevicePhase:
     """A phase's weight tables at a bandwidth (for one baked master), and its denominator volume
     per detector geometry."""
 
-    def __init__(self, band, side, sheets, dev):
+    def __init__(self, band, side, sheets, dev, rotations=()):
         self.host = tables_mod.PhaseHost(band.host, side, sheets)
+        # a 2-fold axis along y among the master's rotations: its correlation repeats past
+        # b = pi / 2 (C(a, b, c) = C(a + pi, pi - b, pi - c)), so the search can stop there
+        self.y2 = any(abs(r[0]) < 1e-4 and abs(abs(r[2]) - 1) < 1e-4 for r in np.asarray(rotations, np.float64).reshape(-1, 4))
         h = self.host
         self.w_f = _dev(h.w_f, dev)
         self.w_f2 = _dev(h.w_f2, dev)
@@ -369,7 +372,7 @@ class Engine:
         p = self.phases[pi]
         k = (L, key)
         if k not in p.bands:
-            p.bands[k] = DevicePhase(self.band(L), p.master.side, host, self._device())
+            p.bands[k] = DevicePhase(self.band(L), p.master.side, host, self._device(), p.master.rotations)
         return p.bands[k]
 
     def _window(self, band):
@@ -454,8 +457,14 @@ class Engine:
         # columns: otherwise the FP32 form
         slots = (ph.host.hm.L + ph.host.l_step - 1) // ph.host.l_step
         if kind == 0 or not (32 <= ph.host.nc <= 64 and slots <= 64):
-            return ph.vtable_of(0)[0]
-        return ph.vtable16(kind, int(self.settings["split_degree"]))
+            vt = ph.vtable_of(0)[0]
+        else:
+            vt = ph.vtable16(kind, int(self.settings["split_degree"]))
+        # b searched to pi / 2 only where the master's 2-fold axis along y repeats the rest (a
+        # copy of the struct: the phase's own tables stay as built)
+        vt = nat.VTable.from_buffer_copy(vt)
+        vt.b_half = 1 if (self.settings["correlation_symmetry"] and ph.y2) else 0
+        return vt
 
     def use_mfma(self, L, pi=None):
         """The matrix-core correlation for phase pi at L (every phase when pi is None)."""

@@ -37,6 +37,10 @@ DEFAULTS = {
     # degrees from split_degree on) and the GEMM's output kept in the same 16-bit type, the
     # transform along a in float32; phases with fewer than 32 points along c stay "fp32" ──
     "correlation_precision": "fp32", "split_degree": 64,
+    # ── the matrix-core correlation searches b only up to pi / 2 when the master has a 2-fold
+    # axis along y (its correlation repeats: C(a, b, c) = C(a + pi, pi - b, pi - c)); this
+    # build's own, the orientations found are the same up to the master's symmetry ──
+    "correlation_symmetry": True,
     # ── how the work is cut up ──
     "batch": 256, "tile_mb": 256, "refine_pixels": 64, "wg_fft": 64, "wg_refine": 64,
     # ── sweeps and acceptance ──
@@ -83,6 +87,7 @@ def check(s):
     bad(not (8 <= s["sweep_bandwidth"] <= 256), "sweep_bandwidth")
     bad(s["correlation_precision"] not in ("fp32", "fp16", "bf16"), "correlation_precision")
     bad(not (0 <= s["split_degree"] <= 256), "split_degree")
+    bad(not isinstance(s["correlation_symmetry"], bool), "correlation_symmetry")
     for k in ("wg_fft", "wg_refine"):
         bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
     for k, allowed in _ONLY.items():
