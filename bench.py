@@ -267,6 +267,8 @@ def main():
                     help="the differentiation form instead: every pattern's pose (crystal 0.5-2 deg off, pattern centre up to 0.02 "
                          "off) fitted over the six directions by --fit-iters Levenberg-Marquardt steps (use with --size 80)")
     ap.add_argument("--fit-iters", type=int, default=20)
+    ap.add_argument("--fit-tol", type=float, default=1e-5,
+                    help="a pose has converged when its last step is below this (pixels and radians)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay each chunk size's batch as a HIP graph captured once (Engine.graph; pays off with equal chunks)")
     ap.add_argument("--lanes", type=int, default=1, help="streams the chunks are spread over (1: one stream)")
@@ -278,6 +280,7 @@ def main():
                     help="the patterns' stored type (u16: the Poisson counts as detector integers)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--full-b", action="store_true", help="search every row of b (correlation_symmetry off)")
+    ap.add_argument("--no-refine-order", action="store_true", help="refine in batch order (refine_order off)")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device three times (the known-size kernel the HBM counters are scaled by)")
@@ -312,7 +315,7 @@ def main():
     if two:
         e.add_phase(GAN_MASTER)
     e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters, correlation_precision=args.precision,
-                     correlation_symmetry=not args.full_b)
+                     correlation_symmetry=not args.full_b, refine_order=not args.no_refine_order)
     e.correlator = args.correlator
     e.set_patterns(np.zeros((1, 1, H, W), np.float32))
     n = args.patterns
@@ -518,13 +521,13 @@ def bench_fit(args, e, pats, items, truth):
     field = fit.PoseField(e, 0, norm)
     p0 = torch.from_numpy(poses0).cuda()
     for _ in range(args.warmup):
-        fit.fit6(field, p0, iters=args.fit_iters, with_ncc=False)
+        fit.fit6(field, p0, iters=args.fit_iters, tol=args.fit_tol, with_ncc=False)
     torch.cuda.synchronize()
     if not args.no_profile:
         nat.profile(True)
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        poses, _, state = fit.fit6(field, p0, iters=args.fit_iters, with_ncc=False)
+        poses, _, state = fit.fit6(field, p0, iters=args.fit_iters, tol=args.fit_tol, with_ncc=False)
     torch.cuda.synchronize()
     el = time.perf_counter() - t_start
     prof = nat.profile_report() if not args.no_profile else {}
@@ -553,7 +556,7 @@ def bench_fit(args, e, pats, items, truth):
         "dtype": "f32 sums, f64 steps", "data": "synthetic: Ni master projected on the GPU at seeded random orientations, Poisson 400 counts; "
                                                "starts 0.5-2 deg off (uniform angle and axis), pattern centre U(+-0.02) off",
         "config": {"workload": f"{n} patterns {H}x{W} f32, Ni, {args.fit_iters} Levenberg-Marquardt steps over the six directions",
-                   "patterns": n, "fit_iters": args.fit_iters},
+                   "patterns": n, "fit_iters": args.fit_iters, "fit_tol": args.fit_tol},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
                      "median_position_px": round(float(np.median(dpos)), 4), "converged": round(float(np.mean(state.cpu().numpy() == 1)), 4),
                      "meets_0.1deg": bool(np.median(dis) <= 0.1)},

@@ -130,6 +130,10 @@ typedef struct zb_refine_opts {
     float scale_max;
     int32_t nmi_bins; /* histogram levels of the normalized mutual information, 2..32 */
     uint32_t stamp;   /* written into each record's last slot */
+    /* a permutation of the n_items items (device memory, zb_refine_order's), or NULL: with it the
+     * items are taken in that order, each XCD a contiguous run of it, so that items refined side
+     * by side look up nearby regions of the master; the records are the same either way */
+    const int32_t* order;
 } zb_refine_opts;
 
 /* Library version string and the last error of this thread (length written, without NUL). */
@@ -223,6 +227,14 @@ int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, co
                   const float* items, const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* opts,
                   float* records, int32_t out0, void* stream);
 
+/* The refinement's item order for zb_refine_opts.order: the n_items items sorted by the master
+ * region their detector's centre ray lands in at their seed orientation (seeds[b * seed_stride
+ * ..], as zb_refine takes them; items: [n][4] (t, pattern index)), as cells of 32 x 32 a Lambert
+ * sheet along a Morton curve; order: [n_items] out. One block of 1 024 threads. No counterpart in
+ * the reference, whose refinement dispatches take items in batch order. */
+int32_t zb_refine_order(const zb_frame* frame, const float* items, const float* seeds, int32_t seed_stride, int32_t n_items, int32_t* order,
+                        void* stream);
+
 /* Flag of zb_sums: zero the sums of the shift directions (orientation refinement's view). */
 #define ZB_SUMS_ROTATION_ONLY 2
 
@@ -253,6 +265,20 @@ int32_t zb_fit6_propose(const float* poses, const float* sums, const double* lam
                         double max_shift, float* trial, double* cur, int32_t* state, void* stream);
 int32_t zb_fit6_accept(const float* trial, const float* trial_sums, int32_t n, double tol, const double* cur, float* poses, float* sums,
                        double* lambda, int32_t* state, void* stream);
+
+/* The same fit from each item's local model instead of its sums, so that the gradient can come
+ * from elsewhere (zebsdsolve_amd/fit.py drives it with the gradient of its autograd function):
+ * local [n][29] f64 = (NCC, ok (1.0 or 0.0), g[6], the Gauss-Newton matrix's upper triangle row
+ * by row [21]) over the six directions of zb_sums. zb_local6 writes it from the sums
+ * (planefit.zig:177-201 localOf; ok 0 and the rest 0 where the master is flat under the mask);
+ * zb_fit6_step proposes the damped step from it as zb_fit6_propose does from the sums
+ * (planefit.zig:203-247 damped, solve); zb_fit6_keep keeps a trial whose local NCC rose (pose
+ * and local copied, lambda / 10), else the old pose (lambda x 10), as zb_fit6_accept. */
+int32_t zb_local6(const float* sums, int32_t n, double* local, void* stream);
+int32_t zb_fit6_step(const float* poses, const double* local, const double* lambda, int32_t n, int32_t unknowns, double max_rot, double max_shift,
+                     float* trial, double* cur, int32_t* state, void* stream);
+int32_t zb_fit6_keep(const float* trial, const double* trial_local, int32_t n, double tol, const double* cur, float* poses, double* local,
+                     double* lambda, int32_t* state, void* stream);
 
 /* The pattern filters before the path. background: [h][w] f32 subtracted from every pattern
  * (the static background, e.g. the mean of patterns over the map), or NULL; dynamic: 0 none,

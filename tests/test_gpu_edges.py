@@ -111,3 +111,46 @@ def test_index_cells_over_two_streams_matches_one():
     assert len(s._e._lanes) >= 2
     for k in ("quaternions", "ncc", "nmi", "step"):
         assert np.array_equal(one[k], two[k]), (k, np.abs(one[k] - two[k]).max())
+
+
+def test_refinement_order_is_a_permutation_and_leaves_the_records_alone():
+    """zb_refine_order sorts a batch's items by the master region they look up; zb_refine with
+    that order gives every item the record it gets in batch order (bit for bit), and the order is
+    a permutation of the items grouped by that region."""
+    import torch
+    from zebsdsolve_amd import _native as nat
+    from zebsdsolve_amd.engine import stream_ptr
+    s = _solver()
+    s.synthesize(toy_rows=40, toy_cols=60, toy_grains=2400, toy_counts=400, toy_seed=13)
+    n = 40 * 60
+    s.configure(refine_order=False)
+    plain = s.index_cells(np.arange(n))
+    s.configure(refine_order=True)
+    ordered = s.index_cells(np.arange(n))
+    for k in ("quaternions", "ncc", "nmi", "step"):
+        assert np.array_equal(plain[k], ordered[k]), k
+    # the permutation itself, on random seeds
+    e = s._e
+    lib = nat.load()
+    rng = np.random.default_rng(3)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    seeds = torch.from_numpy(np.concatenate([q, np.zeros((n, 4))], 1).astype(np.float32)).cuda()
+    items = torch.from_numpy(np.concatenate([e.positions(np.arange(n)), np.arange(n, dtype=np.float32)[:, None]], 1)).cuda()
+    order = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    nat.check(lib.zb_refine_order(e.frame(), items.data_ptr(), seeds.data_ptr(), 8, n, order.data_ptr(), stream_ptr()), "order")
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(n))
+    # neighbours in the order look up nearby master regions: the detector centre's direction at
+    # consecutive items is far closer than at random pairs
+    fr = e.frame()
+    h, w = e.pattern_hw()
+    t = items[:, :3].cpu().numpy().astype(np.float64)
+    u = np.asarray(fr.c0) * (h - 1) / 2 + np.asarray(fr.c1) * (w - 1) / 2 + t
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    # q . u (q acting as q u q*)
+    tt = 2 * np.cross(q[:, 1:], u)
+    v = u + q[:, :1] * tt + np.cross(q[:, 1:], tt)
+    near = np.mean(np.sum(v[o[1:]] * v[o[:-1]], 1))
+    rand = np.mean(np.sum(v[1:] * v[:-1], 1))
+    assert near > 0.95 and near > rand + 0.5, (near, rand)

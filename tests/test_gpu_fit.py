@@ -168,3 +168,57 @@ def test_fit6_to_convergence_from_config4_starts():
     dt = np.abs(got[:, :3] - t[None, :]).max(1)
     print(f"fit6: NCC min {ncc.min():.5f}, orientation max {d.max():.4f} deg, position max {dt.max():.3f} px", flush=True)
     assert ncc.min() > 0.999 and d.max() < 0.02 and dt.max() < 0.05, (ncc, d, dt)
+
+
+def test_ncc_in_pattern_centre_coordinates_against_differences_and_float64(fitcase):
+    """PoseNccPC: the NCC as a function of the crystal rotation (tangent phi at q0) and the
+    pattern centre pc, t(pc) = R (0.5 - pcy h, 0.5 - pcx w, pcz h) (src/geometry.zig:71-75). Its
+    backward pass against central differences of its own forward pass (h = 1e-4 rad, 1e-5 of
+    the pattern centre: the bilinear master is piecewise smooth, so 10 % of the gradient's size)
+    and against the float64 gradient (the oracle's sums at the same pose, carried to phi and pc
+    by the same chain rule written out here: 1e-3 of its size)."""
+    import torch
+    op, fit = fitcase["op"], fitcase["fit"]
+    print("PoseNccPC", flush=True)
+    H = W = 60
+    R = op.bruker_R(70.0, 0.0)
+    poses = fitcase["poses"].astype(np.float64)
+    # the pattern centre of each pose's detector position: t = R v, v = (0.5 - pcy H, 0.5 - pcx W, pcz H)
+    v = poses[:, :3] @ R
+    pc0 = np.stack([(0.5 - v[:, 1]) / W, (0.5 - v[:, 0]) / H, v[:, 2] / H], 1)
+    rng = np.random.default_rng(9)
+    phi0 = rng.normal(size=(N, 3)) * 0.002
+    q0 = poses[:, 4:8]
+    idx = np.arange(N, dtype=np.float64)
+    phi = torch.tensor(phi0, dtype=torch.float64, requires_grad=True)
+    pc = torch.tensor(pc0, dtype=torch.float64, requires_grad=True)
+    ncc = fit.PoseNccPC.apply(phi, pc, fitcase["field"], q0, R, H, W, idx)
+    ncc.sum().backward()
+    g_phi, g_pc = phi.grad.numpy(), pc.grad.numpy()
+
+    def f(ph, c):
+        with torch.no_grad():
+            return fit.PoseNccPC.apply(torch.tensor(ph), torch.tensor(c), fitcase["field"], q0, R, H, W, idx).numpy()
+
+    for i in range(3):
+        e = np.zeros((N, 3))
+        e[:, i] = 1e-4
+        fd = (f(phi0 + e, pc0) - f(phi0 - e, pc0)) / 2e-4
+        assert np.all(np.abs(fd - g_phi[:, i]) < 0.1 * np.abs(g_phi).max(1)), (i, fd, g_phi[:, i])
+        e = np.zeros((N, 3))
+        e[:, i] = 1e-5
+        fd = (f(phi0, pc0 + e) - f(phi0, pc0 - e)) / 2e-5
+        assert np.all(np.abs(fd - g_pc[:, i]) < 0.1 * np.abs(g_pc).max(1)), (i, fd, g_pc[:, i])
+    for b in range(N):
+        pb = fitcase["pats_n"][b].astype(np.float64)
+        t = R @ np.array([0.5 - pc0[b, 1] * H, 0.5 - pc0[b, 0] * W, pc0[b, 2] * H])
+        q = op.qmul(op.qexp(phi0[b]), q0[b])
+        q = q / np.linalg.norm(q)
+        _, n64, g, _ = op.local6(op.sums6(fitcase["grad"], fitcase["side"], fitcase["fr"], pb, t, q))
+        assert abs(ncc[b].item() - n64) < 1e-5
+        J = fit.left_jacobian(phi0[b][None])[0]
+        want_phi = J.T @ g[3:]
+        dt = np.stack([-W * R[:, 1], -H * R[:, 0], H * R[:, 2]])  # d t / d (pcx, pcy, pcz), rows
+        want_pc = dt @ g[:3]
+        assert np.abs(g_phi[b] - want_phi).max() < 1e-3 * np.abs(want_phi).max(), (b, g_phi[b], want_phi)
+        assert np.abs(g_pc[b] - want_pc).max() < 1e-3 * np.abs(want_pc).max(), (b, g_pc[b], want_pc)

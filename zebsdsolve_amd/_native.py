@@ -35,7 +35,7 @@ class VTable(ctypes.Structure):
 
 class RefineOpts(ctypes.Structure):
     _fields_ = [("iters", i32), ("max_step", ctypes.c_float), ("scale0", ctypes.c_float), ("scale_min", ctypes.c_float),
-                ("scale_max", ctypes.c_float), ("nmi_bins", i32), ("stamp", ctypes.c_uint32)]
+                ("scale_max", ctypes.c_float), ("nmi_bins", i32), ("stamp", ctypes.c_uint32), ("order", vp)]
 
 
 class PrepOpts(ctypes.Structure):  # zb_prep_opts
@@ -61,6 +61,10 @@ _SIGS = {
     "zb_vcorr_correlate": ([ctypes.POINTER(VTable), vp, vp, i32, vp, i32, vp, i64, vp, vp], i32),
     "zb_vcorr_denominators": ([ctypes.POINTER(VTable), ctypes.POINTER(VTable), vp, vp, i32, vp, i64, vp, vp], i32),
     "zb_refine": ([vp, vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
+    "zb_refine_order": ([ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),
+    "zb_local6": ([vp, i32, vp, vp], i32),
+    "zb_fit6_step": ([vp, vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp], i32),
+    "zb_fit6_keep": ([vp, vp, i32, ctypes.c_double, vp, vp, vp, vp, vp, vp], i32),
     "zb_sums": ([vp, i32, ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),
     "zb_project": ([vp, i32, ctypes.POINTER(Frame), vp, i32, vp, vp], i32),
     "zb_master_blur": ([vp, vp, i32, ctypes.c_float, vp], i32),

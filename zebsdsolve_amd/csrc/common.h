@@ -33,6 +33,14 @@ bool prof_on();
 void prof_begin(const char* name, hipStream_t s);
 void prof_end(hipStream_t s);
 
+// Blocks are dealt to the 8 XCDs round-robin (block id mod 8); this maps block `id` of `G` to a
+// logical index so that XCD x takes a contiguous run of the logical order (G / 8 blocks, one
+// more when x < G % 8): work adjacent in that order shares the XCD's L2.
+__device__ __forceinline__ int xcd_block(int id, int G) {
+    const int q = G >> 3, r = G & 7, x = id & 7;
+    return x * q + min(x, r) + (id >> 3);
+}
+
 // ── quaternions, (w, x, y, z) in (.x, .y, .z, .w), acting as q v q* ──
 __device__ __forceinline__ float4 qmul(float4 a, float4 b) {
     return make_float4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w,

@@ -124,13 +124,6 @@ struct VEl {
     }
 };
 
-__device__ __forceinline__ int xcd_block(int id, int G) {
-    // blocks are dealt to the 8 XCDs round-robin: give each XCD a contiguous run of the logical
-    // order (XCD x takes G / 8 blocks, one more when x < G % 8), so one XCD's L2 holds the table
-    // rows of a few m'
-    const int q = G >> 3, r = G & 7, x = id & 7;
-    return x * q + min(x, r) + (id >> 3);
-}
 
 // Waves per block: each computes its own tile of 64 items against the same table rows, which
 // the block stages once per j through LDS.

@@ -168,15 +168,18 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
 // MAXP > 0: each lane keeps the master values of its (at most MAXP) pixels at the best
 // orientation from the first NMI pass for the second; MAXP = 0: the second pass looks them up
 // again.
+// With `order` (zb_refine_order's permutation), block i refines item order[xcd_block(i)]: the
+// blocks of one XCD take a contiguous run of items sorted by the master region they look up, so
+// that their gathers share that XCD's L2. Without it, block i refines item i.
 template <int MAXP>
 __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict__ ms, const uint2* __restrict__ mgh, const float* __restrict__ pats,
                                                const float4* __restrict__ items, const float* __restrict__ seeds, int n_items,
-                                               float* __restrict__ records) {
+                                               const int* __restrict__ order, float* __restrict__ records) {
     static_assert(MAXP % kBatch == 0, "MAXP: whole batches");
     extern __shared__ unsigned lds[];
     unsigned* hist = lds;                                               // [nmi_bins][nmi_bins]
     unsigned short* list = (unsigned short*)(lds + a.nmi_bins * a.nmi_bins);  // the masked pixels
-    const int b = blockIdx.x;
+    const int b = order ? order[xcd_block(blockIdx.x, gridDim.x)] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < a.nmi_bins * a.nmi_bins; i += 64) hist[i] = 0u;
     const int n_px = build_list(a, list, lane);
@@ -370,6 +373,69 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
     }
 }
 
+// Cells of the refinement's order: each sheet of the master cut into 32 x 32 Lambert cells
+constexpr int kOrderSide = 32;
+constexpr int kOrderCells = 2 * kOrderSide * kOrderSide;
+
+// The cell the detector centre's ray lands in on the master at orientation q (item position t),
+// numbered by sheet and then along a Morton curve over the cell rows and columns, so that nearby
+// cells get nearby numbers.
+__device__ __forceinline__ int order_cell(const RefArg& a, float4 q, float3 t) {
+    const float fr = 0.5f * (float)(a.h - 1), fc = 0.5f * (float)(a.w - 1);
+    const float3 u = normalize3(make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y,
+                                            a.c0[2] * fr + a.c1[2] * fc + t.z));
+    const float3 v = qrot(q, u);
+    const float2 uv = sphere_to_square(v);
+    const int col = min(max((int)((uv.x + 1.f) * (0.5f * kOrderSide)), 0), kOrderSide - 1);
+    const int row = min(max((int)((uv.y + 1.f) * (0.5f * kOrderSide)), 0), kOrderSide - 1);
+    int m = 0;
+#pragma unroll
+    for (int bit = 0; bit < 5; ++bit) m |= (((col >> bit) & 1) << (2 * bit)) | (((row >> bit) & 1) << (2 * bit + 1));
+    return (v.z < 0.f ? kOrderSide * kOrderSide : 0) + m;
+}
+
+// A counting sort of the items by order_cell, one block: counts per cell in LDS, their exclusive
+// prefix, then each item placed at its cell's next slot (the order within a cell is arbitrary;
+// every item's record is the same whichever block computes it).
+__global__ void __launch_bounds__(1024) k_order(RefArg a, const float4* __restrict__ items, const float* __restrict__ seeds, int seed_stride,
+                                                int n_items, int* __restrict__ order) {
+    __shared__ unsigned cnt[kOrderCells];
+    __shared__ unsigned part[1024];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kOrderCells; i += 1024) cnt[i] = 0u;
+    __syncthreads();
+    auto cell_of = [&](int b) {
+        const float* sd = seeds + (size_t)b * seed_stride;
+        const float4 it = items[b];
+        return order_cell(a, make_float4(sd[0], sd[1], sd[2], sd[3]), make_float3(it.x, it.y, it.z));
+    };
+    for (int b = tid; b < n_items; b += 1024) atomicAdd(&cnt[cell_of(b)], 1u);
+    __syncthreads();
+    // exclusive prefix over the cells: each thread sums its run of cells, a scan over the threads,
+    // then the run's own offsets
+    constexpr int per = kOrderCells / 1024;
+    unsigned run = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) run += cnt[tid * per + k];
+    part[tid] = run;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned x = tid >= o ? part[tid - o] : 0u;
+        __syncthreads();
+        part[tid] += x;
+        __syncthreads();
+    }
+    unsigned at = part[tid] - run;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const unsigned c = cnt[tid * per + k];
+        cnt[tid * per + k] = at;
+        at += c;
+    }
+    __syncthreads();
+    for (int b = tid; b < n_items; b += 1024) order[atomicAdd(&cnt[cell_of(b)], 1u)] = b;
+}
+
 // The 43 sums of the six-direction fit for each item's pose (detector.wgsl:433-494 sums +
 // sums_reduce): count, sum M, sum M^2, sum M e, then for d = (q^-1 grad M / |u|, c x grad M)
 // (a shift of the rays' origin in pixels, a crystal rotation) sum d_i, sum M d_i, sum e d_i and
@@ -522,9 +588,33 @@ extern "C" int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32
     }
     if (n_px <= 64 * 48)
         ZB_LAUNCH("k_refine", (k_refine<48>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
-                  (const float4*)items, seeds, n_items, records);
+                  (const float4*)items, seeds, n_items, (const int*)o->order, records);
     else
         ZB_LAUNCH("k_refine", (k_refine<0>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
-                  (const float4*)items, seeds, n_items, records);
+                  (const float4*)items, seeds, n_items, (const int*)o->order, records);
     return check_launch("zb_refine");
+}
+
+extern "C" int32_t zb_refine_order(const zb_frame* frame, const float* items, const float* seeds, int32_t seed_stride, int32_t n_items,
+                                   int32_t* order, void* stream) {
+    if (!frame || n_items < 0 || seed_stride < 4 || frame->pat_h < 2 || frame->pat_w < 2) {
+        set_error("zb_refine_order: bad arguments");
+        return ZB_EINVAL;
+    }
+    if (n_items == 0) return ZB_OK;
+    if (!items || !seeds || !order) {
+        set_error("zb_refine_order: null pointer");
+        return ZB_EINVAL;
+    }
+    RefArg a = {};
+    for (int i = 0; i < 3; ++i) {
+        a.c0[i] = frame->c0[i];
+        a.c1[i] = frame->c1[i];
+        a.c2[i] = frame->c2[i];
+        a.mask[i] = frame->mask[i];
+    }
+    a.h = frame->pat_h;
+    a.w = frame->pat_w;
+    ZB_LAUNCH("k_order", k_order, dim3(1), dim3(1024), 0, (hipStream_t)stream, a, (const float4*)items, seeds, seed_stride, n_items, order);
+    return check_launch("zb_refine_order");
 }

@@ -600,6 +600,7 @@ class Engine:
             alm = torch.empty((big, pairs, 2), dtype=torch.float32, device=dev)
             stats = torch.empty((big, 4), dtype=torch.float32, device=dev)
             peaks = torch.empty((big, 8), dtype=torch.float32, device=dev)
+            order = torch.empty((big,), dtype=torch.int32, device=dev)
             if workspace is None or workspace.numel() < need:
                 workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
         else:
@@ -609,6 +610,7 @@ class Engine:
             alm = lane.buf("alm", (big, pairs, 2), torch.float32)
             stats = lane.buf("stats", (big, 4), torch.float32)
             peaks = lane.buf("peaks", (big, 8), torch.float32)
+            order = lane.buf("order", (big,), torch.int32)
             workspace = lane.buf("work", (need,), torch.uint8)
             with torch.cuda.stream(lane.stream):
                 records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
@@ -630,6 +632,10 @@ class Engine:
                     else:
                         nat.check(lib.zb_so3_correlate(band.c, ph.c, a_c.data_ptr(), nt, rdens[pi].data_ptr(), nt, workspace.data_ptr(),
                                                        workspace.numel(), p_c.data_ptr(), st), "correlate")
+                if s["refine_order"]:
+                    # items side by side on an XCD look up nearby regions of the master
+                    nat.check(lib.zb_refine_order(fr, it.data_ptr(), peaks.data_ptr(), 8, nbt, order.data_ptr(), st), "refine order")
+                    opts.order = order.data_ptr()
                 nat.check(lib.zb_refine(sheets[pi].data_ptr(), grads_h[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(),
                                         it.data_ptr(), peaks.data_ptr(), 8, nbt, opts, records[pi].data_ptr(), b0, st), "refine")
         return records, opts.stamp

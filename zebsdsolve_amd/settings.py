@@ -30,6 +30,9 @@ DEFAULTS = {
     "bandwidth": 64, "method": "spherical",
     # ── refinement ──
     "refine": True, "refine_iters": 8, "refine_step": 2.0,
+    # (this build's own) refine a batch's items in the order of the master region they look up,
+    # so that items refined side by side share the caches; the records are the same either way
+    "refine_order": True,
     # ── phases ──
     "phase_by": "nmi", "nmi_bins": 16,
     # ── the correlation's GEMM (this build's own; the reference computes in float32): "fp32",
@@ -88,6 +91,7 @@ def check(s):
     bad(s["correlation_precision"] not in ("fp32", "fp16", "bf16"), "correlation_precision")
     bad(not (0 <= s["split_degree"] <= 256), "split_degree")
     bad(not isinstance(s["correlation_symmetry"], bool), "correlation_symmetry")
+    bad(not isinstance(s["refine_order"], bool), "refine_order")
     for k in ("wg_fft", "wg_refine"):
         bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
     for k, allowed in _ONLY.items():
