@@ -226,12 +226,14 @@ def cpu_baseline(e, pats_host, items, gpu_rec, truth, L, iters, slice_n=10_000, 
                           "median_deg_to_truth": round(float(np.median(d_true)), 4)}}
 
 
-def traffic_from_profiles(kernel, patterns_per_launch):
-    """HBM bytes per launch of `kernel`: the committed rocprofv3 --pmc summary
-    (profiles/<kernel>_pmc.txt, lines 'FETCH_bytes_per_pattern <v>' and 'WRITE_bytes_per_pattern
-    <v>', already scaled by the copy-kernel factor) times the patterns one launch takes here; None
-    without the summary."""
-    p = os.path.join(ROOT, "profiles", f"{kernel}_pmc.txt")
+def traffic_from_profiles(kernel, patterns_per_launch, workload="L64"):
+    """HBM bytes per launch of `kernel`: the committed rocprofv3 --pmc summary of the bench's
+    workload (profiles/<kernel>_L64.txt, or <kernel>_pmc.txt for the fit's kernel; lines
+    'FETCH_bytes_per_pattern <v>' and 'WRITE_bytes_per_pattern <v>', already scaled by the
+    copy-kernel factor) times the patterns one launch takes here; None without the summary."""
+    p = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.txt")
+    if not os.path.exists(p):
+        p = os.path.join(ROOT, "profiles", f"{kernel}_pmc.txt")
     if not os.path.exists(p):
         return None
     vals = {}
