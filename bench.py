@@ -268,7 +268,7 @@ def main():
     ap.add_argument("--fit", action="store_true",
                     help="the differentiation form instead: every pattern's pose (crystal 0.5-2 deg off, pattern centre up to 0.02 "
                          "off) fitted over the six directions by --fit-iters Levenberg-Marquardt steps (use with --size 80)")
-    ap.add_argument("--fit-iters", type=int, default=20)
+    ap.add_argument("--fit-iters", type=int, default=30)
     ap.add_argument("--fit-tol", type=float, default=1e-5,
                     help="a pose has converged when its last step is below this (pixels and radians)")
     ap.add_argument("--graphs", action="store_true",
@@ -529,7 +529,8 @@ def bench_fit(args, e, pats, items, truth):
         nat.profile(True)
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        poses, _, state = fit.fit6(field, p0, iters=args.fit_iters, tol=args.fit_tol, with_ncc=False)
+        counts = {}
+        poses, _, state = fit.fit6(field, p0, iters=args.fit_iters, tol=args.fit_tol, with_ncc=False, counts=counts)
     torch.cuda.synchronize()
     el = time.perf_counter() - t_start
     prof = nat.profile_report() if not args.no_profile else {}
@@ -545,7 +546,8 @@ def bench_fit(args, e, pats, items, truth):
     roofline = None
     if "k_sums6" in prof:
         cnt, ms = prof["k_sums6"]
-        ppl = args.steps * n * (args.fit_iters + 1) / cnt
+        # the poses the launches scored (converged items are skipped), the last step's count
+        ppl = args.steps * int(counts["scored"].item()) / cnt
         tf = 120.0 * n_mask * ppl / (ms / cnt * 1e-3) / 1e12
         roofline = {"bound": "mfma", "unit_kind": "valu", "achieved": round(tf, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(tf / VALU_PEAK_TFLOPS, 4),

@@ -250,6 +250,12 @@ int32_t zb_refine_order(const zb_frame* frame, const float* items, const float* 
 int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
                 int32_t flags, float* sums, void* stream);
 
+/* zb_sums for the items still moving: an item whose state [n] (zb_fit6_step's) is not 0 is left
+ * out and its sums are not written. No counterpart in the reference (whose fit re-scores every
+ * pattern each step); a fit to convergence then costs about the steps its slowest items need. */
+int32_t zb_sums_active(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
+                       int32_t flags, const int32_t* state, float* sums, void* stream);
+
 /* The six-direction fit to convergence, one Levenberg-Marquardt step at a time, in FP64 on the
  * device (one thread an item): zb_fit6_propose turns each item's sums (zb_sums at its pose) into
  * the NCC, gradient and Gauss-Newton matrix (planefit.zig:177-201 localOf), damps the matrix by
@@ -273,7 +279,9 @@ int32_t zb_fit6_accept(const float* trial, const float* trial_sums, int32_t n, d
  * (planefit.zig:177-201 localOf; ok 0 and the rest 0 where the master is flat under the mask);
  * zb_fit6_step proposes the damped step from it as zb_fit6_propose does from the sums
  * (planefit.zig:203-247 damped, solve); zb_fit6_keep keeps a trial whose local NCC rose (pose
- * and local copied, lambda / 10), else the old pose (lambda x 10), as zb_fit6_accept. */
+ * and local copied, lambda / 10), else the old pose (lambda x 10), as zb_fit6_accept. Items
+ * that have converged (state 1) stay put: zb_fit6_step leaves their trial at the pose and
+ * zb_fit6_keep leaves them alone, so their trial need not be scored (zb_sums_active). */
 int32_t zb_local6(const float* sums, int32_t n, double* local, void* stream);
 int32_t zb_fit6_step(const float* poses, const double* local, const double* lambda, int32_t n, int32_t unknowns, double max_rot, double max_shift,
                      float* trial, double* cur, int32_t* state, void* stream);

@@ -63,6 +63,7 @@ _SIGS = {
     "zb_refine": ([vp, vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
     "zb_refine_order": ([ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),
     "zb_local6": ([vp, i32, vp, vp], i32),
+    "zb_sums_active": ([vp, i32, ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp, vp], i32),
     "zb_fit6_step": ([vp, vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp], i32),
     "zb_fit6_keep": ([vp, vp, i32, ctypes.c_double, vp, vp, vp, vp, vp, vp], i32),
     "zb_sums": ([vp, i32, ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),

@@ -232,6 +232,11 @@ __global__ void __launch_bounds__(256) k_fit6_step(const float* __restrict__ pos
     const float* p = poses + (size_t)i * 8;
     float* t = trial + (size_t)i * 8;
     double ncc, g[6], H[36];
+    if (state[i] == 1) {
+        // converged: it stays where it is (its trial is not scored)
+        for (int k = 0; k < 8; ++k) t[k] = p[k];
+        return;
+    }
     if (!unpack_local(local + (size_t)i * kLocal, ncc, g, H) || state[i] == 2) {
         state[i] = 2;
         for (int k = 0; k < 8; ++k) t[k] = p[k];
@@ -246,7 +251,7 @@ __global__ void __launch_bounds__(256) k_fit6_keep(const float* __restrict__ tri
                                                    const double* __restrict__ cur, float* __restrict__ poses, double* __restrict__ local,
                                                    double* __restrict__ lambda, int* __restrict__ state) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.n || state[i] == 2) return;
+    if (i >= a.n || state[i] != 0) return;  // flat, or converged before this step
     const double* tl = tlocal + (size_t)i * kLocal;
     if (tl[1] > 0.5 && tl[0] > cur[2 * i]) {
         for (int k = 0; k < 8; ++k) poses[(size_t)i * 8 + k] = trial[(size_t)i * 8 + k];

@@ -444,11 +444,13 @@ __global__ void __launch_bounds__(1024) k_order(RefArg a, const float4* __restri
 // rotation_only zeroes the shift's sums, as sums_reduce does for orientation refinement.
 constexpr int kSums = 43;
 
+// skip (or NULL): items whose entry is not 0 are left out (their sums untouched).
 __global__ void __launch_bounds__(64) k_sums6(RefArg a, const float4* __restrict__ mg, const float* __restrict__ pats, const float* __restrict__ poses,
-                                              int rotation_only, float* __restrict__ out) {
+                                              int rotation_only, const int* __restrict__ skip, float* __restrict__ out) {
     extern __shared__ unsigned lds[];
     unsigned short* list = (unsigned short*)lds;
     const int b = blockIdx.x;
+    if (skip && skip[b] != 0) return;
     const int lane = threadIdx.x;
     const int n_px = build_list(a, list, lane);
     const float inv_w = 1.f / (float)a.w;
@@ -529,8 +531,25 @@ static int ref_frame(const zb_frame* frame, int side, RefArg& a, size_t& lds, co
     return n_px;
 }
 
+static int32_t sums_run(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
+                        int32_t flags, const int32_t* skip, float* sums, void* stream);
+
 extern "C" int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
                            int32_t flags, float* sums, void* stream) {
+    return sums_run(mgrad, side, frame, pats, poses, n_items, flags, nullptr, sums, stream);
+}
+
+extern "C" int32_t zb_sums_active(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses,
+                                  int32_t n_items, int32_t flags, const int32_t* state, float* sums, void* stream) {
+    if (n_items > 0 && !state) {
+        set_error("zb_sums_active: null state");
+        return ZB_EINVAL;
+    }
+    return sums_run(mgrad, side, frame, pats, poses, n_items, flags, state, sums, stream);
+}
+
+static int32_t sums_run(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
+                        int32_t flags, const int32_t* skip, float* sums, void* stream) {
     if (!frame || side < 2 || n_items < 0 || frame->pat_h < 2 || frame->pat_w < 2 || (flags & ~ZB_SUMS_ROTATION_ONLY)) {
         set_error("zb_sums: bad arguments");
         return ZB_EINVAL;
@@ -549,7 +568,7 @@ extern "C" int32_t zb_sums(const float* mgrad, int32_t side, const zb_frame* fra
         return ZB_EINVAL;
     }
     ZB_LAUNCH("k_sums6", k_sums6, dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, poses,
-              (flags & ZB_SUMS_ROTATION_ONLY) ? 1 : 0, sums);
+              (flags & ZB_SUMS_ROTATION_ONLY) ? 1 : 0, (const int*)skip, sums);
     return check_launch("zb_sums");
 }
 

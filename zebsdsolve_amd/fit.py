@@ -89,27 +89,34 @@ class PoseField:
         engine.prep_into(raw, len(cells), h, w, pats, stream_ptr())
         return cls(engine, pi, pats)
 
-    def sums(self, poses, flags=0):
+    def sums(self, poses, flags=0, state=None):
         """Sums [n, 43] (device, float32) at poses [n, 8] = (t, pattern index, q) (device or
-        host, float32)."""
+        host, float32). With `state` (int32 [n], the fit's), only the items whose state is 0
+        are summed (zb_sums_active); the others' rows are zero."""
         from .engine import stream_ptr
         lib = nat.load()
         p = torch.as_tensor(poses, dtype=torch.float32, device=self.pats.device).contiguous()
         n = p.shape[0]
-        out = torch.empty((n, N_SUMS), dtype=torch.float32, device=self.pats.device)
-        nat.check(lib.zb_sums(self.grad.data_ptr(), self.side, self.frame, self.pats.data_ptr(), p.data_ptr(), n, int(flags), out.data_ptr(),
-                              stream_ptr()), "sums")
+        if state is None:
+            out = torch.empty((n, N_SUMS), dtype=torch.float32, device=self.pats.device)
+            nat.check(lib.zb_sums(self.grad.data_ptr(), self.side, self.frame, self.pats.data_ptr(), p.data_ptr(), n, int(flags),
+                                  out.data_ptr(), stream_ptr()), "sums")
+        else:
+            out = torch.zeros((n, N_SUMS), dtype=torch.float32, device=self.pats.device)
+            nat.check(lib.zb_sums_active(self.grad.data_ptr(), self.side, self.frame, self.pats.data_ptr(), p.data_ptr(), n, int(flags),
+                                         state.data_ptr(), out.data_ptr(), stream_ptr()), "sums")
         return out
 
     def local(self, poses, flags=0):
         return local_of(self.sums(poses, flags).cpu().numpy())
 
-    def local6(self, poses, flags=0):
+    def local6(self, poses, flags=0, state=None):
         """The local model at poses on the device: [n, 29] float64 = (NCC, ok, g[6], the
-        Gauss-Newton matrix's upper triangle [21]) (zb_sums, then zb_local6)."""
+        Gauss-Newton matrix's upper triangle [21]) (zb_sums, then zb_local6); with `state`, of
+        the items still moving (the others' rows say flat)."""
         from .engine import stream_ptr
         lib = nat.load()
-        sums = self.sums(poses, flags)
+        sums = self.sums(poses, flags, state)
         n = sums.shape[0]
         out = torch.empty((n, LOCAL), dtype=torch.float64, device=self.pats.device)
         nat.check(lib.zb_local6(sums.data_ptr(), n, out.data_ptr(), stream_ptr()), "local6")
@@ -169,12 +176,12 @@ class PoseNcc(torch.autograd.Function):
     rotation part through the left Jacobian of exp at delta."""
 
     @staticmethod
-    def forward(ctx, delta, field, poses0, flags=0):
+    def forward(ctx, delta, field, poses0, flags=0, state=None):
         dev = field.pats.device  # the work is on the field's device, whatever delta's is
         p0 = torch.as_tensor(poses0).to(device=dev, dtype=torch.float64)
         d = delta.detach().to(device=dev, dtype=torch.float64)
         poses = _poses_t(p0[:, :3] + d[:, :3], p0[:, 3], _qmul_t(_exp_t(d[:, 3:]), p0[:, 4:8]))
-        local = field.local6(poses, flags)
+        local = field.local6(poses, flags, state)
         field.last_local = local
         g = local[:, 2:8]
         J = _left_jacobian_t(d[:, 3:])
@@ -185,7 +192,7 @@ class PoseNcc(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         (gd,) = ctx.saved_tensors
-        return (grad_out[:, None] * gd).to(grad_out.dtype), None, None, None
+        return (grad_out[:, None] * gd).to(grad_out.dtype), None, None, None, None
 
 
 class PoseNccPC(torch.autograd.Function):
@@ -221,26 +228,30 @@ class PoseNccPC(torch.autograd.Function):
         return (go * g_phi).to(grad_out.dtype), (go * g_pc).to(grad_out.dtype), None, None, None, None, None, None
 
 
-def local_with_autograd(field, poses, flags=0):
+def local_with_autograd(field, poses, flags=0, state=None):
     """The local model [n, 29] at poses with its gradient taken from a PoseNcc backward pass at
-    delta = 0 (the matrix and NCC from the same forward pass)."""
+    delta = 0 (the matrix and NCC from the same forward pass); with `state`, of the items still
+    moving."""
     z = torch.zeros((poses.shape[0], 6), dtype=torch.float64, device=field.pats.device, requires_grad=True)
     with torch.enable_grad():
-        ncc = PoseNcc.apply(z, field, poses, flags)
+        ncc = PoseNcc.apply(z, field, poses, flags, state)
         (g,) = torch.autograd.grad(ncc.sum(), z)
     local = field.last_local.clone()
     local[:, 2:8] = g
     return local
 
 
-def fit6(field, poses0, iters=20, unknowns=6, lambda0=1e-3, max_rot=np.radians(2.0), max_shift=2.0, tol=1e-7, with_ncc=True):
+def fit6(field, poses0, iters=20, unknowns=6, lambda0=1e-3, max_rot=np.radians(2.0), max_shift=2.0, tol=1e-7, with_ncc=True, counts=None):
     """Each pose fitted to its pattern by Levenberg-Marquardt steps over the six directions (or
     the rotation alone, unknowns = 3), on the device: per step each item's damped Newton step
     from its local model (zb_fit6_step, FP64: the gradient from a PoseNcc backward pass, the
     Gauss-Newton matrix from the same forward pass), the model at the moved poses (PoseNcc
-    again) deciding which are kept (zb_fit6_keep). Returns (poses [n, 8] float32, NCC [n]
+    again, for the items still moving) deciding which are kept (zb_fit6_keep); an item whose
+    step fell below tol has converged and stays put. Returns (poses [n, 8] float32, NCC [n]
     float64 (None without with_ncc), state [n]: 0 still moving after `iters` steps, 1
-    converged (the last step below tol), 2 flat), all on the device."""
+    converged (the last step below tol), 2 flat), all on the device. `counts` (a dict), if
+    given, receives "scored": a device tensor of the poses scored (the start and every trial of
+    an item still moving)."""
     from .engine import stream_ptr
     lib = nat.load()
     dev = field.pats.device
@@ -248,6 +259,7 @@ def fit6(field, poses0, iters=20, unknowns=6, lambda0=1e-3, max_rot=np.radians(2
     n = poses.shape[0]
     flags = ROTATION_ONLY if unknowns == 3 else 0
     local = local_with_autograd(field, poses, flags)
+    scored = torch.full((), n, dtype=torch.int64, device=dev)
     lam = torch.full((n,), float(lambda0), dtype=torch.float64, device=dev)
     cur = torch.zeros((n, 2), dtype=torch.float64, device=dev)
     state = torch.zeros((n,), dtype=torch.int32, device=dev)
@@ -256,9 +268,13 @@ def fit6(field, poses0, iters=20, unknowns=6, lambda0=1e-3, max_rot=np.radians(2
     for _ in range(iters):
         nat.check(lib.zb_fit6_step(poses.data_ptr(), local.data_ptr(), lam.data_ptr(), n, unknowns, float(max_rot), float(max_shift),
                                    trial.data_ptr(), cur.data_ptr(), state.data_ptr(), st), "fit6 step")
-        tlocal = local_with_autograd(field, trial, flags)
+        # only the items still moving are scored (converged ones stay put)
+        tlocal = local_with_autograd(field, trial, flags, state)
+        scored += (state == 0).sum()
         nat.check(lib.zb_fit6_keep(trial.data_ptr(), tlocal.data_ptr(), n, float(tol), cur.data_ptr(), poses.data_ptr(), local.data_ptr(),
                                    lam.data_ptr(), state.data_ptr(), st), "fit6 keep")
+    if counts is not None:
+        counts["scored"] = scored
     if not with_ncc:
         return poses, None, state
     return poses, torch.where(local[:, 1] > 0.5, local[:, 0], torch.zeros_like(local[:, 0])), state
