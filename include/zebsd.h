@@ -35,6 +35,14 @@ extern "C" {
 #define ZB_FLAG_CENTER 1 /* subtract the window mean */
 #define ZB_FLAG_NORM 2   /* scale to unit harmonic norm */
 #define ZB_FLAG_WINDOW 4 /* project the window itself (ones) instead of a pattern */
+/* The Legendre sums with 16-bit products and FP32 accumulation (the SH-analysis GEMM of the
+ * precision form; this build's own): FP16 or BF16 on the matrix cores, each value as
+ * hi = round16(x) and lo = round16(x - hi), hi lo + lo hi added for the degrees from the split
+ * degree (bits 16..23 of the flags: ZB_FLAG_SPLIT(l)) on. Below n = 32 the FP32 sums are used. */
+#define ZB_FLAG_FP16 8
+#define ZB_FLAG_BF16 16
+#define ZB_FLAG_SPLIT_SHIFT 16
+#define ZB_FLAG_SPLIT(l) (((l) & 0xff) << ZB_FLAG_SPLIT_SHIFT)
 
 /* The detector's attitude and circular mask in pixels of the patterns worked on
  * (spherical.Frame, src/spherical.zig:80-90). */

@@ -457,7 +457,8 @@ class So3:
     (the reference transforms a contraction at pi/2 along k instead; the two are equal by
     d(beta) = i^{m'-m} sum_k d_{km'}(pi/2) d_{km}(pi/2) e^{-ik beta}, src/sh.zig:484-519)."""
 
-    def __init__(self, L, fold):
+    def __init__(self, L, fold, rows=None):
+        """rows: the rows j of b to tabulate (all n/2 + 1 by default; volume needs all)."""
         self.L, self.fold = L, fold
         self.n = osh.fft_size(L)
         self.nc = self.n // fold
@@ -467,7 +468,7 @@ class So3:
         self.mus = mus
         # D[j, l, m', mu] = d^l_{m', fold mu}(beta_j), m' >= 0
         self.D = np.zeros((self.nb, L, L, len(mus)))
-        for j in range(self.nb):
+        for j in (range(self.nb) if rows is None else sorted(set(int(r) for r in rows))):
             beta = TAU * j / self.n
             for l in range(L):
                 d = osh.wigner_d_matrix(l, beta)
@@ -495,6 +496,17 @@ class So3:
         V0[:, 0, :] = V0[:, 0, :].real
         C = np.real(np.einsum("jpc,pa->jca", V0 * k[None, :, None], Ea))
         return C
+
+    def value_at(self, f, g, j, c, a):
+        """C[j, c, a] alone (the same sums as volume, for one row j of b)."""
+        L, n, nc = self.L, self.n, self.nc
+        F = osh.signed(f, L)[:, L - 1 + self.fold * self.mus]
+        G = np.conj(osh.signed(g, L)[:, L - 1:])
+        U = np.einsum("lu,lp,lpu->pu", F, G, self.D[j])                  # [m', mu]
+        V = U @ np.exp(-2j * np.pi * self.mus * c / nc)                   # [m']
+        k = np.where(np.arange(L) == 0, 1.0, 2.0)
+        V[0] = V[0].real
+        return float(np.real(np.sum(k * V * np.exp(-2j * np.pi * np.arange(L) * a / n))))
 
     def rden(self, wf, wf2, area):
         """1 / sqrt(<w,f^2> - <w,f>^2 / area), 0 where that is <= 1e-6 area

@@ -581,3 +581,18 @@ def test_normalized_correlation_of_a_window_limited_match_is_one():
         mq = np.sum(win * fq * wt) / area
         ncc = np.sum(g * fq * wt) / np.sqrt(np.sum(win * (fq - mq) ** 2 * wt)) / gnorm
         assert abs(score[jj, cc, aa] - ncc) < 1e-10, (jj, cc, aa)
+
+
+def test_single_cell_correlation_equals_the_volume():
+    """So3.value_at (one cell from one row of b) against the whole volume, fold 1 and 4."""
+    rng = np.random.default_rng(4)
+    for L, fold in ((8, 1), (12, 4)):
+        f, g = rand_coef(rng, L), rand_coef(rng, L)
+        so = op.So3(L, fold)
+        C = so.volume(f, g)
+        part = op.So3(L, fold, rows=[1, 3])
+        for _ in range(6):
+            c, a = int(rng.integers(0, so.nc)), int(rng.integers(0, so.n))
+            j = int(rng.integers(0, so.nb))
+            assert abs(so.value_at(f, g, j, c, a) - C[j, c, a]) < 1e-9 * np.abs(C).max()
+            assert abs(part.value_at(f, g, 3, c, a) - C[3, c, a]) < 1e-9 * np.abs(C).max()

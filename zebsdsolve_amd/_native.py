@@ -82,6 +82,14 @@ _SIGS = {
 
 EXPORTS = tuple(_SIGS)
 
+# zb_sh_coefficients' flags (include/zebsd.h)
+FLAG_CENTER, FLAG_NORM, FLAG_WINDOW, FLAG_FP16, FLAG_BF16 = 1, 2, 4, 8, 16
+
+
+def flag_split(degree):
+    """The flag bits naming the first degree whose Legendre products take hi + lo."""
+    return (int(degree) & 0xFF) << 16
+
 _lib = None
 
 

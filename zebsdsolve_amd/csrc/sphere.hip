@@ -220,6 +220,98 @@ __global__ void __launch_bounds__(64) k_legendre(SphArg a, int n_items, const fl
     }
 }
 
+// The same sums with 16-bit products (FP16, or BF16 with BF) and FP32 accumulation, on
+// v_mfma_f32_32x32x16_{f16,bf16}: 16 rings a product (A lane -> (row lane % 32, rings
+// (lane / 32) 8 .. + 8), B lane -> (those rings, column lane % 32); the output layout is the
+// 32 x 32 one above). Each value x is carried as hi = round16(x) and lo = round16(x - hi); the
+// column tiles holding a degree >= split take hi hi + hi lo + lo hi (an error near FP32's), the
+// others hi hi. FP16's exponent range is narrow: the Legendre table's values (below ~1e-3 at
+// L = 96) and their lo parts would fall into its subnormals, so with FP16 the table is scaled by
+// 2^12 on the way in and the sums by 2^-12 on the way out (exact: powers of two).
+template <bool BF>
+struct Leg16;
+template <>
+struct Leg16<false> {
+    typedef _Float16 T;
+    typedef _Float16 V8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ floatx16 mfma(V8 a, V8 b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Leg16<true> {
+    typedef __bf16 T;
+    typedef __bf16 V8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ floatx16 mfma(V8 a, V8 b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+};
+
+template <bool BF>
+__global__ void __launch_bounds__(64) k_legendre16(SphArg a, int n_items, int split, const float* __restrict__ wplm,
+                                                   const float4* __restrict__ spectra, float2* __restrict__ alm) {
+    typedef typename Leg16<BF>::T T;
+    typedef typename Leg16<BF>::V8 V8;
+    const int lane = threadIdx.x;
+    const int half = lane >> 5;
+    const int r32 = lane & 31;
+    const int m = blockIdx.y;
+    const int b0 = blockIdx.x * 16;
+    const int nh = a.n >> 1;
+    const int L = a.L;
+    const int base = m * L - (m * (m - 1)) / 2 - m;
+    const int item = b0 + (r32 >> 1);
+    const bool im = (r32 & 1) != 0;
+    const bool iok = item < n_items;
+    const float4* sp = spectra + ((size_t)(iok ? item : 0) * L + m) * nh;
+    const int ncol = (L - m + 1) / 2;
+    constexpr float kScale = BF ? 1.f : 4096.f;
+    for (int ct = 0; ct * 32 < ncol; ++ct) {
+        const int le = m + 2 * (ct * 32 + r32);
+        const int lo = le + 1;
+        // the tile's highest degree decides the split (uniform over the wave)
+        const bool two = m + 2 * (ct * 32 + 31) + 1 >= split;
+        floatx16 ce, co;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) ce[r] = co[r] = 0.f;
+        for (int ks = 0; ks < nh / 16; ++ks) {
+            V8 aeh, aol, ael, aoh, beh, bel, boh, bol;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int y = 16 * ks + 8 * half + i;
+                const float4 v = iok ? sp[y] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float xe = im ? v.y : v.x, xo = im ? v.w : v.z;
+                const float* wy = wplm + (size_t)y * a.pairs + base;
+                const float we = le < L ? kScale * wy[le] : 0.f, wo = lo < L ? kScale * wy[lo] : 0.f;
+                const T h0 = (T)xe, h1 = (T)xo, h2 = (T)we, h3 = (T)wo;
+                aeh[i] = h0;
+                aoh[i] = h1;
+                beh[i] = h2;
+                boh[i] = h3;
+                ael[i] = (T)(xe - (float)h0);
+                aol[i] = (T)(xo - (float)h1);
+                bel[i] = (T)(we - (float)h2);
+                bol[i] = (T)(wo - (float)h3);
+            }
+            ce = Leg16<BF>::mfma(aeh, beh, ce);
+            co = Leg16<BF>::mfma(aoh, boh, co);
+            if (two) {
+                ce = Leg16<BF>::mfma(aeh, bel, ce);
+                ce = Leg16<BF>::mfma(ael, beh, ce);
+                co = Leg16<BF>::mfma(aoh, bol, co);
+                co = Leg16<BF>::mfma(aol, boh, co);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r += 2) {
+            const int row = (r >> 2) * 8 + half * 4 + (r & 3);
+            const int it = b0 + (row >> 1);
+            if (it < n_items) {
+                float2* out = alm + (size_t)it * a.pairs + base;
+                constexpr float inv = 1.f / kScale;
+                if (le < L) out[le] = make_float2(ce[r] * inv, ce[r + 1] * inv);
+                if (lo < L) out[lo] = make_float2(co[r] * inv, co[r + 1] * inv);
+            }
+        }
+    }
+}
+
 // Unit norm sqrt(sum_m k |a|^2), k = 1 for m = 0 else 2, and the stats (sphere.wgsl:228-262).
 __global__ void __launch_bounds__(256) k_norm(SphArg a, const float2* __restrict__ sums, float2* __restrict__ alm, float* __restrict__ stats) {
     __shared__ float red[4];
@@ -276,6 +368,10 @@ static int sh_check(const char* who, const zb_band* band, const zb_frame* frame,
     if (!band || !frame || n_items < 0 || band->L < 2 || ilog2(band->n) < 4 || band->n < 2 * band->L - 1 || 2 * band->n > ZB_TW ||
         frame->pat_h < 2 || frame->pat_w < 2) {
         set_error("%s: bad band or frame", who);
+        return ZB_EINVAL;
+    }
+    if ((flags & ZB_FLAG_FP16) && (flags & ZB_FLAG_BF16)) {
+        set_error("%s: FP16 and BF16 products both asked for", who);
         return ZB_EINVAL;
     }
     if (n_items == 0) return 1;
@@ -379,8 +475,19 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
     ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,
                        (const float4*)items, (const float2*)sums, spectra);
     if (int rc = check_launch("zb_sh_coefficients/rings")) return rc;
-    ZB_LAUNCH("k_legendre", k_legendre, dim3((n_items + 15) / 16, L), dim3(64), 0, s, a, n_items, band->wplm, (const float4*)spectra,
-              (float2*)alm);
+    if ((flags & (ZB_FLAG_FP16 | ZB_FLAG_BF16)) && nh >= 16) {
+        // 16-bit products, 16 rings a product (below n = 32 the FP32 sums stand in)
+        const int split = (flags >> ZB_FLAG_SPLIT_SHIFT) & 0xff;
+        if (flags & ZB_FLAG_BF16)
+            ZB_LAUNCH("k_legendre", k_legendre16<true>, dim3((n_items + 15) / 16, L), dim3(64), 0, s, a, n_items, split, band->wplm,
+                      (const float4*)spectra, (float2*)alm);
+        else
+            ZB_LAUNCH("k_legendre", k_legendre16<false>, dim3((n_items + 15) / 16, L), dim3(64), 0, s, a, n_items, split, band->wplm,
+                      (const float4*)spectra, (float2*)alm);
+    } else {
+        ZB_LAUNCH("k_legendre", k_legendre, dim3((n_items + 15) / 16, L), dim3(64), 0, s, a, n_items, band->wplm, (const float4*)spectra,
+                  (float2*)alm);
+    }
     if (int rc = check_launch("zb_sh_coefficients/legendre")) return rc;
     ZB_LAUNCH("k_norm", k_norm, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
     return check_launch("zb_sh_coefficients/norm");

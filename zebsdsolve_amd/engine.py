@@ -466,6 +466,17 @@ class Engine:
         vt.b_half = 1 if (self.settings["correlation_symmetry"] and ph.y2) else 0
         return vt
 
+    def sh_flags(self):
+        """zb_sh_coefficients' flags for patterns: centred, unit norm, and the Legendre sums'
+        products at the correlation's precision (16-bit with hi + lo from split_degree)."""
+        kind = self.settings["correlation_precision"]
+        f = nat.FLAG_CENTER | nat.FLAG_NORM
+        if kind == "fp16":
+            f |= nat.FLAG_FP16 | nat.flag_split(min(int(self.settings["split_degree"]), 255))
+        elif kind == "bf16":
+            f |= nat.FLAG_BF16 | nat.flag_split(min(int(self.settings["split_degree"]), 255))
+        return f
+
     def use_mfma(self, L, pi=None):
         """The matrix-core correlation for phase pi at L (every phase when pi is None)."""
         if self.correlator != "mfma":
@@ -617,10 +628,11 @@ class Engine:
             raw.record_stream(lane.stream)
             items.record_stream(lane.stream)
         self.prep_into(raw, raw.shape[0], h, w, pats, st)
+        sh_flags = self.sh_flags()
         for b0 in range(0, n, big):
             nbt = min(big, n - b0)
             it = items[b0:b0 + nbt]
-            nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nbt, 3, workspace.data_ptr(), workspace.numel(),
+            nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nbt, sh_flags, workspace.data_ptr(), workspace.numel(),
                                              alm.data_ptr(), stats.data_ptr(), st), "coefficients")
             for pi, ph in enumerate(phs):
                 for c0 in range(0, nbt, chunk):
