@@ -261,6 +261,7 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp16", "bf16"), default="fp32",
                     help="products of the correlation's GEMM: float32, or 16-bit with float32 accumulation (hi + lo for l >= 64)")
     ap.add_argument("--size", type=int, default=60, help="pattern side in pixels (60: the plain FP32 form; 120: the precision form)")
+    ap.add_argument("--shape", default=None, help="pattern height x width (e.g. 240x320), in place of --size")
     ap.add_argument("--phases", choices=("ni", "ni,gan"), default="ni",
                     help="masters: nickel alone, or nickel and GaN with each pattern drawn from nickel at --p-first")
     ap.add_argument("--p-first", type=float, default=0.7, help="with two phases, the share of patterns drawn from nickel")
@@ -303,6 +304,8 @@ def main():
 
     global H, W
     H = W = args.size
+    if args.shape:
+        H, W = (int(x) for x in args.shape.lower().split("x"))
 
     import torch
     from zebsdsolve_amd import _native as nat, engine, sym
@@ -464,7 +467,7 @@ def main():
     hbm_comp = {"bytes_per_pattern": comp, "gbs": round(comp * value / 1e9, 2), "frac": round(comp * value / 1e9 / HBM_PEAK_GBS, 5)}
 
     cpu = None
-    if not args.no_cpu and not two and not args.from_host and args.pixel == "f32" and H == 60:
+    if not args.no_cpu and not two and not args.from_host and args.pixel == "f32" and H == W == 60:
         k = min(10_000, n)
         cpu = cpu_baseline(e, pats[:k].cpu().numpy(), items[:k], rec[:k], truth[:k], args.bandwidth, args.iters)
 

@@ -154,3 +154,39 @@ def test_refinement_order_is_a_permutation_and_leaves_the_records_alone():
     near = np.mean(np.sum(v[o[1:]] * v[o[:-1]], 1))
     rand = np.mean(np.sum(v[1:] * v[:-1], 1))
     assert near > 0.95 and near > rand + 0.5, (near, rand)
+
+
+def test_frames_past_65536_pixels_index_and_refine_as_the_oracle():
+    """240 x 320 patterns (76 800 pixels: the refinement's pixel list no longer fits LDS, so the
+    whole frame is walked with the mask tested per pixel): a toy scan is recovered to the
+    reference's thresholds (median < 0.15 deg, max < 1 deg), the refinement's NCC from the same
+    seed matches the float64 oracle's, and zb_sums matches the oracle's sums."""
+    from oracle import pipeline as op
+    from zebsdsolve_amd import fit
+    print("240 x 320 frames", flush=True)
+    s = _solver()
+    truth = s.synthesize(toy_rows=4, toy_cols=6, toy_grains=24, toy_counts=400, toy_seed=17, toy_height=240, toy_width=320)
+    r = s.index()
+    d = np.degrees(s.disorientation(r["quaternions"].reshape(-1, 4), truth["quaternions"].reshape(-1, 4)))
+    assert np.median(d) < 0.15 and d.max() < 1.0, d
+    e = s._e
+    rec = e.index_cells(np.arange(2))[0]
+    # the oracle's NCC at the GPU's refined orientation (iters 0: the score there)
+    from zebsdsolve_amd.geometry import frame_of
+    f = frame_of(e.geo, 240, 320, True, 1.0)
+    _, (_, grad_d, host) = e.baked(0)
+    side = e.phases[0].master.side
+    g = grad_d.cpu().numpy().reshape(-1, 4)
+    t = e.positions(np.arange(2))
+    raw = np.asarray(e.patterns).reshape(-1, 240, 320)[:2].astype(np.float64)
+    for b in range(2):
+        pn = op.normalize(raw[b], f["mask"])
+        _, ncc, _ = op.refine(g, side, f, pn, t[b].astype(np.float64), rec[b, :4].astype(np.float64), iters=0)
+        assert abs(ncc - rec[b, 4]) < 1e-3, (b, ncc, rec[b, 4])
+    field = fit.PoseField.for_cells(e, 0, np.arange(2))
+    poses = np.concatenate([t, np.arange(2, dtype=np.float32)[:, None], rec[:, :4]], 1).astype(np.float32)
+    got = field.sums(poses).cpu().numpy().astype(np.float64)
+    pats_n = field.pats.cpu().numpy()
+    for b in range(2):
+        ref = op.sums6(g, side, f, pats_n[b].astype(np.float64), poses[b, :3].astype(np.float64), poses[b, 4:8].astype(np.float64))
+        assert got[b, 0] == ref[0] and np.abs(got[b] - ref).max() < 1e-4 * np.abs(ref).max()

@@ -19,6 +19,9 @@ struct RefArg {
     int iters, nmi_bins, seed_stride, out0;
     float max_step, scale0, scale_min, scale_max;
     float stamp;
+    // 1: the masked pixels as a list of 16-bit indices in LDS (frames up to 65 536 pixels whose
+    // list fits); 0: the whole frame walked, each pixel tested against the mask (larger frames)
+    int use_list;
 };
 
 struct Sums {
@@ -71,9 +74,27 @@ __device__ int build_list(const RefArg& a, unsigned short* list, int lane) {
     return total;
 }
 
-// The ray of list pixel `pix` (row-major index) from the item's origin t, in the sample frame.
+// Pixel k of a pass: the list's k-th (the masked pixels, in LDS) or, without a list, the
+// frame's k-th; `ok` says whether it counts (k inside the pass, the pixel under the mask). Past
+// the end the last pixel is looked up again and counts for nothing.
+template <bool LIST>
+__device__ __forceinline__ int pass_pixel(const RefArg& a, const unsigned short* list, int n_px, int k, bool& ok) {
+    const int kk = min(k, n_px - 1);
+    if constexpr (LIST) {
+        ok = k < n_px;
+        return list[kk];
+    }
+    const int row = kk / a.w;
+    const float dr = (float)row - a.mask[0], dc = (float)(kk - row * a.w) - a.mask[1];
+    ok = k < n_px && (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2]);
+    return kk;
+}
+
+// The ray of pixel `pix` (row-major index) from the item's origin t, in the sample frame; the
+// row by a float product below 65 536 pixels (exact there), by division above.
+template <bool LIST>
 __device__ __forceinline__ float3 pixel_ray(const RefArg& a, int pix, float inv_w, float3 t) {
-    const int row = (int)(((float)pix + 0.5f) * inv_w);
+    const int row = LIST ? (int)(((float)pix + 0.5f) * inv_w) : pix / a.w;
     const float fr = (float)row, fc = (float)(pix - row * a.w);
     return make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
 }
@@ -82,6 +103,7 @@ __device__ __forceinline__ float3 pixel_ray(const RefArg& a, int pix, float inv_
 // (3 each) before it blends any.
 constexpr int kBatch = 4;
 
+template <bool LIST>
 __device__ Sums rot_sums(const RefArg& a, __amdgpu_buffer_rsrc_t mgh, const float* __restrict__ pat, const unsigned short* list, int n_px,
                          float inv_w, float4 q, float3 t, int lane) {
     Sums s;
@@ -91,18 +113,19 @@ __device__ Sums rot_sums(const RefArg& a, __amdgpu_buffer_rsrc_t mgh, const floa
         TapsH tp[kBatch];
         float3 c[kBatch];
         float e[kBatch];
+        bool ok[kBatch];
 #pragma unroll
         for (int p = 0; p < kBatch; ++p) {
-            // lanes past the end look up the last pixel again and add nothing
-            const int pix = list[min(k0 + 64 * p, n_px - 1)];
-            c[p] = qrot(q, normalize3(pixel_ray(a, pix, inv_w, t)));
+            // lanes past the end (or off the mask) look up a pixel and add nothing
+            const int pix = pass_pixel<LIST>(a, list, n_px, k0 + 64 * p, ok[p]);
+            c[p] = qrot(q, normalize3(pixel_ray<LIST>(a, pix, inv_w, t)));
             sheet_taps_h(mgh, a.side, c[p], tp[p]);
             e[p] = pat[pix];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int p = 0; p < kBatch; ++p) {
-            if (k0 + 64 * p < n_px) {
+            if (ok[p]) {
                 const float4 m = taps_blend(tp[p]);
                 const float3 d = cross3(c[p], make_float3(m.x, m.y, m.z));
                 s.n += 1.f;
@@ -132,14 +155,15 @@ __device__ Sums rot_sums(const RefArg& a, __amdgpu_buffer_rsrc_t mgh, const floa
 // The master's values at orientation q for the list pixels k0 + 64 p, p < kBatch (lanes past the
 // end get the last pixel's), with the pattern's values there: all gathers issued before any
 // blend. msh: a buffer resource over the float32 sheets.
+template <bool LIST>
 __device__ __forceinline__ void sims_batch(const RefArg& a, __amdgpu_buffer_rsrc_t msh, const float* __restrict__ pat,
                                            const unsigned short* list, int n_px, float inv_w, float4 q, float3 t, int k0,
-                                           float* sim, float* e) {
+                                           float* sim, float* e, bool* ok) {
     TapsM tp[kBatch];
 #pragma unroll
     for (int p = 0; p < kBatch; ++p) {
-        const int pix = list[min(k0 + 64 * p, n_px - 1)];
-        sheet_taps_m(msh, a.side, qrot(q, normalize3(pixel_ray(a, pix, inv_w, t))), tp[p]);
+        const int pix = pass_pixel<LIST>(a, list, n_px, k0 + 64 * p, ok[p]);
+        sheet_taps_m(msh, a.side, qrot(q, normalize3(pixel_ray<LIST>(a, pix, inv_w, t))), tp[p]);
         e[p] = pat[pix];
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -171,18 +195,20 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
 // With `order` (zb_refine_order's permutation), block i refines item order[xcd_block(i)]: the
 // blocks of one XCD take a contiguous run of items sorted by the master region they look up, so
 // that their gathers share that XCD's L2. Without it, block i refines item i.
-template <int MAXP>
+template <int MAXP, bool LIST>
 __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict__ ms, const uint2* __restrict__ mgh, const float* __restrict__ pats,
                                                const float4* __restrict__ items, const float* __restrict__ seeds, int n_items,
                                                const int* __restrict__ order, float* __restrict__ records) {
     static_assert(MAXP % kBatch == 0, "MAXP: whole batches");
     extern __shared__ unsigned lds[];
     unsigned* hist = lds;                                               // [nmi_bins][nmi_bins]
-    unsigned short* list = (unsigned short*)(lds + a.nmi_bins * a.nmi_bins);  // the masked pixels
+    unsigned short* list = LIST ? (unsigned short*)(lds + a.nmi_bins * a.nmi_bins) : nullptr;  // the masked pixels
     const int b = order ? order[xcd_block(blockIdx.x, gridDim.x)] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < a.nmi_bins * a.nmi_bins; i += 64) hist[i] = 0u;
-    const int n_px = build_list(a, list, lane);
+    int n_px = a.h * a.w;
+    if constexpr (LIST) n_px = build_list(a, list, lane);
+    else __syncthreads();
     const float inv_w = 1.f / (float)a.w;
     const float4 it = items[b];
     const float3 t = make_float3(it.x, it.y, it.z);
@@ -201,7 +227,7 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
     const __amdgpu_buffer_rsrc_t fm = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(ms), (short)0, 2 * a.side * a.side * 4, 0x00020000);
 
     for (int itn = 0; itn <= a.iters; ++itn) {
-        const Sums s = rot_sums(a, fh, pat, list, n_px, inv_w, cur, t, lane);
+        const Sums s = rot_sums<LIST>(a, fh, pat, list, n_px, inv_w, cur, t, lane);
         // NCC, its gradient and Gauss-Newton matrix over the rotation (detector.wgsl:561-583)
         const float n = fmaxf(s.n, 1.f);
         const float mean = s.m / n;
@@ -277,20 +303,22 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
 #pragma unroll
         for (int i0 = 0; i0 < MAXP; i0 += kBatch) {
             float sim[kBatch] = {}, e[kBatch] = {};
-            if (64 * i0 < n_px) sims_batch(a, fm, pat, list, n_px, inv_w, qbest, t, lane + 64 * i0, sim, e);
+            bool ok[kBatch] = {};
+            if (64 * i0 < n_px) sims_batch<LIST>(a, fm, pat, list, n_px, inv_w, qbest, t, lane + 64 * i0, sim, e, ok);
 #pragma unroll
             for (int p = 0; p < kBatch; ++p) {
                 simv[i0 + p] = sim[p];
-                if (lane + 64 * (i0 + p) < n_px) extremes(e[p], sim[p]);
+                if (ok[p]) extremes(e[p], sim[p]);
             }
         }
     } else {
         for (int k0 = lane; k0 < n_px; k0 += 64 * kBatch) {
             float sim[kBatch], e[kBatch];
-            sims_batch(a, fm, pat, list, n_px, inv_w, qbest, t, k0, sim, e);
+            bool ok[kBatch];
+            sims_batch<LIST>(a, fm, pat, list, n_px, inv_w, qbest, t, k0, sim, e, ok);
 #pragma unroll
             for (int p = 0; p < kBatch; ++p)
-                if (k0 + 64 * p < n_px) extremes(e[p], sim[p]);
+                if (ok[p]) extremes(e[p], sim[p]);
         }
     }
     for (int o = 1; o < 64; o <<= 1) {
@@ -321,20 +349,22 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
         for (int i0 = 0; i0 < MAXP; i0 += kBatch) {
             if (64 * i0 < n_px) {
                 float e[kBatch];
+                bool ok[kBatch];
 #pragma unroll
-                for (int p = 0; p < kBatch; ++p) e[p] = pat[list[min(lane + 64 * (i0 + p), n_px - 1)]];
+                for (int p = 0; p < kBatch; ++p) e[p] = pat[pass_pixel<LIST>(a, list, n_px, lane + 64 * (i0 + p), ok[p])];
 #pragma unroll
                 for (int p = 0; p < kBatch; ++p)
-                    if (lane + 64 * (i0 + p) < n_px) bin(e[p], simv[i0 + p]);
+                    if (ok[p]) bin(e[p], simv[i0 + p]);
             }
         }
     } else {
         for (int k0 = lane; k0 < n_px; k0 += 64 * kBatch) {
             float sim[kBatch], e[kBatch];
-            sims_batch(a, fm, pat, list, n_px, inv_w, qbest, t, k0, sim, e);
+            bool ok[kBatch];
+            sims_batch<LIST>(a, fm, pat, list, n_px, inv_w, qbest, t, k0, sim, e, ok);
 #pragma unroll
             for (int p = 0; p < kBatch; ++p)
-                if (k0 + 64 * p < n_px) bin(e[p], sim[p]);
+                if (ok[p]) bin(e[p], sim[p]);
         }
     }
     __syncthreads();
@@ -445,6 +475,7 @@ __global__ void __launch_bounds__(1024) k_order(RefArg a, const float4* __restri
 constexpr int kSums = 43;
 
 // skip (or NULL): items whose entry is not 0 are left out (their sums untouched).
+template <bool LIST>
 __global__ void __launch_bounds__(64) k_sums6(RefArg a, const float4* __restrict__ mg, const float* __restrict__ pats, const float* __restrict__ poses,
                                               int rotation_only, const int* __restrict__ skip, float* __restrict__ out) {
     extern __shared__ unsigned lds[];
@@ -452,7 +483,9 @@ __global__ void __launch_bounds__(64) k_sums6(RefArg a, const float4* __restrict
     const int b = blockIdx.x;
     if (skip && skip[b] != 0) return;
     const int lane = threadIdx.x;
-    const int n_px = build_list(a, list, lane);
+    int n_px = a.h * a.w;
+    if constexpr (LIST) n_px = build_list(a, list, lane);
+    const unsigned short* lst = LIST ? list : nullptr;
     const float inv_w = 1.f / (float)a.w;
     const float* ps = poses + (size_t)b * 8;
     const float3 t = make_float3(ps[0], ps[1], ps[2]);
@@ -463,8 +496,10 @@ __global__ void __launch_bounds__(64) k_sums6(RefArg a, const float4* __restrict
 #pragma unroll
     for (int i = 0; i < kSums; ++i) s[i] = 0.f;
     for (int k = lane; k < n_px; k += 64) {
-        const int pix = list[k];
-        const int row = (int)(((float)pix + 0.5f) * inv_w);
+        bool ok;
+        const int pix = pass_pixel<LIST>(a, lst, n_px, k, ok);
+        if (!ok) continue;
+        const int row = LIST ? (int)(((float)pix + 0.5f) * inv_w) : pix / a.w;
         const float fr = (float)row, fc = (float)(pix - row * a.w);
         const float3 u = make_float3(a.c0[0] * fr + a.c1[0] * fc + t.x, a.c0[1] * fr + a.c1[1] * fc + t.y, a.c0[2] * fr + a.c1[2] * fc + t.z);
         const float dist = sqrtf(dot3(u, u));
@@ -517,18 +552,23 @@ static int ref_frame(const zb_frame* frame, int side, RefArg& a, size_t& lds, co
     a.h = frame->pat_h;
     a.w = frame->pat_w;
     a.side = side;
-    if ((long)a.h * a.w > 65536) {
-        set_error("%s: %d x %d pixels: at most 65536", who, a.h, a.w);
+    if ((long)a.h * a.w > (1L << 26)) {
+        set_error("%s: %d x %d pixels: at most 2^26", who, a.h, a.w);
         return ZB_EINVAL;
     }
-    int n_px = 0;
+    long n_px = 0;
     for (int r = 0; r < a.h; ++r)
         for (int c = 0; c < a.w; ++c) {
             const float dr = (float)r - a.mask[0], dc = (float)c - a.mask[1];
             n_px += (a.mask[2] <= 0.f || dr * dr + dc * dc <= a.mask[2]) ? 1 : 0;
         }
-    lds = ((size_t)n_px * 2 + 15) / 16 * 16;
-    return n_px;
+    // the masked pixels as a 16-bit list in LDS while their indices fit 16 bits and the list
+    // (with the NMI's histogram, at most 32 x 32 words) stays within 60 KB; else the whole frame
+    // is walked with the mask tested per pixel
+    const size_t list_bytes = ((size_t)n_px * 2 + 15) / 16 * 16;
+    a.use_list = ((long)a.h * a.w <= 65536 && list_bytes <= 60 * 1024) ? 1 : 0;
+    lds = a.use_list ? list_bytes : 0;
+    return (int)n_px;
 }
 
 static int32_t sums_run(const float* mgrad, int32_t side, const zb_frame* frame, const float* pats, const float* poses, int32_t n_items,
@@ -563,12 +603,12 @@ static int32_t sums_run(const float* mgrad, int32_t side, const zb_frame* frame,
     size_t lds = 0;
     const int n_px = ref_frame(frame, side, a, lds, "zb_sums");
     if (n_px < 0) return n_px;
-    if (lds > 64 * 1024) {
-        set_error("zb_sums: %d masked pixels: at most %d", n_px, 32 * 1024);
-        return ZB_EINVAL;
-    }
-    ZB_LAUNCH("k_sums6", k_sums6, dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, poses,
-              (flags & ZB_SUMS_ROTATION_ONLY) ? 1 : 0, (const int*)skip, sums);
+    if (a.use_list)
+        ZB_LAUNCH("k_sums6", k_sums6<true>, dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, poses,
+                  (flags & ZB_SUMS_ROTATION_ONLY) ? 1 : 0, (const int*)skip, sums);
+    else
+        ZB_LAUNCH("k_sums6", k_sums6<false>, dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, (const float4*)mgrad, pats, poses,
+                  (flags & ZB_SUMS_ROTATION_ONLY) ? 1 : 0, (const int*)skip, sums);
     return check_launch("zb_sums");
 }
 
@@ -601,15 +641,14 @@ extern "C" int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32
     a.stamp = (float)o->stamp;
     // the joint histogram, then the list (the histogram's size keeps the list 4-byte aligned)
     const size_t lds = (size_t)o->nmi_bins * o->nmi_bins * 4 + list_bytes;
-    if (lds > 64 * 1024) {
-        set_error("zb_refine: %d masked pixels: at most %d", n_px, (int)(64 * 1024 - o->nmi_bins * o->nmi_bins * 4) / 2);
-        return ZB_EINVAL;
-    }
-    if (n_px <= 64 * 48)
-        ZB_LAUNCH("k_refine", (k_refine<48>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
+    if (a.use_list && n_px <= 64 * 48)
+        ZB_LAUNCH("k_refine", (k_refine<48, true>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
+                  (const float4*)items, seeds, n_items, (const int*)o->order, records);
+    else if (a.use_list)
+        ZB_LAUNCH("k_refine", (k_refine<0, true>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
                   (const float4*)items, seeds, n_items, (const int*)o->order, records);
     else
-        ZB_LAUNCH("k_refine", (k_refine<0>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
+        ZB_LAUNCH("k_refine", (k_refine<0, false>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
                   (const float4*)items, seeds, n_items, (const int*)o->order, records);
     return check_launch("zb_refine");
 }
