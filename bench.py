@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--full-b", action="store_true", help="search every row of b (correlation_symmetry off)")
     ap.add_argument("--no-refine-order", action="store_true", help="refine in batch order (refine_order off)")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
+    ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
+                    help="a launch-shape knob of the library (zb_knob_set), e.g. refine_queue=1 or rings_group=16")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device three times (the known-size kernel the HBM counters are scaled by)")
     args = ap.parse_args()
@@ -311,6 +313,12 @@ def main():
     from zebsdsolve_amd import _native as nat, engine, sym
     from zebsdsolve_amd.geometry import Geometry
 
+    knobs = {}
+    for kv in args.knob:
+        k, v = kv.split("=")
+        if nat.load().zb_knob_set(k.encode(), int(v)) < 0:
+            raise SystemExit(f"--knob {kv}: {nat.last_error()}")
+        knobs[k] = int(v)
     e = engine.Engine()
     # the workload is the path alone: no pattern filters before it
     e.settings.apply(static_background="none", dynamic_background="none")
@@ -493,7 +501,7 @@ def main():
                    "b_half": [bool(e.use_mfma(args.bandwidth, pi) and e.corr_vtable(ph).b_half) for pi, ph in enumerate(phs)],
                    "streams": max(args.lanes, 1), "precision": args.precision, "phases": args.phases, "graphs": bool(args.graphs),
                    "input": "pinned host memory, uploaded per chunk in the timed region" if args.from_host else "resident in HBM",
-                   "pixel": args.pixel},
+                   "pixel": args.pixel, "knobs": knobs},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
                      "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1),
                      "phase_right": round(float(np.mean(right)), 5)},

@@ -235,6 +235,32 @@ int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, co
                   const float* items, const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* opts,
                   float* records, int32_t out0, void* stream);
 
+/* One phase of zb_refine_phases: zb_refine's per-phase arguments. */
+typedef struct zb_refine_phase {
+    const float* master;      /* [2][side][side] f32 */
+    const uint16_t* mgrad_h;  /* the master's half-precision gradient field */
+    int32_t side;
+    const float* seeds;       /* item b's orientation at seeds[b * seed_stride .. + 4] */
+    int32_t seed_stride;
+    const int32_t* order;     /* zb_refine_order's permutation of the items for this phase, or NULL */
+    float* records;           /* [*][8]: item b's record at (out0 + b) * 8 */
+} zb_refine_phase;
+
+/* Bytes of device workspace zb_refine_phases needs (its queue counters). */
+#define ZB_REFINE_QUEUE_BYTES 1024
+
+/* zb_refine for n_phases (1..4) phases over the same items, frame and patterns, in one launch:
+ * a grid of as many blocks as stay resident pulls the n_phases x n_items refinements from
+ * device-side queues (one per XCD, over contiguous runs of the phase-major, per-phase ordered
+ * items), each block building the masked-pixel list once. Every record is bit-identical to
+ * zb_refine's for the same phase (opts.order is ignored: each phase brings its own). work:
+ * ZB_REFINE_QUEUE_BYTES of device memory, cleared by a kernel on `stream` at the start of the
+ * call (calls that overlap on one work buffer race).
+ * Replaces the per-phase loop of Detector.refine dispatches (src/engine.zig:2085-2119). */
+int32_t zb_refine_phases(const zb_refine_phase* phases, int32_t n_phases, const zb_frame* frame, const float* pats,
+                         const float* items, int32_t n_items, const zb_refine_opts* opts, int32_t out0, void* work,
+                         int64_t work_bytes, void* stream);
+
 /* The refinement's item order for zb_refine_opts.order: the n_items items sorted by the master
  * region their detector's centre ray lands in at their seed orientation (seeds[b * seed_stride
  * ..], as zb_refine takes them; items: [n][4] (t, pattern index)), as cells of 32 x 32 a Lambert
@@ -374,6 +400,21 @@ int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* grad_h, void* 
  * for the recorded launches and writes one line "kernel count total_ms" per kernel. */
 int32_t zb_profile(int32_t on);
 int32_t zb_profile_report(char* buf, int32_t cap);
+
+/* Launch-shape knobs, process-wide, read at each launch; none changes any result bit, only how
+ * the work is cut into blocks (the choices the host's tuner makes by timing). zb_knob_set
+ * returns the previous value (ZB_EINVAL for an unknown name or a value out of range),
+ * zb_knob_get the current one.
+ *   "refine_queue" 0..2 (0): zb_refine_phases as one block per work item (0) or as a persistent
+ *                            grid draining device-side queues (1; 2: with the NMI's second pass
+ *                            looking the master up again rather than keeping its values, fewer
+ *                            registers)
+ *   "rings_group"  0..64 (0): ring pairs per block of the sphere stage's ring transforms, a power
+ *                            of two (0: 16 KB of LDS worth; at most n / 2 and 64 KB)
+ * The reference fixes the ring lines a workgroup transforms at TW_LEN / n
+ * (src/spherical.zig:305, 376) and has no counterpart of these. */
+int32_t zb_knob_set(const char* name, int32_t value);
+int32_t zb_knob_get(const char* name);
 
 #ifdef __cplusplus
 }

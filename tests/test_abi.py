@@ -62,6 +62,35 @@ def test_bad_arguments_fail_with_a_message_and_launch_nothing():
     opts = nat.RefineOpts(5, 0.035, 1.0, 0.02, 1e4, 64, 1)  # 64 NMI bins: too many
     rc = lib.zb_refine(None, None, 501, ctypes.byref(nat.Frame()), None, None, None, 8, 4, ctypes.byref(opts), None, 0, None)
     assert rc == -1
+    # zb_refine_phases: phase count, per-phase arguments, workspace size
+    opts = nat.RefineOpts(5, 0.035, 1.0, 0.02, 1e4, 16, 1)
+    fr = nat.Frame()
+    fr.pat_h = fr.pat_w = 60
+    fake = 256  # never dereferenced: every call below fails its checks before any launch
+    ph = (nat.RefinePhase * 5)(*[nat.RefinePhase(fake, fake, 501, fake, 8, None, fake) for _ in range(5)])
+    args = lambda n_ph, work_bytes: (ph, n_ph, ctypes.byref(fr), fake, fake, 4, ctypes.byref(opts), 0, fake, work_bytes, None)
+    assert lib.zb_refine_phases(*args(0, 1024)) == -1 and "bad arguments" in nat.last_error()
+    assert lib.zb_refine_phases(*args(5, 1024)) == -1
+    assert lib.zb_refine_phases(*args(2, 512)) == -2 and "work_bytes" in nat.last_error()
+    ph[1].seed_stride = 3
+    assert lib.zb_refine_phases(*args(2, 1024)) == -1 and "phase 1" in nat.last_error()
+    ph[1].seed_stride = 8
+    ph[1].records = None
+    assert lib.zb_refine_phases(*args(2, 1024)) == -1 and "phase 1: null pointer" in nat.last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
+def test_knobs_set_get_and_reject_unknown_names_and_ranges():
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
+    assert lib.zb_knob_get(b"refine_queue") == 0 and lib.zb_knob_get(b"rings_group") == 0
+    assert lib.zb_knob_set(b"rings_group", 16) == 0
+    assert lib.zb_knob_get(b"rings_group") == 16
+    assert lib.zb_knob_set(b"rings_group", 0) == 16
+    assert lib.zb_knob_set(b"rings_group", 65) == -1 and "outside" in nat.last_error()
+    assert lib.zb_knob_set(b"refine_queue", 3) == -1
+    assert lib.zb_knob_set(b"no_such_knob", 1) == -1 and "no knob" in nat.last_error()
+    assert lib.zb_knob_get(b"no_such_knob") == -1
 
 
 def test_flop_model_reproduces_the_exact_term_counts():

@@ -160,20 +160,33 @@ def test_graph_replays_match_eager_batches():
 def test_tune_climbs_the_knobs_and_keeps_the_results():
     """Solver.tune (python/zebsdsolve/__init__.py tune): it times a sample, leaves knobs within
     their ranges, reports before/after and a summary, stops when progress says so, and the map
-    indexes to the same orientations afterwards (the knobs cut the work, nothing else)."""
+    indexes to the same records afterwards, bit for bit (the knobs cut the work, nothing else).
+    The library's knobs are process-wide: put back to their defaults at the end."""
     s = _solver()
     s.add_phase(NI)
     s.synthesize(toy_rows=16, toy_cols=24, toy_grains=96, toy_counts=400, toy_seed=5)
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
     r0 = s.index()
     calls = []
     out = s.tune(lambda k: calls.append(k) or len(calls) < 5)
-    assert set(out["before"]) == set(out["after"]) == {"step_cells", "spec_budget_mb", "index_lanes"}
+    keys = {"step_cells", "spec_budget_mb", "index_lanes", "refine_queue", "rings_group"}
+    assert set(out["after"]) == keys
     assert len(calls) <= 5 and "tuned on" in out["summary"]
-    assert 256 <= out["after"]["step_cells"] <= 1 << 20 and 64 <= out["after"]["spec_budget_mb"] <= 1 << 16
-    assert 1 <= out["after"]["index_lanes"] <= 4
-    r1 = s.index()
-    q0, q1 = r0["quaternions"].reshape(-1, 4), r1["quaternions"].reshape(-1, 4)
-    assert np.allclose(np.abs(np.sum(q0 * q1, 1)), 1.0, atol=1e-6)
+    try:
+        # a whole run: every knob tried, the library's launch-shape knobs included
+        out = s.tune()
+        a = out["after"]
+        assert set(a) == keys
+        assert 256 <= a["step_cells"] <= 1 << 20 and 64 <= a["spec_budget_mb"] <= 1 << 16 and 1 <= a["index_lanes"] <= 4
+        assert a["refine_queue"] in (0, 1, 2) and a["rings_group"] in (0, 4, 8, 16, 32)
+        assert lib.zb_knob_get(b"refine_queue") == a["refine_queue"]
+        r1 = s.index()
+        for k in ("quaternions", "ncc", "nmi", "phase"):
+            assert np.array_equal(r0[k], r1[k]), k
+    finally:
+        lib.zb_knob_set(b"refine_queue", 0)
+        lib.zb_knob_set(b"rings_group", 0)
 
 
 def test_steps_launched_ahead_match_one_step_and_follow_configure():

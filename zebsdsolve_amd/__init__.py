@@ -19,6 +19,7 @@ import numpy as np
 
 from . import _native
 from . import rot as _rot
+from . import sh
 from . import sym as _sym
 from .api import Result, index
 from .settings import SettingError
@@ -577,7 +578,10 @@ class Solver:
         turn climbs from the value it has, doubled while that is faster, else halved while that
         is: the cells indexed a step (`index_step`, 16 384), the correlation's workspace a
         launch may take (the engine's V budget, 4 GiB) and the streams a step is spread over
-        (`Engine.index_lanes`, 1 to 4). The sample is sized by the bandwidth
+        (`Engine.index_lanes`, 1 to 4); then the library's launch-shape knobs (zb_knob_set,
+        process-wide), each value of a small set tried in turn: the refinement's launch form
+        (`refine_queue` 0, 1, 2) and the ring pairs a block of the sphere stage transforms
+        (`rings_group` 4 to 32, 0 the library's default). The sample is sized by the bandwidth
         (8 192 patterns up to 32, 4 096 up to 64, 1 024 up to 128, 128 beyond) and timed the
         way index_step runs it. A move is tried only when it changes the sample's work (a step
         of at least the sample, or a budget that leaves the chunks as they are, is the same
@@ -638,7 +642,7 @@ class Solver:
                 torch.cuda.synchronize()
                 t = float("inf")
             done += 1
-            self._progress("tune", done, 12)
+            self._progress("tune", done, 18)
             go = progress(done) if progress is not None else True
             return t, go is not False
 
@@ -668,9 +672,37 @@ class Solver:
                         break
                 if moved:
                     break
+        # the library's launch-shape knobs (zb_knob_set): each value of a small set tried in
+        # turn, kept when faster by the same margin; values that give the same launches as the
+        # present one are skipped
+        lib = _native.load()
+        n = sh.fft_size(L)
+
+        def group_of(v):
+            # k_rings' ring pairs a block, as the library derives them from the knob
+            g = v if v > 0 else 2048 // (2 * n)
+            g = 1 << (max(g, 1).bit_length() - 1)
+            return max(1, min(g, 8192 // (2 * n), n // 2))
+
+        choices = {"refine_queue": ((0, 1, 2), lambda v: v), "rings_group": ((0, 4, 8, 16, 32), group_of)}
+        for name, (values, launch) in choices.items():
+            before[name] = lib.zb_knob_get(name.encode())
+            for v in values:
+                if not go:
+                    break
+                cur = lib.zb_knob_get(name.encode())
+                if launch(v) == launch(cur):
+                    continue
+                lib.zb_knob_set(name.encode(), v)
+                t, go = trial()
+                if t < 0.97 * best:
+                    best = t
+                else:
+                    lib.zb_knob_set(name.encode(), cur)
         after = {k: g() for k, (g, _, _, _, _) in knobs.items()}
+        after.update({name: lib.zb_knob_get(name.encode()) for name in choices})
         summary = f"tuned on {ns} patterns at L={L}: {ns / best:.0f} patterns/s in {done} trials; " + \
-                  ", ".join(f"{k} {before[k]} -> {after[k]}" for k in knobs)
+                  ", ".join(f"{k} {before[k]} -> {after[k]}" for k in after)
         self._log(summary)
         return {"before": before, "after": after, "summary": summary}
 

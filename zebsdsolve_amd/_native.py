@@ -38,6 +38,14 @@ class RefineOpts(ctypes.Structure):
                 ("scale_max", ctypes.c_float), ("nmi_bins", i32), ("stamp", ctypes.c_uint32), ("order", vp)]
 
 
+class RefinePhase(ctypes.Structure):  # zb_refine_phase
+    _fields_ = [("master", vp), ("mgrad_h", vp), ("side", i32), ("seeds", vp), ("seed_stride", i32), ("order", vp), ("records", vp)]
+
+
+REFINE_QUEUE_BYTES = 1024  # ZB_REFINE_QUEUE_BYTES
+REFINE_MAX_PHASES = 4
+
+
 class PrepOpts(ctypes.Structure):  # zb_prep_opts
     _fields_ = [("background", ctypes.c_void_p), ("dynamic", i32), ("dynamic_sigma", ctypes.c_float),
                 ("clahe", i32), ("clahe_grid", i32), ("clahe_bins", i32), ("clahe_clip", ctypes.c_float),
@@ -62,6 +70,9 @@ _SIGS = {
     "zb_vcorr_denominators": ([ctypes.POINTER(VTable), ctypes.POINTER(VTable), vp, vp, i32, vp, i64, vp, vp], i32),
     "zb_refine": ([vp, vp, i32, ctypes.POINTER(Frame), vp, vp, vp, i32, i32, ctypes.POINTER(RefineOpts), vp, i32, vp], i32),
     "zb_refine_order": ([ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp], i32),
+    "zb_refine_phases": ([ctypes.POINTER(RefinePhase), i32, ctypes.POINTER(Frame), vp, vp, i32, ctypes.POINTER(RefineOpts), i32, vp, i64, vp], i32),
+    "zb_knob_set": ([ctypes.c_char_p, i32], i32),
+    "zb_knob_get": ([ctypes.c_char_p], i32),
     "zb_local6": ([vp, i32, vp, vp], i32),
     "zb_sums_active": ([vp, i32, ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp, vp], i32),
     "zb_fit6_step": ([vp, vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp], i32),

@@ -66,7 +66,53 @@ void prof_end(hipStream_t s) {
     if (!g_recs.empty() && g_recs.back().b) (void)hipEventRecord(g_recs.back().b, s);
 }
 
+// Launch-shape knobs (the choices an autotuner makes): none changes any result bit.
+struct Knob {
+    const char* name;
+    int lo, hi, def;
+    std::atomic<int> v;
+};
+static Knob g_knobs[] = {
+    {"refine_queue", 0, 2, 0, {0}},  // zb_refine_phases: 0 a block per work item, 1 the persistent queue grid, 2 that
+                                     // grid with the NMI's master values looked up again (fewer registers)
+    {"rings_group", 0, 64, 0, {0}},  // ring pairs per k_rings block (0: 2048 / (2n), at most n / 2)
+};
+static_assert(sizeof g_knobs / sizeof g_knobs[0] == KNOB_COUNT, "a knob per id");
+
+int knob(int id) { return g_knobs[id].v.load(std::memory_order_relaxed); }
+
+static Knob* find_knob(const char* name) {
+    if (!name) return nullptr;
+    for (auto& k : g_knobs)
+        if (!strcmp(k.name, name)) return &k;
+    return nullptr;
+}
+
 }  // namespace zb
+
+// Sets a knob; returns its previous value, or ZB_EINVAL for an unknown name or a value out of
+// range. Process-wide, read at each launch.
+extern "C" int32_t zb_knob_set(const char* name, int32_t value) {
+    zb::Knob* k = zb::find_knob(name);
+    if (!k) {
+        zb::set_error("zb_knob_set: no knob %s", name ? name : "(null)");
+        return ZB_EINVAL;
+    }
+    if (value < k->lo || value > k->hi) {
+        zb::set_error("zb_knob_set: %s = %d outside %d..%d", k->name, value, k->lo, k->hi);
+        return ZB_EINVAL;
+    }
+    return k->v.exchange(value);
+}
+
+extern "C" int32_t zb_knob_get(const char* name) {
+    zb::Knob* k = zb::find_knob(name);
+    if (!k) {
+        zb::set_error("zb_knob_get: no knob %s", name ? name : "(null)");
+        return ZB_EINVAL;
+    }
+    return k->v.load();
+}
 
 extern "C" const char* zb_version(void) { return "0.1.0"; }
 

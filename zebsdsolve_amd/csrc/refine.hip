@@ -8,6 +8,11 @@
 // xor-butterfly over the wave leaves every lane with the same totals (a + b == b + a bit for
 // bit), so every lane takes the same step with no broadcast and no barrier. The master's
 // gradient field holds (grad M, M) per texel: one bilinear lookup reads both.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "common.h"
 
 namespace zb {
@@ -38,6 +43,10 @@ __device__ __forceinline__ float wsum(float v) {
 }
 
 __device__ __forceinline__ float3 wsum3(float3 v) { return make_float3(wsum(v.x), wsum(v.y), wsum(v.z)); }
+
+// A value every lane holds alike, read from the first lane into a scalar register.
+__device__ __forceinline__ float ufloat(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ __forceinline__ float4 uniform4(float4 v) { return make_float4(ufloat(v.x), ufloat(v.y), ufloat(v.z), ufloat(v.w)); }
 
 // The masked pixels of the frame, as pixel indices in row-major order, built by each block into
 // LDS (the same list for every item): the passes then walk only the pixels under the mask, 64 at
@@ -185,6 +194,8 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
     return make_float3(det3(g, m1, m2) / det, det3(m0, g, m2) / det, det3(m0, m1, g) / det);
 }
 
+// One item's refinement, NMI and record (at records[out0 + b]), with the masked-pixel list (LIST)
+// already built and the histogram cleared.
 // The Gauss-Newton passes read the half-precision gradient field (mgh: one 16-byte load per
 // texel row pair, half the vector-memory instructions of float32 texels); the recorded NCC and
 // the NMI are those of the float32 master (ms, the values the field holds in .w) at the best
@@ -192,29 +203,19 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
 // MAXP > 0: each lane keeps the master values of its (at most MAXP) pixels at the best
 // orientation from the first NMI pass for the second; MAXP = 0: the second pass looks them up
 // again.
-// With `order` (zb_refine_order's permutation), block i refines item order[xcd_block(i)]: the
-// blocks of one XCD take a contiguous run of items sorted by the master region they look up, so
-// that their gathers share that XCD's L2. Without it, block i refines item i.
 template <int MAXP, bool LIST>
-__global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict__ ms, const uint2* __restrict__ mgh, const float* __restrict__ pats,
-                                               const float4* __restrict__ items, const float* __restrict__ seeds, int n_items,
-                                               const int* __restrict__ order, float* __restrict__ records) {
+__device__ __forceinline__ void refine_item(const RefArg& a, const float* __restrict__ ms, const uint2* __restrict__ mgh, const float* __restrict__ pats,
+                                            const float4* __restrict__ items, const float* __restrict__ seeds, int seed_stride,
+                                            float* __restrict__ records, int out0, unsigned* hist, const unsigned short* list, int n_px, int b,
+                                            int lane) {
     static_assert(MAXP % kBatch == 0, "MAXP: whole batches");
-    extern __shared__ unsigned lds[];
-    unsigned* hist = lds;                                               // [nmi_bins][nmi_bins]
-    unsigned short* list = LIST ? (unsigned short*)(lds + a.nmi_bins * a.nmi_bins) : nullptr;  // the masked pixels
-    const int b = order ? order[xcd_block(blockIdx.x, gridDim.x)] : (int)blockIdx.x;
-    const int lane = threadIdx.x;
-    for (int i = lane; i < a.nmi_bins * a.nmi_bins; i += 64) hist[i] = 0u;
-    int n_px = a.h * a.w;
-    if constexpr (LIST) n_px = build_list(a, list, lane);
-    else __syncthreads();
     const float inv_w = 1.f / (float)a.w;
-    const float4 it = items[b];
+    // the item's values are the same in every lane: held in scalar registers
+    const float4 it = uniform4(items[b]);
     const float3 t = make_float3(it.x, it.y, it.z);
     const float* pat = pats + (size_t)(int)it.w * a.h * a.w;
-    const float* sd = seeds + (size_t)b * a.seed_stride;
-    float4 cur = make_float4(sd[0], sd[1], sd[2], sd[3]);
+    const float* sd = seeds + (size_t)b * seed_stride;
+    float4 cur = uniform4(make_float4(sd[0], sd[1], sd[2], sd[3]));
 
     // refinement state (detector.wgsl refine_init)
     float4 qbest = cur;
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
         const float h_joint = logf(nn) + hj / nn;
         const float h_marg = 2.f * logf(nn) + hm / nn;
         const float nmi = h_marg > 1e-12f ? 2.f * (h_marg - h_joint) / h_marg : 0.f;
-        float* r = records + (size_t)(a.out0 + b) * 8;
+        float* r = records + (size_t)(out0 + b) * 8;
         r[0] = qbest.x;
         r[1] = qbest.y;
         r[2] = qbest.z;
@@ -401,6 +402,130 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
         r[6] = last_step;
         r[7] = a.stamp;
     }
+}
+
+// One block per item. With `order` (zb_refine_order's permutation), block i refines item
+// order[xcd_block(i)]: the blocks of one XCD take a contiguous run of items sorted by the master
+// region they look up, so that their gathers share that XCD's L2. Without it, block i refines
+// item i.
+template <int MAXP, bool LIST>
+__global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict__ ms, const uint2* __restrict__ mgh, const float* __restrict__ pats,
+                                               const float4* __restrict__ items, const float* __restrict__ seeds, int n_items,
+                                               const int* __restrict__ order, float* __restrict__ records) {
+    extern __shared__ unsigned lds[];
+    unsigned* hist = lds;                                               // [nmi_bins][nmi_bins]
+    unsigned short* list = LIST ? (unsigned short*)(lds + a.nmi_bins * a.nmi_bins) : nullptr;  // the masked pixels
+    const int b = order ? order[xcd_block(blockIdx.x, gridDim.x)] : (int)blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < a.nmi_bins * a.nmi_bins; i += 64) hist[i] = 0u;
+    int n_px = a.h * a.w;
+    if constexpr (LIST) n_px = build_list(a, list, lane);
+    else __syncthreads();
+    refine_item<MAXP, LIST>(a, ms, mgh, pats, items, seeds, a.seed_stride, records, a.out0, hist, list, n_px, b, lane);
+}
+
+// Several phases' refinements of one batch in one launch. The n_phases x n_items work items are
+// taken phase-major (item k of phase p is p * n_items + k, in that phase's order).
+// k_refine_grid: one block per work item, block i taking work item xcd_block(i), so that each XCD
+// takes a contiguous run of the sequence; the hardware deals blocks to CUs as others finish.
+// k_refine_queue: a persistent grid (as many blocks as stay resident) pulling work items from
+// device-side queues: the sequence cut into kQueues contiguous runs, one per XCD, each with its
+// own counter; a block first drains its own XCD's run (block id mod 8, the round-robin dealing
+// xcd_block assumes), then the other runs in turn, and leaves when all are empty, so that every
+// block reaches that exit. Each block builds the masked-pixel list once for all its items.
+// The phases' array pointers are separate restrict kernel arguments (so that the compiler keeps
+// k_refine's register budget: without the no-alias guarantee it holds more values live); their
+// sides, seed strides and orders sit in a table in the work buffer after the queue counters,
+// written with them by k_queue_setup on the same stream before the launch.
+constexpr int kMaxRefinePhases = 4;
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 16;
+struct RefPhase {
+    const int* order;
+    int side, seed_stride;
+};
+struct RefPhases {
+    RefPhase p[kMaxRefinePhases];
+};
+constexpr int kTableOffset = kQueues * kQueueStride * 4;
+static_assert(kTableOffset + sizeof(RefPhases) <= ZB_REFINE_QUEUE_BYTES, "the counters and the phase table fit the work buffer");
+
+// the phases' master, half-precision field, seeds and records, as restrict arguments
+#define ZB_PHASE_ARGS(i) \
+    const float *__restrict__ ms##i, const uint2 *__restrict__ mgh##i, const float *__restrict__ sd##i, float *__restrict__ rec##i
+template <typename T>
+__device__ __forceinline__ T pick(int p, T x0, T x1, T x2, T x3) {
+    return p == 0 ? x0 : p == 1 ? x1 : p == 2 ? x2 : x3;
+}
+
+// Work item w of the sequence: phase p, item b (in that phase's order); then the item refined.
+template <int MAXP, bool LIST>
+__device__ __forceinline__ void refine_work(const RefArg& a, const RefPhase* table, unsigned w, int n_items, ZB_PHASE_ARGS(0), ZB_PHASE_ARGS(1),
+                                            ZB_PHASE_ARGS(2), ZB_PHASE_ARGS(3), const float* __restrict__ pats,
+                                            const float4* __restrict__ items, unsigned* hist, const unsigned short* list, int n_px, int lane) {
+    const int p = __builtin_amdgcn_readfirstlane((int)(w / (unsigned)n_items));
+    const int i = __builtin_amdgcn_readfirstlane((int)(w - (unsigned)p * (unsigned)n_items));
+    const RefPhase& e = table[p];
+    const int* order = e.order;
+    const int b = order ? __builtin_amdgcn_readfirstlane(order[i]) : i;
+    RefArg ap = a;
+    ap.side = __builtin_amdgcn_readfirstlane(e.side);
+    refine_item<MAXP, LIST>(ap, pick(p, ms0, ms1, ms2, ms3), pick(p, mgh0, mgh1, mgh2, mgh3), pats, items, pick(p, sd0, sd1, sd2, sd3),
+                            __builtin_amdgcn_readfirstlane(e.seed_stride), pick(p, rec0, rec1, rec2, rec3), a.out0, hist, list, n_px, b, lane);
+}
+
+template <int MAXP, bool LIST>
+__global__ void __launch_bounds__(64) k_refine_grid(RefArg a, ZB_PHASE_ARGS(0), ZB_PHASE_ARGS(1), ZB_PHASE_ARGS(2), ZB_PHASE_ARGS(3),
+                                                    const float* __restrict__ pats, const float4* __restrict__ items, int n_items,
+                                                    const unsigned* __restrict__ work) {
+    extern __shared__ unsigned lds[];
+    unsigned* hist = lds;
+    unsigned short* list = LIST ? (unsigned short*)(lds + a.nmi_bins * a.nmi_bins) : nullptr;
+    const int lane = threadIdx.x;
+    for (int j = lane; j < a.nmi_bins * a.nmi_bins; j += 64) hist[j] = 0u;
+    int n_px = a.h * a.w;
+    if constexpr (LIST) n_px = build_list(a, list, lane);
+    else __syncthreads();
+    refine_work<MAXP, LIST>(a, (const RefPhase*)((const char*)work + kTableOffset), (unsigned)xcd_block(blockIdx.x, gridDim.x), n_items,
+                            ms0, mgh0, sd0, rec0, ms1, mgh1, sd1, rec1, ms2, mgh2, sd2, rec2, ms3, mgh3, sd3, rec3, pats, items, hist, list, n_px,
+                            lane);
+}
+
+template <int MAXP, bool LIST>
+__global__ void __launch_bounds__(64) k_refine_queue(RefArg a, ZB_PHASE_ARGS(0), ZB_PHASE_ARGS(1), ZB_PHASE_ARGS(2), ZB_PHASE_ARGS(3),
+                                                     int n_phases, const float* __restrict__ pats, const float4* __restrict__ items, int n_items,
+                                                     unsigned* __restrict__ work) {
+    extern __shared__ unsigned lds[];
+    unsigned* hist = lds;
+    unsigned short* list = LIST ? (unsigned short*)(lds + a.nmi_bins * a.nmi_bins) : nullptr;
+    const RefPhase* table = (const RefPhase*)((const char*)work + kTableOffset);
+    const int lane = threadIdx.x;
+    int n_px = a.h * a.w;
+    if constexpr (LIST) n_px = build_list(a, list, lane);
+    const unsigned total = (unsigned)n_items * (unsigned)n_phases;  // below 2^31 (the entry point's check)
+    for (int r = 0; r < kQueues; ++r) {
+        const unsigned x = (blockIdx.x + r) & (kQueues - 1);
+        const unsigned lo = (unsigned)((unsigned long long)total * x / kQueues);
+        const unsigned len = (unsigned)((unsigned long long)total * (x + 1) / kQueues) - lo;
+        for (;;) {
+            unsigned k = 0;
+            if (lane == 0) k = atomicAdd(work + x * kQueueStride, 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (k >= len) break;
+            for (int j = lane; j < a.nmi_bins * a.nmi_bins; j += 64) hist[j] = 0u;
+            __syncthreads();
+            refine_work<MAXP, LIST>(a, table, lo + k, n_items, ms0, mgh0, sd0, rec0, ms1, mgh1, sd1, rec1, ms2, mgh2, sd2, rec2, ms3, mgh3, sd3,
+                                    rec3, pats, items, hist, list, n_px, lane);
+            __syncthreads();  // every lane done with the histogram before the next item clears it
+        }
+    }
+}
+
+// The queue counters cleared and the phase table written into the work buffer.
+__global__ void k_queue_setup(RefPhases ph, int n_phases, unsigned* work) {
+    const int t = threadIdx.x;
+    if (t < kQueues) work[t * kQueueStride] = 0u;
+    if (t < n_phases) ((RefPhase*)((char*)work + kTableOffset))[t] = ph.p[t];
 }
 
 // Cells of the refinement's order: each sheet of the master cut into 32 x 32 Lambert cells
@@ -612,6 +737,17 @@ static int32_t sums_run(const float* mgrad, int32_t side, const zb_frame* frame,
     return check_launch("zb_sums");
 }
 
+static void ref_opts(RefArg& a, const zb_refine_opts* o, int out0) {
+    a.iters = o->iters;
+    a.nmi_bins = o->nmi_bins;
+    a.out0 = out0;
+    a.max_step = o->max_step;
+    a.scale0 = o->scale0;
+    a.scale_min = o->scale_min;
+    a.scale_max = o->scale_max;
+    a.stamp = (float)o->stamp;
+}
+
 extern "C" int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32_t side, const zb_frame* frame, const float* pats, const float* items,
                              const float* seeds, int32_t seed_stride, int32_t n_items, const zb_refine_opts* o, float* records, int32_t out0,
                              void* stream) {
@@ -630,15 +766,8 @@ extern "C" int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32
     // the masked pixels (the list each block builds) and the LDS they take
     const int n_px = ref_frame(frame, side, a, list_bytes, "zb_refine");
     if (n_px < 0) return n_px;
-    a.iters = o->iters;
-    a.nmi_bins = o->nmi_bins;
+    ref_opts(a, o, out0);
     a.seed_stride = seed_stride;
-    a.out0 = out0;
-    a.max_step = o->max_step;
-    a.scale0 = o->scale0;
-    a.scale_min = o->scale_min;
-    a.scale_max = o->scale_max;
-    a.stamp = (float)o->stamp;
     // the joint histogram, then the list (the histogram's size keeps the list 4-byte aligned)
     const size_t lds = (size_t)o->nmi_bins * o->nmi_bins * 4 + list_bytes;
     if (a.use_list && n_px <= 64 * 48)
@@ -651,6 +780,121 @@ extern "C" int32_t zb_refine(const float* master, const uint16_t* mgrad_h, int32
         ZB_LAUNCH("k_refine", (k_refine<0, false>), dim3(n_items), dim3(64), lds, (hipStream_t)stream, a, master, (const uint2*)mgrad_h, pats,
                   (const float4*)items, seeds, n_items, (const int*)o->order, records);
     return check_launch("zb_refine");
+}
+
+// Blocks of k_refine_queue (form `kind`, `lds` bytes of LDS) resident at once on the current
+// device: CUs x blocks per CU, queried once per device, form and LDS size (host queries only, so
+// that a call under stream capture records nothing else).
+static int queue_blocks(int kind, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, size_t>, int> known;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> g(mu);
+    const auto key = std::make_tuple(dev, kind, lds);
+    auto it = known.find(key);
+    if (it != known.end()) return it->second;
+    int cus = 0, per_cu = 0;
+    const void* kern = kind == 0 ? (const void*)k_refine_queue<48, true> : kind == 1 ? (const void*)k_refine_queue<0, true>
+                                                                                       : (const void*)k_refine_queue<0, false>;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds) != hipSuccess || per_cu < 1 || cus < 1)
+        return 0;
+    return known[key] = cus * per_cu;
+}
+
+extern "C" int32_t zb_refine_phases(const zb_refine_phase* phases, int32_t n_phases, const zb_frame* frame, const float* pats, const float* items,
+                                    int32_t n_items, const zb_refine_opts* o, int32_t out0, void* work, int64_t work_bytes, void* stream) {
+    if (!frame || !o || n_phases < 1 || n_phases > kMaxRefinePhases || n_items < 0 || frame->pat_h < 2 || frame->pat_w < 2 || out0 < 0 ||
+        o->iters < 0 || o->nmi_bins < 2 || o->nmi_bins > 32 || (long)n_items * n_phases >= (1L << 31)) {
+        set_error("zb_refine_phases: bad arguments");
+        return ZB_EINVAL;
+    }
+    if (!phases) {
+        set_error("zb_refine_phases: null pointer");
+        return ZB_EINVAL;
+    }
+    RefPhases ph = {};
+    for (int p = 0; p < n_phases; ++p) {
+        const zb_refine_phase& q = phases[p];
+        if (q.side < 2 || q.seed_stride < 4) {
+            set_error("zb_refine_phases: phase %d: bad arguments", p);
+            return ZB_EINVAL;
+        }
+        if (n_items > 0 && (!q.master || !q.mgrad_h || !q.seeds || !q.records)) {
+            set_error("zb_refine_phases: phase %d: null pointer", p);
+            return ZB_EINVAL;
+        }
+        ph.p[p] = RefPhase{q.order, q.side, q.seed_stride};
+    }
+    if (n_items == 0) return ZB_OK;
+    if (!pats || !items || !work) {
+        set_error("zb_refine_phases: null pointer");
+        return ZB_EINVAL;
+    }
+    if (work_bytes < ZB_REFINE_QUEUE_BYTES) {
+        set_error("zb_refine_phases: work_bytes %lld below %d", (long long)work_bytes, ZB_REFINE_QUEUE_BYTES);
+        return ZB_ESPACE;
+    }
+    if ((uintptr_t)work % 8) {
+        set_error("zb_refine_phases: work not 8-byte aligned");
+        return ZB_EINVAL;
+    }
+    RefArg a = {};
+    size_t list_bytes = 0;
+    const int n_px = ref_frame(frame, phases[0].side, a, list_bytes, "zb_refine_phases");
+    if (n_px < 0) return n_px;
+    ref_opts(a, o, out0);
+    const size_t lds = (size_t)o->nmi_bins * o->nmi_bins * 4 + list_bytes;
+    const long total = (long)n_items * n_phases;
+    unsigned* queues = (unsigned*)work;
+    const hipStream_t st = (hipStream_t)stream;
+    // 0: the list and each lane's master values kept between the NMI's passes; 1: the list, the
+    // values looked up again (fewer registers: with the queue knob at 2); 2: the whole frame
+    const int kind = (a.use_list && n_px <= 64 * 48 && knob(KNOB_REFINE_QUEUE) != 2) ? 0 : a.use_list ? 1 : 2;
+    ZB_LAUNCH("k_queue_setup", k_queue_setup, dim3(1), dim3(64), 0, st, ph, n_phases, queues);
+    if (int rc = check_launch("zb_refine_phases/setup")) return rc;
+    // the phases' arrays as kernel arguments (phases past n_phases repeat the last: never read)
+    const float* ms[kMaxRefinePhases];
+    const uint2* mgh[kMaxRefinePhases];
+    const float* sd[kMaxRefinePhases];
+    float* rec[kMaxRefinePhases];
+    for (int p = 0; p < kMaxRefinePhases; ++p) {
+        const zb_refine_phase& q = phases[std::min(p, n_phases - 1)];
+        ms[p] = q.master;
+        mgh[p] = (const uint2*)q.mgrad_h;
+        sd[p] = q.seeds;
+        rec[p] = q.records;
+    }
+#define ZB_PHASES ms[0], mgh[0], sd[0], rec[0], ms[1], mgh[1], sd[1], rec[1], ms[2], mgh[2], sd[2], rec[2], ms[3], mgh[3], sd[3], rec[3]
+    if (knob(KNOB_REFINE_QUEUE)) {
+        // as many blocks as stay resident at once (more would only wait for the queues to empty)
+        const int resident = queue_blocks(kind, lds);
+        if (resident < 1) {
+            set_error("zb_refine_phases: device or occupancy query failed");
+            return ZB_ELAUNCH;
+        }
+        const int grid = (int)std::min<long>(total, resident);
+        if (kind == 0)
+            ZB_LAUNCH("k_refine", (k_refine_queue<48, true>), dim3(grid), dim3(64), lds, st, a, ZB_PHASES, n_phases, pats, (const float4*)items,
+                      n_items, queues);
+        else if (kind == 1)
+            ZB_LAUNCH("k_refine", (k_refine_queue<0, true>), dim3(grid), dim3(64), lds, st, a, ZB_PHASES, n_phases, pats, (const float4*)items,
+                      n_items, queues);
+        else
+            ZB_LAUNCH("k_refine", (k_refine_queue<0, false>), dim3(grid), dim3(64), lds, st, a, ZB_PHASES, n_phases, pats, (const float4*)items,
+                      n_items, queues);
+    } else {
+        if (kind == 0)
+            ZB_LAUNCH("k_refine", (k_refine_grid<48, true>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items, queues);
+        else if (kind == 1)
+            ZB_LAUNCH("k_refine", (k_refine_grid<0, true>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items, queues);
+        else
+            ZB_LAUNCH("k_refine", (k_refine_grid<0, false>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items,
+                      queues);
+    }
+#undef ZB_PHASES
+    return check_launch("zb_refine_phases");
 }
 
 extern "C" int32_t zb_refine_order(const zb_frame* frame, const float* items, const float* seeds, int32_t seed_stride, int32_t n_items,

@@ -469,7 +469,11 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
     if (int rc = check_launch("zb_sh_coefficients/sums")) return rc;
     const int nh = n / 2;
     const int nphi = 2 * n;
-    int G = 2048 / nphi;
+    // ring pairs a block transforms: the knob's (a power of two), else 16 KB of LDS worth; at
+    // most n / 2 and 64 KB of LDS
+    int G = knob(KNOB_RINGS_GROUP) > 0 ? knob(KNOB_RINGS_GROUP) : 2048 / nphi;
+    while (G & (G - 1)) G &= G - 1;
+    if (G > 8192 / nphi) G = 8192 / nphi;
     if (G < 1) G = 1;
     if (G > nh) G = nh;
     ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,

@@ -610,8 +610,9 @@ class Engine:
             records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
             alm = torch.empty((big, pairs, 2), dtype=torch.float32, device=dev)
             stats = torch.empty((big, 4), dtype=torch.float32, device=dev)
-            peaks = torch.empty((big, 8), dtype=torch.float32, device=dev)
-            order = torch.empty((big,), dtype=torch.int32, device=dev)
+            peaks = torch.empty((n_ph, big, 8), dtype=torch.float32, device=dev)
+            order = torch.empty((n_ph, big), dtype=torch.int32, device=dev)
+            qwork = torch.empty((nat.REFINE_QUEUE_BYTES,), dtype=torch.uint8, device=dev)
             if workspace is None or workspace.numel() < need:
                 workspace = torch.empty((need,), dtype=torch.uint8, device=dev)
         else:
@@ -620,8 +621,9 @@ class Engine:
             pats = lane.buf("pats", (raw.shape[0], h, w), torch.float32)
             alm = lane.buf("alm", (big, pairs, 2), torch.float32)
             stats = lane.buf("stats", (big, 4), torch.float32)
-            peaks = lane.buf("peaks", (big, 8), torch.float32)
-            order = lane.buf("order", (big,), torch.int32)
+            peaks = lane.buf("peaks", (n_ph, big, 8), torch.float32)
+            order = lane.buf("order", (n_ph, big), torch.int32)
+            qwork = lane.buf("qwork", (nat.REFINE_QUEUE_BYTES,), torch.uint8)
             workspace = lane.buf("work", (need,), torch.uint8)
             with torch.cuda.stream(lane.stream):
                 records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
@@ -637,7 +639,7 @@ class Engine:
             for pi, ph in enumerate(phs):
                 for c0 in range(0, nbt, chunk):
                     nt = min(chunk, nbt - c0)
-                    a_c, p_c = alm[c0:c0 + nt], peaks[c0:c0 + nt]
+                    a_c, p_c = alm[c0:c0 + nt], peaks[pi, c0:c0 + nt]
                     if mfma[pi]:
                         nat.check(lib.zb_vcorr_correlate(vts[pi], band.tw.data_ptr(), a_c.data_ptr(), nt, rdens[pi].data_ptr(), nt,
                                                          workspace.data_ptr(), workspace.numel(), p_c.data_ptr(), st), "vcorr correlate")
@@ -646,10 +648,17 @@ class Engine:
                                                        workspace.numel(), p_c.data_ptr(), st), "correlate")
                 if s["refine_order"]:
                     # items side by side on an XCD look up nearby regions of the master
-                    nat.check(lib.zb_refine_order(fr, it.data_ptr(), peaks.data_ptr(), 8, nbt, order.data_ptr(), st), "refine order")
-                    opts.order = order.data_ptr()
-                nat.check(lib.zb_refine(sheets[pi].data_ptr(), grads_h[pi].data_ptr(), self.phases[pi].master.side, fr, pats.data_ptr(),
-                                        it.data_ptr(), peaks.data_ptr(), 8, nbt, opts, records[pi].data_ptr(), b0, st), "refine")
+                    nat.check(lib.zb_refine_order(fr, it.data_ptr(), peaks[pi].data_ptr(), 8, nbt, order[pi].data_ptr(), st), "refine order")
+            # every phase's refinement of the batch in one launch (up to four phases a launch; its
+            # form, a block per item or a persistent grid on device-side queues, is the library's
+            # refine_queue knob)
+            for p0 in range(0, n_ph, nat.REFINE_MAX_PHASES):
+                grp = range(p0, min(n_ph, p0 + nat.REFINE_MAX_PHASES))
+                arr = (nat.RefinePhase * len(grp))(*[
+                    nat.RefinePhase(sheets[pi].data_ptr(), grads_h[pi].data_ptr(), self.phases[pi].master.side, peaks[pi].data_ptr(), 8,
+                                    order[pi].data_ptr() if s["refine_order"] else None, records[pi].data_ptr()) for pi in grp])
+                nat.check(lib.zb_refine_phases(arr, len(grp), fr, pats.data_ptr(), it.data_ptr(), nbt, opts, b0, qwork.data_ptr(),
+                                               qwork.numel(), st), "refine phases")
         return records, opts.stamp
 
     def graph(self, n, h, w, dtype=torch.float32, L=None):
