@@ -275,6 +275,10 @@ def main():
     ap.add_argument("--graphs", action="store_true",
                     help="replay each chunk size's batch as a HIP graph captured once (Engine.graph; pays off with equal chunks)")
     ap.add_argument("--lanes", type=int, default=1, help="streams the chunks are spread over (1: one stream)")
+    ap.add_argument("--batch-items", type=int, default=0, help="items a launch of the coefficients and refinement takes (0: the engine's)")
+    ap.add_argument("--spec-mb", type=int, default=0, help="MiB of correlation workspace a launch may take (0: the engine's)")
+    ap.add_argument("--coalesce", type=int, default=1, metavar="K",
+                    help="chunks indexed together as one batch, each stage one launch over all of them (0: every chunk of a step)")
     ap.add_argument("--from-host", action="store_true",
                     help="keep the patterns in pinned host memory and upload each chunk inside the timed region, on the "
                          "stream that indexes it (with --lanes 2 one lane's upload overlaps the other's kernels): the "
@@ -330,6 +334,10 @@ def main():
     e.settings.apply(bandwidth=args.bandwidth, refine_iters=args.iters, correlation_precision=args.precision,
                      correlation_symmetry=not args.full_b, refine_order=not args.no_refine_order)
     e.correlator = args.correlator
+    if args.batch_items:
+        e.batch_items = args.batch_items
+    if args.spec_mb:
+        e.spec_budget = args.spec_mb << 20
     e.set_patterns(np.zeros((1, 1, H, W), np.float32))
     n = args.patterns
     pats, items, truth, truth_phase = make_workload(e, n, args.seed, args.counts, args.p_first if two else 1.0)
@@ -339,6 +347,8 @@ def main():
         pats = pats.round().clamp_(0, 65535).to(torch.int32).to(torch.uint16)
     if args.from_host and args.graphs:
         sys.exit("--from-host uploads each chunk; --graphs replays batches captured on device buffers")
+    if args.coalesce != 1 and (args.from_host or args.graphs):
+        sys.exit("--coalesce takes device-resident chunks, eagerly")
     items_d = torch.from_numpy(items).cuda()
     clo, chi = (int(x) for x in args.chunks.split(","))
     chunks = chunks_of(n, args.seed, clo, chi)
@@ -378,6 +388,15 @@ def main():
 
     def step():
         recs = []
+        if args.coalesce != 1:
+            # uneven chunks as they arrive, K at a time, each group one batch
+            k = args.coalesce if args.coalesce > 0 else len(chunks)
+            for g0 in range(0, len(chunks), k):
+                grp = range(g0, min(len(chunks), g0 + k))
+                lane = lanes[(g0 // k) % len(lanes)]
+                r, _ = e.run_batch([pats[chunks[i][0]:chunks[i][1]] for i in grp], [local[i] for i in grp], args.bandwidth, lane=lane)
+                recs.append(r)
+            return recs
         for i, ((c0, c1), it) in enumerate(zip(chunks, local)):
             if args.graphs:
                 recs.append(graphs[c1 - c0](pats[c0:c1], it).clone())
@@ -499,7 +518,7 @@ def main():
                    "patterns": n, "bandwidth": args.bandwidth, "refine_iters": args.iters, "chunks": len(chunks),
                    "correlator": "mfma" if e.use_mfma(args.bandwidth) else ("slab" if not e.use_mfma(args.bandwidth, 0) else "mfma+slab"),
                    "b_half": [bool(e.use_mfma(args.bandwidth, pi) and e.corr_vtable(ph).b_half) for pi, ph in enumerate(phs)],
-                   "streams": max(args.lanes, 1), "precision": args.precision, "phases": args.phases, "graphs": bool(args.graphs),
+                   "streams": max(args.lanes, 1), "coalesce": args.coalesce, "batch_items": e.batch_items, "spec_mb": e.spec_budget >> 20, "precision": args.precision, "phases": args.phases, "graphs": bool(args.graphs),
                    "input": "pinned host memory, uploaded per chunk in the timed region" if args.from_host else "resident in HBM",
                    "pixel": args.pixel, "knobs": knobs},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),

@@ -170,7 +170,7 @@ def test_tune_climbs_the_knobs_and_keeps_the_results():
     r0 = s.index()
     calls = []
     out = s.tune(lambda k: calls.append(k) or len(calls) < 5)
-    keys = {"step_cells", "spec_budget_mb", "index_lanes", "refine_queue", "rings_group"}
+    keys = {"step_cells", "spec_budget_mb", "index_lanes", "batch_items", "refine_queue", "rings_group"}
     assert set(out["after"]) == keys
     assert len(calls) <= 5 and "tuned on" in out["summary"]
     try:
@@ -179,6 +179,7 @@ def test_tune_climbs_the_knobs_and_keeps_the_results():
         a = out["after"]
         assert set(a) == keys
         assert 256 <= a["step_cells"] <= 1 << 20 and 64 <= a["spec_budget_mb"] <= 1 << 16 and 1 <= a["index_lanes"] <= 4
+        assert 1024 <= a["batch_items"] <= 1 << 16
         assert a["refine_queue"] in (0, 1, 2) and a["rings_group"] in (0, 4, 8, 16, 32)
         assert lib.zb_knob_get(b"refine_queue") == a["refine_queue"]
         r1 = s.index()
@@ -187,6 +188,32 @@ def test_tune_climbs_the_knobs_and_keeps_the_results():
     finally:
         lib.zb_knob_set(b"refine_queue", 0)
         lib.zb_knob_set(b"rings_group", 0)
+
+
+def test_tune_schedule_keeps_a_schedule_that_agrees_with_one_sweep():
+    """Solver.tune_schedule times two-sweep bandwidth schedules against the one sweep on a sample
+    and keeps the fastest that agrees with it: the chosen schedule is in the settings, every
+    schedule is reported with its time, and indexing the map with the chosen schedule accepts
+    as many cells as the one sweep, in the same phase and within 0.1 deg of it."""
+    s = _solver()
+    s.add_phase(NI)
+    s.synthesize(toy_rows=32, toy_cols=48, toy_grains=300, toy_counts=400, toy_seed=19)
+    ref = s.index()
+    sb0 = s.settings["sweep_bandwidth"]
+    out = s.tune_schedule(bandwidths=(24, 32, 48))
+    print(out["summary"], flush=True)
+    sw = [(a, b) for a, b, _, _ in out["schedules"]]
+    assert sw[0] == (1, sb0) and set(sw[1:]) == {(2, 24), (2, 32), (2, 48)}
+    assert all(t > 0 for _, _, t, _ in out["schedules"]) and out["schedules"][0][3]
+    assert (s.settings["sweeps"], s.settings["sweep_bandwidth"]) == out["chosen"]
+    assert "chosen" in out["summary"]
+    got = s.index()
+    acc_r, acc_g = ref["sweep"].reshape(-1) >= 0, got["sweep"].reshape(-1) >= 0
+    assert acc_g.sum() >= acc_r.sum() - 0.005 * acc_r.size
+    both = acc_r & acc_g
+    d = np.degrees(s.disorientation(got["quaternions"].reshape(-1, 4)[both], ref["quaternions"].reshape(-1, 4)[both]))
+    assert np.mean(d > 0.1) <= 0.005
+    s.configure(sweeps=1)
 
 
 def test_steps_launched_ahead_match_one_step_and_follow_configure():

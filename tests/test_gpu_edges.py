@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from tests.conftest import NI, gpu_available
+from tests.conftest import GAN, NI, gpu_available
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
 
@@ -45,6 +45,36 @@ def test_ragged_batches_match_one_batch(precision):
     part = s.index_cells(cells)
     assert np.allclose(part["quaternions"][0], one["quaternions"][0, cells], atol=1e-6, rtol=0)
     assert s.index_cells(np.arange(0))["quaternions"].shape == (1, 0, 4)
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_uneven_chunks_in_one_batch_match_each_chunk_alone(two):
+    """Engine.run_batch over a list of uneven chunks (each with its own patterns and local
+    indices; u16 and f32 pixels) gives each item, bit for bit, the record that chunk gets alone,
+    with one and with two phases, and on a lane's stream."""
+    import torch
+    print(f"uneven chunks, two phases {two}", flush=True)
+    s = _solver()
+    if two:
+        s.add_phase(GAN)
+    s.synthesize(toy_rows=16, toy_cols=24, toy_grains=200, toy_counts=400, toy_seed=23)
+    e = s._e
+    bounds = [0, 1, 34, 35, 200, 331, 384]
+    for dtype in (torch.float32, torch.uint16):
+        chunks, items = [], []
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            cells = np.arange(a, b)
+            raw = e._raw_for(cells).to(dtype) if dtype != torch.float32 else e._raw_for(cells).float()
+            chunks.append(raw.contiguous())
+            items.append(torch.from_numpy(np.concatenate([e.positions(cells), np.arange(b - a, dtype=np.float32)[:, None]], 1)).cuda())
+        alone = [e.run_batch(c, it)[0][..., :7].cpu().numpy() for c, it in zip(chunks, items)]
+        together = e.run_batch(chunks, items)[0][..., :7].cpu().numpy()
+        assert together.shape[1] == 384
+        assert np.array_equal(np.concatenate(alone, 1), together), dtype
+        lane = e.lanes(2)[1]
+        r, _ = e.run_batch(chunks[1:], items[1:], lane=lane)
+        torch.cuda.synchronize()
+        assert np.array_equal(r[..., :7].cpu().numpy(), together[:, 1:]), dtype
 
 
 def test_degenerate_patterns_are_finite_and_not_accepted():

@@ -578,7 +578,8 @@ class Solver:
         turn climbs from the value it has, doubled while that is faster, else halved while that
         is: the cells indexed a step (`index_step`, 16 384), the correlation's workspace a
         launch may take (the engine's V budget, 4 GiB) and the streams a step is spread over
-        (`Engine.index_lanes`, 1 to 4); then the library's launch-shape knobs (zb_knob_set,
+        (`Engine.index_lanes`, 1 to 4), the items a coefficient and refinement launch takes
+        (`Engine.batch_items`, 16 384); then the library's launch-shape knobs (zb_knob_set,
         process-wide), each value of a small set tried in turn: the refinement's launch form
         (`refine_queue` 0, 1, 2) and the ring pairs a block of the sphere stage transforms
         (`rings_group` 4 to 32, 0 the library's default). The sample is sized by the bandwidth
@@ -619,10 +620,15 @@ class Solver:
         def lanes_of(v):
             return tuple(max(1, min(v, k // LANE_MIN_ITEMS)) for k in steps_of(self._step_cells))
 
+        def batches_of(v):
+            # the items a coefficient and refinement launch takes, per step (run_batch's rule)
+            return tuple(max(e.corr_chunk(k, L), min(k, v)) for k in steps_of(self._step_cells))
+
         # name: (get, put, lowest, highest, the work a value gives the sample)
         knobs = {"step_cells": (lambda: self._step_cells, lambda v: setattr(self, "_step_cells", v), 256, 1 << 20, steps_of),
                  "spec_budget_mb": (lambda: e.spec_budget >> 20, lambda v: setattr(e, "spec_budget", v << 20), 64, 1 << 16, chunks_of),
-                 "index_lanes": (lambda: e.index_lanes, lambda v: setattr(e, "index_lanes", v), 1, 4, lanes_of)}
+                 "index_lanes": (lambda: e.index_lanes, lambda v: setattr(e, "index_lanes", v), 1, 4, lanes_of),
+                 "batch_items": (lambda: e.batch_items, lambda v: setattr(e, "batch_items", v), 1024, 1 << 16, batches_of)}
         before = {k: g() for k, (g, _, _, _, _) in knobs.items()}
         done = 0
 
@@ -642,7 +648,7 @@ class Solver:
                 torch.cuda.synchronize()
                 t = float("inf")
             done += 1
-            self._progress("tune", done, 18)
+            self._progress("tune", done, 20)
             go = progress(done) if progress is not None else True
             return t, go is not False
 
@@ -705,6 +711,88 @@ class Solver:
                   ", ".join(f"{k} {before[k]} -> {after[k]}" for k in after)
         self._log(summary)
         return {"before": before, "after": after, "summary": summary}
+
+    def tune_schedule(self, bandwidths=(16, 24, 32, 48), tol_deg=0.1, progress=None):
+        """Pick the bandwidth schedule of the sweeps on a sample of the region: every schedule
+        of two sweeps that starts at one of `bandwidths` below `bandwidth` (the cells the cheap
+        first sweep does not accept go to the second at `bandwidth`) is timed against the one
+        sweep at `bandwidth`, the way index() runs it (src/engine.zig:2426-2465), and the fastest
+        whose results agree with the one sweep's is kept in the settings (`sweeps`,
+        `sweep_bandwidth`): as many cells accepted (at most 0.5 % of the sample fewer), and of
+        the cells both accept, at most 0.5 % in another phase or more than tol_deg apart. Unlike
+        tune(), this changes results (within that agreement), so it is a separate call.
+        Returns {"schedules": [(sweeps, sweep_bandwidth, seconds, agreed)], "chosen": (sweeps,
+        sweep_bandwidth), "summary"}; progress(trials_done) after each trial, False stops."""
+        import time
+
+        import torch
+
+        if not self._e.phases:
+            raise Error("tune_schedule: no phases")
+        s = self.settings
+        L = int(s["bandwidth"])
+        rows, cols = self._e.shape[0], self._e.shape[1]
+        want = 8192 if L <= 32 else 4096 if L <= 64 else 1024 if L <= 128 else 128
+        region = rows * cols
+        ns = min(want, region)
+        # cells spread over the region
+        sample = np.array([(2 * j + 1) * region // (2 * ns) for j in range(ns)], np.int64)
+        before = (int(s["sweeps"]), int(s["sweep_bandwidth"]))
+        self._cancel = False
+
+        def run(sweeps, sb):
+            self.configure(sweeps=sweeps, sweep_bandwidth=sb)
+            self._calibrate()  # chance's NMI, outside the timing (the same for every schedule)
+            self.index_begin()
+            self._todo = sample.copy()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            while self.index_step():
+                pass
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if self._cancel:
+                raise Error("tune_schedule: cancelled")
+            r = self._res
+            return t, {k: r[k][sample].copy() for k in ("quat", "phase", "sweep")}
+
+        cands = [(2, int(b)) for b in sorted(set(bandwidths)) if 8 <= b < L]
+        out, done = [], 0
+        try:
+            run(1, before[1])  # tables at every bandwidth tried, outside the timings
+            for sw, sb in cands:
+                run(sw, sb)
+            t_one, ref = run(1, before[1])
+            out.append((1, before[1], t_one, True))
+            best, chosen = t_one, (1, before[1])
+            acc_ref = ref["sweep"] >= 0
+            for sw, sb in cands:
+                t, got = run(sw, sb)
+                acc = got["sweep"] >= 0
+                both = acc & acc_ref
+                bad = both & (got["phase"] != ref["phase"])
+                same = both & ~bad
+                far = np.zeros(ns, bool)
+                for k in range(len(self._e.phases)):
+                    sel = same & (ref["phase"] == k)
+                    if sel.any():
+                        far[sel] = np.degrees(self.disorientation(got["quat"][sel], ref["quat"][sel], phase=k)) > tol_deg
+                agreed = acc.sum() >= acc_ref.sum() - 0.005 * ns and (bad.sum() + far.sum()) <= 0.005 * max(int(both.sum()), 1)
+                out.append((sw, sb, t, bool(agreed)))
+                if agreed and t < 0.97 * best:
+                    best, chosen = t, (sw, sb)
+                done += 1
+                if progress is not None and progress(done) is False:
+                    break
+        finally:
+            self._res = None
+            self._todo = np.zeros(0, np.int64)
+        self.configure(sweeps=chosen[0], sweep_bandwidth=chosen[1])
+        summary = f"schedule on {ns} patterns at L={L}: " + ", ".join(
+            f"{sw} sweep{'s' if sw > 1 else ''} from {sb if sw > 1 else L}: {ns / t:.0f}/s{'' if ok else ' (disagrees)'}" for sw, sb, t, ok in out) + \
+            f"; chosen {chosen[0]} sweep{'s' if chosen[0] > 1 else ''}" + (f" from {chosen[1]}" if chosen[0] > 1 else "")
+        self._log(summary)
+        return {"schedules": out, "chosen": chosen, "summary": summary}
 
     def profile(self, on=True):
         """Time every kernel launch on the GPU from now on (events around each); returns True

@@ -184,6 +184,8 @@ class Engine:
         # bytes of the correlation's workspace (V) a launch may take: it cuts a batch into
         # correlation chunks (Solver.tune climbs it)
         self.spec_budget = SPEC_BUDGET
+        # items a launch of the coefficients and the refinement takes (Solver.tune climbs it)
+        self.batch_items = BATCH_ITEMS
         self._lanes = []
         # streams a large index_cells request is spread over (1: one; more measured no faster
         # through Solver.index, whose steps each wait for their records)
@@ -566,6 +568,17 @@ class Engine:
         o.bp_fine, o.bp_coarse = float(s["bp_fine"]), float(s["bp_coarse"])  # pixels of the pattern as stored (binning 1)
         nat.check(lib.zb_prep(raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), out.data_ptr(), st), "prep")
 
+    @staticmethod
+    def _pack_items(chunks, items):
+        """The items of several chunks as one [sum n_i, 4] tensor, each pattern index moved past
+        the chunks before its own (on the current stream)."""
+        counts = [int(c.shape[0]) for c in chunks]
+        sizes = torch.tensor([int(it.shape[0]) for it in items], device=items[0].device)
+        starts = torch.tensor(np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.float32), device=items[0].device)
+        out = torch.cat([it.to(torch.float32) for it in items])
+        out[:, 3] += torch.repeat_interleave(starts, sizes)
+        return out
+
     def run_batch(self, raw, items, L=None, workspace=None, lane=None):
         """The pipeline on device-resident inputs: raw [n, h, w] (uint8/uint16/float32 tensor) and
         items [n, 4] float32 (t, index into raw). Returns records [n_phases, n, 8] on the device
@@ -573,14 +586,27 @@ class Engine:
 
         With `lane` (from lanes()), the batch runs on that lane's stream with its buffers, after
         what the current stream has enqueued so far, and the call returns without joining it:
-        the caller synchronizes before reading the records."""
+        the caller synchronizes before reading the records.
+
+        Uneven chunks of a scan as they arrive (lists: raw_i [n_i, h, w], items_i with indices
+        into raw_i) are indexed as one batch, each stage one launch over all of them (the
+        refinement and the coefficients at most batch_items items a launch): the records are
+        [n_phases, sum n_i, 8], chunk after chunk, each the record its item gets alone (no record
+        depends on the batch it is in)."""
         lib = nat.load()
         dev = self._device()
         s = self.settings
         L = int(L or s["bandwidth"])
+        chunks = list(raw) if isinstance(raw, (list, tuple)) else [raw]
+        if isinstance(items, (list, tuple)):
+            if len(items) != len(chunks):
+                raise ValueError("run_batch: one items tensor per chunk")
+            items = self._pack_items(chunks, items)
         n = int(items.shape[0])
-        h, w = int(raw.shape[-2]), int(raw.shape[-1])
-        kind = TORCH_PIXEL_KIND[raw.dtype]
+        h, w = int(chunks[0].shape[-2]), int(chunks[0].shape[-1])
+        if any(c.dtype != chunks[0].dtype or tuple(c.shape[-2:]) != (h, w) for c in chunks):
+            raise ValueError("run_batch: chunks of one pixel type and pattern size")
+        n_raw = sum(int(c.shape[0]) for c in chunks)
         band = self.band(L)
         fr = self.frame()
         mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
@@ -601,12 +627,12 @@ class Engine:
         # that an uneven batch is one launch of each, not one per correlation chunk
         per_item = max(f(4096) for f in corr_ws) // 4096 + 1
         chunk = max(1, min(n, self.spec_budget // max(per_item, 1)))
-        big = max(chunk, min(n, BATCH_ITEMS))
+        big = max(chunk, min(n, self.batch_items))
         pairs = band.host.pairs
         need = max(int(lib.zb_sh_workspace(band.c, big)), max(f(chunk) for f in corr_ws))
         if lane is None:
             st = stream_ptr()
-            pats = torch.empty((raw.shape[0], h, w), dtype=torch.float32, device=dev)
+            pats = torch.empty((n_raw, h, w), dtype=torch.float32, device=dev)
             records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
             alm = torch.empty((big, pairs, 2), dtype=torch.float32, device=dev)
             stats = torch.empty((big, 4), dtype=torch.float32, device=dev)
@@ -618,7 +644,7 @@ class Engine:
         else:
             lane.stream.wait_stream(torch.cuda.current_stream(dev))
             st = lane.stream.cuda_stream
-            pats = lane.buf("pats", (raw.shape[0], h, w), torch.float32)
+            pats = lane.buf("pats", (n_raw, h, w), torch.float32)
             alm = lane.buf("alm", (big, pairs, 2), torch.float32)
             stats = lane.buf("stats", (big, 4), torch.float32)
             peaks = lane.buf("peaks", (n_ph, big, 8), torch.float32)
@@ -627,9 +653,13 @@ class Engine:
             workspace = lane.buf("work", (need,), torch.uint8)
             with torch.cuda.stream(lane.stream):
                 records = torch.empty((n_ph, n, RECORD), dtype=torch.float32, device=dev)
-            raw.record_stream(lane.stream)
+            for c in chunks:
+                c.record_stream(lane.stream)
             items.record_stream(lane.stream)
-        self.prep_into(raw, raw.shape[0], h, w, pats, st)
+        off = 0
+        for c in chunks:
+            self.prep_into(c, int(c.shape[0]), h, w, pats[off:], st)
+            off += int(c.shape[0])
         sh_flags = self.sh_flags()
         for b0 in range(0, n, big):
             nbt = min(big, n - b0)
