@@ -400,6 +400,10 @@ int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* grad_h, void* 
  * for the recorded launches and writes one line "kernel count total_ms" per kernel. */
 int32_t zb_profile(int32_t on);
 int32_t zb_profile_report(char* buf, int32_t cap);
+/* Only the kernels named in `names` (comma-separated, e.g. "k_refine,k_vgemm") are timed while
+ * profiling is on; NULL or "" times every kernel (the default). Fewer events in the stream: the
+ * other launches run as they do unprofiled. No counterpart in the reference. */
+int32_t zb_profile_select(const char* names);
 
 /* Launch-shape knobs, process-wide, read at each launch; none changes any result bit, only how
  * the work is cut into blocks (the choices the host's tuner makes by timing). zb_knob_set

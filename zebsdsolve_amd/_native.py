@@ -89,6 +89,7 @@ _SIGS = {
     "zb_master_pack": ([vp, i32, vp, vp], i32),
     "zb_profile": ([i32], i32),
     "zb_profile_report": ([ctypes.c_char_p, i32], i32),
+    "zb_profile_select": ([ctypes.c_char_p], i32),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -139,6 +140,11 @@ def check(rc, what):
 
 def profile(on):
     load().zb_profile(1 if on else 0)
+
+
+def profile_select(names=None):
+    """Time only the named kernels while profiling is on (None: every kernel)."""
+    load().zb_profile_select(",".join(names).encode() if names else None)
 
 
 def profile_report():

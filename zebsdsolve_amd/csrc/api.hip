@@ -54,6 +54,16 @@ static hipEvent_t take_event() {
 
 bool prof_on() { return g_prof.load(std::memory_order_relaxed); }
 
+static std::vector<std::string> g_prof_only;  // kernels to time (empty: all), under g_prof_mu
+
+bool prof_wants(const char* name) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (g_prof_only.empty()) return true;
+    for (auto& k : g_prof_only)
+        if (k == name) return true;
+    return false;
+}
+
 void prof_begin(const char* name, hipStream_t s) {
     std::lock_guard<std::mutex> g(g_prof_mu);
     ProfRec r{name, take_event(), take_event()};
@@ -127,6 +137,25 @@ extern "C" int32_t zb_profile(int32_t on) {
     }
     zb::g_recs.clear();
     zb::g_prof.store(on != 0);
+    return ZB_OK;
+}
+
+// Only the kernels named in `names` (comma-separated) are timed while profiling is on; NULL or
+// "" times every kernel. Fewer events in the stream: the others run as they do unprofiled.
+extern "C" int32_t zb_profile_select(const char* names) {
+    std::lock_guard<std::mutex> g(zb::g_prof_mu);
+    zb::g_prof_only.clear();
+    if (!names) return ZB_OK;
+    std::string cur;
+    for (const char* c = names;; ++c) {
+        if (*c == ',' || *c == 0) {
+            if (!cur.empty()) zb::g_prof_only.push_back(cur);
+            cur.clear();
+            if (!*c) break;
+        } else if (*c != ' ') {
+            cur += *c;
+        }
+    }
     return ZB_OK;
 }
 

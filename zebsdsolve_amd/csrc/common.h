@@ -16,7 +16,7 @@
 // Launch a kernel, bracketed by timing events when profiling is on.
 #define ZB_LAUNCH(name, kern, grid, block, lds, stream, ...)                      \
     do {                                                                          \
-        const bool zb_p_ = zb::prof_on();                                         \
+        const bool zb_p_ = zb::prof_on() && zb::prof_wants(name);                 \
         if (zb_p_) zb::prof_begin(name, stream);                                  \
         hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);          \
         if (zb_p_) zb::prof_end(stream);                                          \
@@ -34,6 +34,8 @@ enum { KNOB_REFINE_QUEUE = 0, KNOB_RINGS_GROUP = 1, KNOB_COUNT = 2 };
 int knob(int id);
 
 bool prof_on();
+// whether kernel `name` is timed (zb_profile_select's list, all when empty)
+bool prof_wants(const char* name);
 void prof_begin(const char* name, hipStream_t s);
 void prof_end(hipStream_t s);
 
