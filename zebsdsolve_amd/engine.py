@@ -18,6 +18,8 @@ PIXEL_KIND = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32
 TORCH_PIXEL_KIND = {torch.uint8: 0, torch.uint16: 1, torch.float32: 2} if hasattr(torch, "uint16") else {torch.uint8: 0, torch.float32: 2}
 # device memory the correlation's spectra may take per chunk of items
 SPEC_BUDGET = 4 << 30
+# host scans up to this size are kept page-locked as well (Engine._pin)
+PIN_BYTES = 8 << 30
 # items whose coefficients and refinement go in one launch (the correlation takes them in
 # SPEC_BUDGET chunks)
 BATCH_ITEMS = 16384
@@ -223,9 +225,22 @@ class Engine:
             if a.dtype not in PIXEL_KIND:
                 a = a.astype(np.float32)
             self.patterns = np.ascontiguousarray(a)
+        self._pinned = self._pin(self.patterns)
         self.shape = tuple(int(x) for x in self.patterns.shape)
         if self.shape[2] < 4 or self.shape[3] < 4:
             raise ValueError("patterns smaller than 4 x 4")
+
+    @staticmethod
+    def _pin(a):
+        """A host scan (numpy) copied once into page-locked memory, up to PIN_BYTES: the steps'
+        uploads then go by DMA at the link's rate, asynchronously, where a pageable upload is
+        staged by the driver at about a third of it (and waits for the stream). None for a
+        device tensor, without a GPU, or past the limit (then the pageable upload)."""
+        if isinstance(a, torch.Tensor) or a.nbytes > PIN_BYTES or not torch.cuda.is_available():
+            return None
+        t = torch.empty(a.shape, dtype=torch.from_numpy(a[:0]).dtype, pin_memory=True)
+        t.copy_(torch.from_numpy(a))
+        return t
 
     def add_phase(self, path):
         self.phases.append(Phase(master_mod.load(path)))
@@ -704,6 +719,17 @@ class Engine:
         cells = np.asarray(cells, np.int64)
         # a run of consecutive cells (the usual request) is a slice: no gather copy
         run = len(cells) > 0 and cells[-1] - cells[0] == len(cells) - 1 and bool(np.all(np.diff(cells) == 1))
+        pinned = getattr(self, "_pinned", None)
+        if run and pinned is not None:
+            # page-locked and never written again: the upload goes on a copy stream of its own,
+            # beside the kernels of the step before, and the current stream waits for it alone
+            up = self._upload_stream(dev)
+            cur = torch.cuda.current_stream(dev)
+            with torch.cuda.stream(up):
+                out = pinned.reshape(rows * cols, h, w)[int(cells[0]):int(cells[-1]) + 1].to(dev, non_blocking=True)
+            cur.wait_stream(up)
+            out.record_stream(cur)
+            return out
         if isinstance(flat, torch.Tensor):
             part = flat[int(cells[0]):int(cells[-1]) + 1] if run else flat[torch.as_tensor(cells, device=flat.device)]
             return part.to(dev).contiguous()
@@ -753,7 +779,8 @@ class Engine:
     def _items_to_device(self, items, dev):
         """items [n, 4] float32 on the device without the host waiting for the stream's earlier
         kernels (a pageable upload would): through one of two pinned buffers allocated once and
-        used in turn, each refilled only after its previous upload has completed."""
+        used in turn, each refilled only after its previous upload has completed, the upload on
+        the copy stream."""
         if torch.device(dev).type != "cuda":
             return torch.from_numpy(items).to(dev)
         self._stage_k = 1 - getattr(self, "_stage_k", 0)
@@ -765,13 +792,28 @@ class Engine:
             ev.synchronize()
         if buf is None or buf.numel() < flat.size:
             buf = torch.empty((max(flat.size, 1 << 16),), dtype=torch.float32, pin_memory=True)
-        buf[:flat.size].copy_(torch.from_numpy(flat))
-        out = torch.empty(items.shape, dtype=torch.float32, device=dev)
-        out.view(-1).copy_(buf[:flat.size], non_blocking=True)
-        ev = torch.cuda.Event()
-        ev.record()
+        # (a numpy copy: torch's CPU copy hands 64K elements to its thread pool, which measured
+        # 5 ms a call beside the GPU's runtime threads, against 4 us here)
+        buf.numpy()[:flat.size] = flat
+        # on the copy stream (beside the kernels queued before), the current stream waiting for it
+        up = self._upload_stream(dev)
+        cur = torch.cuda.current_stream(dev)
+        with torch.cuda.stream(up):
+            out = torch.empty(items.shape, dtype=torch.float32, device=dev)
+            out.view(-1).copy_(buf[:flat.size], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(up)
+        cur.wait_stream(up)
+        out.record_stream(cur)
         stage[self._stage_k] = [buf, ev]
         return out
+
+    def _upload_stream(self, dev):
+        """The stream host uploads go on, so that they run beside the kernels queued before them."""
+        up = getattr(self, "_upload", None)
+        if up is None or up.device != torch.device(dev):
+            up = self._upload = torch.cuda.Stream(device=dev)
+        return up
 
     def index_cells_finish(self, handle):
         """The records of an index_cells_begin, read back and stamp-checked."""
