@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--full-b", action="store_true", help="search every row of b (correlation_symmetry off)")
     ap.add_argument("--no-refine-order", action="store_true", help="refine in batch order (refine_order off)")
     ap.add_argument("--no-profile", action="store_true", help="no kernel timing events in the timed region")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick the library's launch-shape knobs on the workload first (an untimed step a value)")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="a launch-shape knob of the library (zb_knob_set), e.g. refine_queue=1 or rings_group=16")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -426,6 +428,33 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    tuned = None
+    if args.tune:
+        # the library's launch-shape knobs picked on this workload, one untimed step a value
+        # (each value kept only when 3 % faster than the best so far), as Solver.tune does
+        lib = nat.load()
+
+        def one():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            step()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        best = one()
+        tuned = {}
+        for name, values in (("refine_queue", (0, 1, 2)), ("rings_group", (0, 4, 16))):
+            keep = lib.zb_knob_get(name.encode())
+            for v in values:
+                if v == keep:
+                    continue
+                lib.zb_knob_set(name.encode(), v)
+                t = one()
+                if t < 0.97 * best:
+                    best, keep = t, v
+            lib.zb_knob_set(name.encode(), keep)
+            tuned[name] = keep
+            knobs[name] = keep
     prof_all, dom = {}, None
     if not args.no_profile:
         # one untimed step with every kernel timed (the per-kernel breakdown, and which kernel
@@ -534,7 +563,7 @@ def main():
                    "b_half": [bool(e.use_mfma(args.bandwidth, pi) and e.corr_vtable(ph).b_half) for pi, ph in enumerate(phs)],
                    "streams": max(args.lanes, 1), "coalesce": args.coalesce, "batch_items": e.batch_items, "spec_mb": e.spec_budget >> 20, "precision": args.precision, "phases": args.phases, "graphs": bool(args.graphs),
                    "input": "pinned host memory, uploaded per chunk in the timed region" if args.from_host else "resident in HBM",
-                   "pixel": args.pixel, "knobs": knobs},
+                   "pixel": args.pixel, "knobs": knobs, "tuned": tuned is not None},
         "accuracy": {"median_misorientation_deg": round(float(np.median(dis)), 4), "p99_deg": round(float(np.percentile(dis, 99)), 4),
                      "mean_ncc": round(float(rec[:, 4].mean()), 4), "meets_0.1deg": bool(np.median(dis) <= 0.1),
                      "phase_right": round(float(np.mean(right)), 5)},
