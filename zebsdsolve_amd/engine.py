@@ -225,6 +225,12 @@ class Engine:
             if a.dtype not in PIXEL_KIND:
                 a = a.astype(np.float32)
             self.patterns = np.ascontiguousarray(a)
+        if getattr(self, "_pinned", None) is not None:
+            # the previous scan's page-locked copy goes back to the system, not to torch's cache
+            self._pinned = None
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+                torch._C._host_emptyCache()
         self._pinned = self._pin(self.patterns)
         self.shape = tuple(int(x) for x in self.patterns.shape)
         if self.shape[2] < 4 or self.shape[3] < 4:
