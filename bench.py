@@ -510,7 +510,7 @@ def main():
             sec = per_launch_ms * 1e-3
             tf = fpp.get(k, 0.0) * ppl / sec / 1e12
             if k in MFMA_KERNELS:
-                peak, unit = (LP_PEAK_TFLOPS, "mfma-16bit") if (lp and k == "k_vgemm") else (FP32_PEAK_TFLOPS, "mfma")
+                peak, unit = (LP_PEAK_TFLOPS, "mfma-16bit") if (lp and k in ("k_vgemm", "k_legendre")) else (FP32_PEAK_TFLOPS, "mfma")
             else:
                 peak, unit = VALU_PEAK_TFLOPS, "valu"
             return {"bound": "mfma", "unit_kind": unit, "achieved": round(tf, 3), "peak": peak, "unit": "TFLOP/s",
