@@ -265,6 +265,20 @@ def test_chance_is_measured_again_after_what_it_depends_on_changes():
     assert lines[0].endswith("dispatches  total") and any("k_refine" in ln for ln in lines[1:])
     ms = [float(ln.split()[0]) for ln in lines[1:]]
     assert ms == sorted(ms, reverse=True)
+    # zb_profile_select: only the named kernels are timed, then every kernel again
+    from zebsdsolve_amd import _native as nat
+    nat.profile_select(["k_refine", "k_vgemm"])
+    nat.profile(True)
+    s.index_cells(np.arange(48))
+    only = nat.profile_report()
+    nat.profile(False)
+    nat.profile_select(None)
+    assert set(only) == {"k_refine", "k_vgemm"}, only
+    nat.profile(True)
+    s.index_cells(np.arange(48))
+    every = nat.profile_report()
+    nat.profile(False)
+    assert {"k_refine", "k_vgemm", "k_apeak", "k_rings"} <= set(every)
     c0 = s.chance(0)
     s.configure(dynamic_background="subtract")
     assert s._chance is None
