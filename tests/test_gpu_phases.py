@@ -104,3 +104,20 @@ def test_two_phase_scan_records_equal_one_phase_solvers():
     for pi, path in enumerate((NI, GAN)):
         one = _solver(path, patterns=pats)._e.index_cells(cells)
         assert np.array_equal(both[pi][:, :7], one[0][:, :7]), pi
+
+
+def test_five_phases_go_in_two_launches_with_the_same_records():
+    """Past four phases the engine launches the refinement in groups of four: five phases (the
+    two masters alternating) give each phase the records a one-phase solver gives it."""
+    print("five phases", flush=True)
+    s = _solver(NI, GAN, NI, GAN, NI)
+    s.synthesize(toy_rows=4, toy_cols=6, toy_grains=24, toy_counts=400, toy_seed=31)
+    cells = np.arange(24)
+    five = s._e.index_cells(cells)
+    assert five.shape[0] == 5
+    pats = np.asarray(s._e.patterns)
+    ni = _solver(NI, patterns=pats)._e.index_cells(cells)[0]
+    gan = _solver(GAN, patterns=pats)._e.index_cells(cells)[0]
+    for pi in range(5):
+        want = ni if pi % 2 == 0 else gan
+        assert np.array_equal(five[pi][:, :7], want[:, :7]), pi
