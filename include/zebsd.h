@@ -36,7 +36,7 @@ extern "C" {
 #define ZB_FLAG_NORM 2   /* scale to unit harmonic norm */
 #define ZB_FLAG_WINDOW 4 /* project the window itself (ones) instead of a pattern */
 /* The Legendre sums with 16-bit products and FP32 accumulation (the SH-analysis GEMM of the
- * precision form; this build's own): FP16 or BF16 on the matrix cores, each value as
+ * precision form; no counterpart in the reference): FP16 or BF16 on the matrix cores, each value as
  * hi = round16(x) and lo = round16(x - hi), hi lo + lo hi added for the degrees from the split
  * degree (bits 16..23 of the flags: ZB_FLAG_SPLIT(l)) on. Below n = 32 the FP32 sums are used. */
 #define ZB_FLAG_FP16 8

@@ -1,7 +1,7 @@
 """The oracle: a float64 CPU restatement of the reference's indexing path, for tests only.
 
 TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import it, and only as the checker (or the timed CPU baseline): the product path
+import it, and only to check results (or as the timed CPU baseline): the product path
 (zebsdsolve_amd/) never calls it and has no CPU fallback.
 
 Where it is cheap the oracle takes a different route to the same numbers than the product, so

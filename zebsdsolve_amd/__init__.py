@@ -150,7 +150,7 @@ class Solver:
                 ebsd = root[scans[0]]["EBSD"]
                 hdr, data = ebsd["Header"], ebsd["Data"]
                 if "pcx" not in hdr or "patterns" not in data:
-                    raise Error(f"open: {path}: an h5ebsd file this build does not read (kikuchipy's is read)")
+                    raise Error(f"open: {path}: an h5ebsd layout this reader does not take (kikuchipy's is read)")
 
                 def num(g, k, default=np.nan):
                     return float(np.asarray(g[k].read()).reshape(-1)[0]) if k in g else default
@@ -369,7 +369,7 @@ class Solver:
         not reduced), ncc, nmi, step (radians of the last improving refinement step) and z (NMI
         above chance in chance's standard deviations) (python/zebsdsolve/__init__.py:666-683)."""
         if method not in (None, "spherical"):
-            raise Error(f"index_cells: method {method!r} is outside this build (spherical only)")
+            raise Error(f"index_cells: method {method!r} is not implemented here (spherical only)")
         if not self._e.phases:
             raise Error("index_cells: no phases")
         return self._out_of(self._e.index_cells(np.asarray(cells, np.int64).reshape(-1), bandwidth))
@@ -524,7 +524,7 @@ class Solver:
     def sweep_plan(self):
         """What each sweep searches with, in order: a list of (bandwidth, dictionary grid,
         method) (python/zebsdsolve/__init__.py:748-755; src/settings.zig:465-483). The
-        dictionary methods are outside this build, so the grid is 0 and the method spherical."""
+        dictionary methods are not implemented here, so the grid is 0 and the method spherical."""
         return [(int(self._sweep_bandwidth(i)), 0, "spherical") for i in range(int(self.settings["sweeps"]))]
 
     def index(self, progress=None):
@@ -574,7 +574,7 @@ class Solver:
 
     def tune(self, progress=None):
         """Fit how the work is cut up to this machine and this scan (python/zebsdsolve/__init__.py
-        tune): a sample of the region is indexed and timed, and each of this build's knobs in
+        tune): a sample of the region is indexed and timed, and each of the engine's knobs in
         turn climbs from the value it has, doubled while that is faster, else halved while that
         is: the cells indexed a step (`index_step`, 16 384), the correlation's workspace a
         launch may take (the engine's V budget, 4 GiB) and the streams a step is spread over

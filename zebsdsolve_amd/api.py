@@ -36,7 +36,7 @@ def _open(solver, scan, phases, kwargs):
     solver.configure(**kwargs)
     if isinstance(scan, (str, bytes)) or hasattr(scan, "__fspath__"):
         from . import Error
-        raise Error("scan files are outside this build: pass the patterns as an array")
+        raise Error("scan files are not read by ze.index here: pass the patterns as an array (Solver.open reads kikuchipy's h5ebsd)")
     solver.set_patterns(scan)
     if geo:
         solver.set_geometry(**geo)
@@ -45,11 +45,11 @@ def _open(solver, scan, phases, kwargs):
 
 
 def index(scan, phases, fit=None, progress=None, **kwargs):
-    """Index a scan against candidate phases. fit: None only (the geometry fits are outside this
-    build). progress(status) is called after each step; returning False stops early."""
+    """Index a scan against candidate phases. fit: None only (the geometry fits are not
+    implemented here). progress(status) is called after each step; returning False stops early."""
     from . import Error, Solver
     if fit is not None:
-        raise Error("fit: the pattern-centre and plane fits are outside this build")
+        raise Error("fit: the pattern-centre and plane fits are not implemented here")
     with Solver() as s:
         _open(s, scan, phases, kwargs)
         r = s.index(progress)

@@ -240,7 +240,7 @@ class Engine:
     def _pin(a):
         """A host scan (numpy) copied once into page-locked memory, up to PIN_BYTES: the steps'
         uploads then go by DMA at the link's rate, asynchronously, where a pageable upload is
-        staged by the driver at about a third of it (and waits for the stream). None for a
+        staged by the HIP runtime at about a third of it (and waits for the stream). None for a
         device tensor, without a GPU, or past the limit (then the pageable upload)."""
         if isinstance(a, torch.Tensor) or a.nbytes > PIN_BYTES or not torch.cuda.is_available():
             return None

@@ -1,7 +1,7 @@
 """Options of the indexing path, by the reference's names and defaults (src/settings.zig).
 
 `apply` rejects unknown keys and values out of range atomically, as Settings.apply/check does
-(src/settings.zig:404-463). Keys of the reference that belong to stages outside this build
+(src/settings.zig:404-463). Keys of the reference that belong to stages not implemented here
 (NLPAR, dictionary and PCA search, binning, the fits) are accepted only at the values that leave
 those stages out, and say so otherwise.
 """
@@ -17,7 +17,7 @@ DEFAULTS = {
     "scan_x_sign": -1, "scan_y_sign": -1,
     # ── resolution ──
     "binning": 1.0, "full_resolution": "none",
-    # ── averaging and pattern filters, in the order applied (NLPAR is outside this build: only
+    # ── averaging and pattern filters, in the order applied (NLPAR is not implemented: only
     # its "off" value) ──
     "nlpar": False, "static_background": "average", "static_patterns": 1024, "dynamic_background": "subtract",
     "dynamic_sigma": 0.15, "clahe": False, "clahe_grid": 4, "clahe_bins": 64, "clahe_clip": 40.0,
@@ -30,12 +30,12 @@ DEFAULTS = {
     "bandwidth": 64, "method": "spherical",
     # ── refinement ──
     "refine": True, "refine_iters": 8, "refine_step": 2.0,
-    # (this build's own) refine a batch's items in the order of the master region they look up,
+    # (not in the reference) refine a batch's items in the order of the master region they look up,
     # so that items refined side by side share the caches; the records are the same either way
     "refine_order": True,
     # ── phases ──
     "phase_by": "nmi", "nmi_bins": 16,
-    # ── the correlation's GEMM (this build's own; the reference computes in float32): "fp32",
+    # ── the correlation's GEMM (not in the reference, which computes in float32): "fp32",
     # or "fp16" / "bf16": 16-bit products with float32 accumulation (hi + lo products for the
     # degrees from split_degree on) and the GEMM's output kept in the same 16-bit type, the
     # transform along a in float32; phases with fewer than 32 points along c stay "fp32" ──
@@ -50,7 +50,7 @@ DEFAULTS = {
     "sweeps": 1, "sweep_bandwidth": 32, "sweep_method": "same", "accept_nmi": 0.0, "accept_z": 15.0, "accept_step_deg": 0.2,
 }
 
-# values a key may take where this build has only some of the reference's
+# values a key may take where only some of the reference's are implemented
 _ONLY = {
     "nlpar": (False,), "static_background": ("average", "file", "none"), "dynamic_background": ("subtract", "divide", "none"),
     "method": ("spherical",), "binning": (1, 1.0), "full_resolution": ("none",),
@@ -96,7 +96,7 @@ def check(s):
         bad(not (16 <= s[k] <= 256 and (s[k] & (s[k] - 1)) == 0), k)
     for k, allowed in _ONLY.items():
         if s[k] not in allowed:
-            raise SettingError(f"setting {k}={s[k]!r} selects a stage outside this build (supported: {allowed[0]!r})")
+            raise SettingError(f"setting {k}={s[k]!r} selects a stage not implemented here (supported: {allowed[0]!r})")
 
 
 class Settings:
