@@ -227,13 +227,11 @@ def cpu_baseline(e, pats_host, items, gpu_rec, truth, L, iters, slice_n=10_000, 
 
 
 def traffic_from_profiles(kernel, patterns_per_launch, workload="L64"):
-    """HBM bytes per launch of `kernel`: the committed rocprofv3 --pmc summary of the bench's
-    workload (profiles/<kernel>_L64.txt, or <kernel>_pmc.txt for the fit's kernel; lines
+    """HBM bytes per launch of `kernel`: the committed rocprofv3 --pmc summary of the workload
+    (profiles/<kernel>_L64.txt for the bench's, <kernel>_fit80.txt for the fit's; lines
     'FETCH_bytes_per_pattern <v>' and 'WRITE_bytes_per_pattern <v>', already scaled by the
     copy-kernel factor) times the patterns one launch takes here; None without the summary."""
     p = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.txt")
-    if not os.path.exists(p):
-        p = os.path.join(ROOT, "profiles", f"{kernel}_pmc.txt")
     if not os.path.exists(p):
         return None
     vals = {}
@@ -624,7 +622,7 @@ def bench_fit(args, e, pats, items, truth):
         tf = 120.0 * n_mask * ppl / (ms / cnt * 1e-3) / 1e12
         roofline = {"bound": "mfma", "unit_kind": "valu", "achieved": round(tf, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(tf / VALU_PEAK_TFLOPS, 4),
-                    "traffic": traffic_from_profiles("k_sums6", ppl), "kernel": "k_sums6", "launch_ms": round(ms / cnt, 4),
+                    "traffic": traffic_from_profiles("k_sums6", ppl, "fit80"), "kernel": "k_sums6", "launch_ms": round(ms / cnt, 4),
                     "patterns_per_launch": round(ppl, 1)}
     out = {
         "metric": "patterns/s fitted to convergence (six-direction Levenberg-Marquardt); % of roofline",
