@@ -356,6 +356,20 @@ typedef struct zb_prep_opts {
 int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* opts,
                 float* out, void* stream);
 
+/* Bytes of workspace zb_prep_work needs for n patterns h x w with these filters: 0 where one
+ * block's LDS holds the image and its scratch (zb_prep's case), else a slot of image and
+ * scratch for each of up to 1 024 blocks. */
+int64_t zb_prep_workspace(int32_t n, int32_t h, int32_t w, const zb_prep_opts* opts);
+
+/* zb_prep for patterns of any size (the reference's 240 x 320 frames with its default filters):
+ * where LDS holds the image, zb_prep itself; past it, the same arithmetic in the same order over
+ * a slot of global memory a block (work: zb_prep_workspace bytes; a smaller one takes fewer
+ * blocks, down to one slot), each block taking patterns in turn. The results are bit-identical
+ * to zb_prep's where both run (the knob "prep_global" forces the global form). Replaces the
+ * same Prep.run + Prep.filter as zb_prep. */
+int32_t zb_prep_work(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* opts,
+                     void* work, int64_t work_bytes, float* out, void* stream);
+
 /* CLAHE of n h x w float images in place, in global memory (for images larger than zb_prep's
  * LDS: a master's two sheets): each image stretched to [0, 1] by its extremes, then equalized
  * over grid x grid tiles (1..64) of bins levels (2..256) with the histograms clipped at clip x
@@ -415,6 +429,8 @@ int32_t zb_profile_select(const char* names);
  *                            registers)
  *   "rings_group"  0..64 (0): ring pairs per block of the sphere stage's ring transforms, a power
  *                            of two (0: 16 KB of LDS worth; at most n / 2 and 64 KB)
+ *   "prep_global"  0..1 (0): zb_prep_work's filters through global memory even where LDS holds
+ *                            the image (1)
  * The reference fixes the ring lines a workgroup transforms at TW_LEN / n
  * (src/spherical.zig:305, 376) and has no counterpart of these. */
 int32_t zb_knob_set(const char* name, int32_t value);

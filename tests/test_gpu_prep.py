@@ -153,3 +153,124 @@ def test_master_symmetrize_is_the_point_group_mean():
     assert np.abs(got[tid] - want).max() < 1e-4, np.abs(got[tid] - want).max()
     assert np.sqrt(np.mean((got - smooth.reshape(-1)) ** 2)) < 0.1
     assert lib.zb_master_symmetrize(src.data_ptr(), src.data_ptr(), side, ops.data_ptr(), len(m.ops), None) < 0
+
+
+def _prep_call(lib, nat, t_raw, kind, n, h, w, mask, o, out, work=True):
+    from zebsdsolve_amd import engine
+    import torch
+    need = int(lib.zb_prep_workspace(n, h, w, nat.ctypes.byref(o)))
+    assert need >= 0, nat.last_error()
+    ws = torch.empty((max(need, 1),), dtype=torch.uint8, device="cuda") if work else None
+    nat.check(lib.zb_prep_work(t_raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), ws.data_ptr() if work else None,
+                               need if work else 0, out.data_ptr(), engine.stream_ptr()), "prep work")
+    return need
+
+
+@pytest.mark.parametrize("h,w", [(60, 60), (120, 120), (45, 70)])
+def test_prep_through_global_memory_equals_the_lds_form(h, w):
+    """zb_prep_work with the filters forced through global memory (knob prep_global) gives every
+    pixel of zb_prep's LDS form, bit for bit, for every filter: static and dynamic backgrounds
+    (both blur paths), CLAHE, the band pass with and without its fine Gaussian."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import _native as nat, engine
+    print(f"prep global {h}x{w}", flush=True)
+    n = 6
+    raw = _patterns(n, h, w, seed=7 + h).astype(np.float32)
+    bg = torch.from_numpy(raw.astype(np.float64).mean(0).astype(np.float32)).cuda()
+    fr = op.frame(np.eye(3), h, w)
+    mask = (nat.ctypes.c_float * 3)(*[float(x) for x in fr["mask"]])
+    lib = nat.load()
+    t_raw = torch.from_numpy(raw).cuda()
+    combos = [dict(dynamic=1, dynamic_sigma=0.15 * w), dict(dynamic=2, dynamic_sigma=0.15 * w, clahe=1, clahe_grid=4, clahe_bins=64, clahe_clip=2.0),
+              dict(dynamic=1, dynamic_sigma=4.0, bandpass=1, bp_fine=1.5, bp_coarse=0.2 * w), dict(bandpass=1, bp_fine=0.0, bp_coarse=13.0)]
+    prev = lib.zb_knob_set(b"prep_global", 0)
+    tested = 0
+    try:
+        for kw in combos:
+            o = nat.PrepOpts()
+            o.background = bg.data_ptr()
+            for k, v in kw.items():
+                setattr(o, k, v)
+            a = torch.empty((n, h, w), dtype=torch.float32, device="cuda")
+            b = torch.full((n, h, w), np.nan, dtype=torch.float32, device="cuda")
+            if int(lib.zb_prep_workspace(n, h, w, nat.ctypes.byref(o))) > 0:
+                continue  # past one block's LDS with these filters: no LDS form to hold it to
+            tested += 1
+            nat.check(lib.zb_prep(t_raw.data_ptr(), 2, n, h, w, mask, nat.ctypes.byref(o), a.data_ptr(), engine.stream_ptr()), "prep")
+            lib.zb_knob_set(b"prep_global", 1)
+            need = _prep_call(lib, nat, t_raw, 2, n, h, w, mask, o, b)
+            lib.zb_knob_set(b"prep_global", 0)
+            assert need > 0
+            assert torch.equal(a, b), (kw, (a - b).abs().max().item())
+    finally:
+        lib.zb_knob_set(b"prep_global", prev)
+    assert tested >= 3
+
+
+def test_prep_of_240x320_patterns_matches_the_oracle():
+    """240 x 320 patterns (the reference's large frames; 300 KB an image, past one block's LDS)
+    through the default filters (the scan's average subtracted, the dynamic background of
+    sigma 48 px subtracted by the box path) and the band pass: zb_prep refuses them, zb_prep_work
+    takes them through global memory and matches the float64 oracle; a workspace of one slot
+    (one block) gives the same pixels, and less than a slot is refused."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import _native as nat, engine
+    print("prep 240x320", flush=True)
+    n, h, w = 3, 240, 320
+    raw = _patterns(n, h, w, seed=5).astype(np.uint16)
+    bg64 = raw.astype(np.float64).mean(0)
+    bg = torch.from_numpy(bg64.astype(np.float32)).cuda()
+    fr = op.frame(np.eye(3), h, w)
+    mask = (nat.ctypes.c_float * 3)(*[float(x) for x in fr["mask"]])
+    lib = nat.load()
+    t_raw = torch.from_numpy(raw.astype(np.int32)).cuda().to(torch.uint16)
+    o = nat.PrepOpts()
+    o.background = bg.data_ptr()
+    o.dynamic, o.dynamic_sigma = 1, 0.15 * w
+    out = torch.empty((n, h, w), dtype=torch.float32, device="cuda")
+    assert lib.zb_prep(t_raw.data_ptr(), 1, n, h, w, mask, nat.ctypes.byref(o), out.data_ptr(), engine.stream_ptr()) != 0
+    assert "zb_prep_work" in nat.last_error()
+    _prep_call(lib, nat, t_raw, 1, n, h, w, mask, o, out)
+    got = out.cpu().numpy()
+    for b in range(n):
+        ref = op.prep(raw[b], fr["mask"], background=bg64.astype(np.float32).astype(np.float64), dynamic="subtract", dynamic_sigma=0.15)
+        err = np.abs(got[b] - ref).max() / np.abs(ref).max()
+        assert err < 2e-5, (b, err)
+    # one slot: one block takes the patterns in turn
+    one = torch.full_like(out, np.nan)
+    need = int(lib.zb_prep_workspace(n, h, w, nat.ctypes.byref(o)))
+    slot = need // n
+    ws = torch.empty((slot,), dtype=torch.uint8, device="cuda")
+    nat.check(lib.zb_prep_work(t_raw.data_ptr(), 1, n, h, w, mask, nat.ctypes.byref(o), ws.data_ptr(), slot, one.data_ptr(),
+                               engine.stream_ptr()), "prep one slot")
+    assert torch.equal(one, out)
+    assert lib.zb_prep_work(t_raw.data_ptr(), 1, n, h, w, mask, nat.ctypes.byref(o), ws.data_ptr(), slot - 256, one.data_ptr(),
+                            engine.stream_ptr()) == -2
+    # the band pass at this size (its coarse Gaussian by boxes of 6 pixels); CLAHE's global form
+    # is held to the LDS form's pixels above
+    o.bandpass, o.bp_fine, o.bp_coarse = 1, 2.0, 40.0
+    _prep_call(lib, nat, t_raw, 1, n, h, w, mask, o, out)
+    got = out.cpu().numpy()
+    for b in range(n):
+        ref = op.prep(raw[b], fr["mask"], background=bg64.astype(np.float32).astype(np.float64), dynamic="subtract", dynamic_sigma=0.15,
+                      bandpass=(2.0, 40.0))
+        err = np.abs(got[b] - ref).max() / np.abs(ref).max()
+        assert err < 2e-5, (b, err)
+
+
+def test_240x320_toy_scan_through_the_default_filters():
+    """A toy scan of 240 x 320 patterns with the reference's default filters (the map's average
+    and the dynamic background subtracted, through global memory) is recovered to the
+    reference's thresholds (median < 0.15 deg, max < 1 deg)."""
+    import zebsdsolve_amd as ze
+    print("240 x 320 toy scan, default filters", flush=True)
+    s = ze.Solver()
+    s.set_geometry(pc=(0.44, 0.24, 0.53), sample_tilt_deg=70, detector_tilt_deg=0, px_um=70 * 8, step_x_um=20, step_y_um=20)
+    s.add_phase(__import__("tests.conftest", fromlist=["NI"]).NI)
+    truth = s.synthesize(toy_rows=4, toy_cols=6, toy_grains=24, toy_counts=400, toy_seed=17, toy_height=240, toy_width=320)
+    assert s.settings["static_background"] == "average" and s.settings["dynamic_background"] == "subtract"
+    r = s.index()
+    d = np.degrees(s.disorientation(r["quaternions"].reshape(-1, 4), truth["quaternions"].reshape(-1, 4)))
+    assert np.median(d) < 0.15 and d.max() < 1.0, (np.median(d), d.max())

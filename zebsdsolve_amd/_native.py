@@ -57,6 +57,8 @@ _SIGS = {
     "zb_last_error": ([ctypes.c_char_p, i32], i32),
     "zb_normalize": ([vp, i32, i32, i32, i32, f32p, vp, vp], i32),
     "zb_prep": ([vp, i32, i32, i32, i32, f32p, ctypes.POINTER(PrepOpts), vp, vp], i32),
+    "zb_prep_workspace": ([i32, i32, i32, ctypes.POINTER(PrepOpts)], i64),
+    "zb_prep_work": ([vp, i32, i32, i32, i32, f32p, ctypes.POINTER(PrepOpts), vp, i64, vp, vp], i32),
     "zb_sh_workspace": ([ctypes.POINTER(Band), i32], i64),
     "zb_sh_coefficients": ([ctypes.POINTER(Band), ctypes.POINTER(Frame), vp, vp, i32, i32, vp, i64, vp, vp, vp], i32),
     "zb_sh_coefficients_noalloc": ([ctypes.POINTER(Band), ctypes.POINTER(Frame), vp, vp, i32, i32, vp, vp, vp], i32),

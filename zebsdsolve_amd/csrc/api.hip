@@ -86,6 +86,7 @@ static Knob g_knobs[] = {
     {"refine_queue", 0, 2, 0, {0}},  // zb_refine_phases: 0 a block per work item, 1 the persistent queue grid, 2 that
                                      // grid with the NMI's master values looked up again (fewer registers)
     {"rings_group", 0, 64, 0, {0}},  // ring pairs per k_rings block (0: 2048 / (2n), at most n / 2)
+    {"prep_global", 0, 1, 0, {0}},   // zb_prep_work: 1 the filters through global memory even where LDS holds the image
 };
 static_assert(sizeof g_knobs / sizeof g_knobs[0] == KNOB_COUNT, "a knob per id");
 

@@ -30,7 +30,7 @@ int check_launch(const char* what);
 // With profiling on, prof_begin records an event on `s` before a launch and prof_end one after
 // it, under the kernel's name; zb_profile_report sums them per name.
 // The launch-shape knobs (zb_knob_set): their current values, by id.
-enum { KNOB_REFINE_QUEUE = 0, KNOB_RINGS_GROUP = 1, KNOB_COUNT = 2 };
+enum { KNOB_REFINE_QUEUE = 0, KNOB_RINGS_GROUP = 1, KNOB_PREP_GLOBAL = 2, KNOB_COUNT = 3 };
 int knob(int id);
 
 bool prof_on();

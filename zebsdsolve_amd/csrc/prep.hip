@@ -417,20 +417,16 @@ __global__ void __launch_bounds__(256) k_clahe_apply(float* __restrict__ img, Cl
     img[tid] = clahe_value(img[tid], pix / a.w, pix % a.w, a.h, a.w, a.tiles, a.bins, cdf + (size_t)b * a.tiles * a.tiles * a.bins);
 }
 
-// The filter chain and the normalization, one block per pattern with the image in LDS: the
-// static background subtracted on the way in, the dynamic background, CLAHE, the band pass
-// (the coarse Gaussian parked in the pattern's output row), then zero mean and unit norm under
-// the mask; HBM sees the raw pattern and the result once each.
+// The filter chain and the normalization of one pattern, the block's 256 threads over the
+// image `img` and the scratch `aux` (LDS, or a block's slot of global memory for images past
+// what LDS holds): the static background subtracted on the way in, the dynamic background,
+// CLAHE, the band pass (the coarse Gaussian parked in the pattern's output row), then zero mean
+// and unit norm under the mask. The same arithmetic in the same order wherever img and aux live.
 template <typename T>
-__global__ void __launch_bounds__(256) k_prep(const T* __restrict__ raw, const float* __restrict__ bg, PrepArg a, float* __restrict__ out) {
-    extern __shared__ float lds[];
-    __shared__ float red[8];
+__device__ void prep_one(const T* __restrict__ src, const float* __restrict__ bg, const PrepArg& a, float* __restrict__ dst, float* img,
+                         float* aux, float* red) {
     const int hw = a.h * a.w;
     const int tid = threadIdx.x;
-    float* img = lds;
-    float* aux = lds + ((hw + 3) & ~3);
-    const T* src = raw + (size_t)blockIdx.x * hw;
-    float* dst = out + (size_t)blockIdx.x * hw;
     for (int i = tid; i < hw; i += 256) img[i] = (float)src[i] - (bg ? bg[i] : 0.f);
     __syncthreads();
     if (a.dyn) {
@@ -469,6 +465,29 @@ __global__ void __launch_bounds__(256) k_prep(const T* __restrict__ raw, const f
     ss = block_sum<256>(ss, red);
     const float inv = 1.f / sqrtf(fmaxf(ss, 1e-30f));
     for (int i = tid; i < hw; i += 256) dst[i] = img[i] * inv;
+}
+
+// One block per pattern with the image in LDS: HBM sees the raw pattern and the result once each.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prep(const T* __restrict__ raw, const float* __restrict__ bg, PrepArg a, float* __restrict__ out) {
+    extern __shared__ float lds[];
+    __shared__ float red[8];
+    const int hw = a.h * a.w;
+    prep_one(raw + (size_t)blockIdx.x * hw, bg, a, out + (size_t)blockIdx.x * hw, lds, lds + ((hw + 3) & ~3), red);
+}
+
+// Images past one block's LDS: each block works in its own slot of `work` (slot floats, the
+// image then the scratch) and takes patterns blockIdx.x, + gridDim.x, ... in turn.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prep_g(const T* __restrict__ raw, const float* __restrict__ bg, PrepArg a, int n, float* __restrict__ work,
+                                                long slot, float* __restrict__ out) {
+    __shared__ float red[8];
+    const int hw = a.h * a.w;
+    float* img = work + (size_t)blockIdx.x * slot;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        prep_one(raw + (size_t)b * hw, bg, a, out + (size_t)b * hw, img, img + ((hw + 3) & ~3), red);
+        __syncthreads();  // every thread done with the slot before the next pattern's loads
+    }
 }
 
 struct FrameArg {
@@ -631,24 +650,23 @@ static size_t blur_arg(double sigma, int h, int w, BlurArg& g) {
     return (size_t)2 * g.hp * (g.wp > w ? g.wp : w) * 2;
 }
 
-extern "C" int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* o,
-                           float* out, void* stream) {
-    if (n < 0 || h < 2 || w < 2 || (n > 0 && (!raw || !out)) || !mask || !o || pixel < 0 || pixel > 2 || o->dynamic < 0 ||
-        o->dynamic > 2 || (o->dynamic && !(o->dynamic_sigma > 0.f)) ||
+// The filter chain's parameters, and the LDS words its scratch takes beside the image; -1 (with
+// the reason kept) for bad arguments.
+static long prep_arg(const char* who, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* o, PrepArg& a) {
+    if (n < 0 || h < 2 || w < 2 || !mask || !o || pixel < 0 || pixel > 2 || o->dynamic < 0 || o->dynamic > 2 ||
+        (o->dynamic && !(o->dynamic_sigma > 0.f)) ||
         (o->clahe && (o->clahe_grid < 1 || o->clahe_grid > 16 || o->clahe_bins < 2 || o->clahe_bins > 256 || !(o->clahe_clip > 0.f))) ||
-        (o->bandpass && !(o->bp_fine >= 0.f && o->bp_coarse > o->bp_fine))) {
-        set_error("zb_prep: bad arguments (n %d, h %d, w %d, pixel %d, dynamic %d, clahe %d, bandpass %d)", n, h, w, pixel, o ? o->dynamic : 0,
+        (o->bandpass && !(o->bp_fine >= 0.f && o->bp_coarse > o->bp_fine)) || (long)h * w > (1L << 28)) {
+        set_error("%s: bad arguments (n %d, h %d, w %d, pixel %d, dynamic %d, clahe %d, bandpass %d)", who, n, h, w, pixel, o ? o->dynamic : 0,
                   o ? o->clahe : 0, o ? o->bandpass : 0);
-        return ZB_EINVAL;
+        return -1;
     }
-    if (n == 0) return ZB_OK;
-    PrepArg a = {};
+    a = PrepArg{};
     a.h = h;
     a.w = w;
     for (int i = 0; i < 3; ++i) a.mask[i] = mask[i];
     a.dyn = o->dynamic;
-    const long hw = (long)h * w;
-    size_t scratch = 0;  // LDS words beside the image
+    size_t scratch = 0;  // words beside the image
     if (a.dyn) scratch = std::max(scratch, blur_arg(o->dynamic_sigma, h, w, a.dynb));
     if (o->clahe) {
         a.clahe = 1;
@@ -669,25 +687,103 @@ extern "C" int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h,
             scratch = std::max(scratch, blur_arg(o->bp_fine, h, w, a.fine));
         }
     }
-    const size_t lds = ((size_t)((hw + 3) & ~3L) + scratch) * 4;
-    int dev = 0, limit = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
-        set_error("zb_prep: no device");
+    return (long)scratch;
+}
+
+// Bytes of LDS a block of k_prep takes, and what the device gives one.
+static int prep_lds(const PrepArg& a, long scratch, size_t& lds, int& limit) {
+    const long hw = (long)a.h * a.w;
+    lds = ((size_t)((hw + 3) & ~3L) + (size_t)scratch) * 4;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return -1;
+    return 0;
+}
+
+// Blocks of k_prep_g, each with its slot of the workspace.
+constexpr int kPrepBlocks = 1024;
+
+static long prep_slot(const PrepArg& a, long scratch) {
+    const long hw = (long)a.h * a.w;
+    return ((((hw + 3) & ~3L) + scratch) + 63) & ~63L;  // floats, slots 256-byte aligned
+}
+
+template <typename T>
+static void prep_launch(const void* raw, const float* bg, const PrepArg& a, int n, bool global, size_t lds, float* work, long slot, int grid,
+                        float* out, hipStream_t s) {
+    if (global)
+        ZB_LAUNCH("k_prep", k_prep_g<T>, dim3(grid), dim3(256), 0, s, (const T*)raw, bg, a, n, work, slot, out);
+    else
+        ZB_LAUNCH("k_prep", k_prep<T>, dim3(n), dim3(256), lds, s, (const T*)raw, bg, a, out);
+}
+
+static int32_t prep_run(const char* who, const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* o,
+                        void* work, int64_t work_bytes, bool may_global, float* out, void* stream) {
+    PrepArg a;
+    const long scratch = prep_arg(who, pixel, n, h, w, mask, o, a);
+    if (scratch < 0) return ZB_EINVAL;
+    if (n == 0) return ZB_OK;
+    if (!raw || !out) {
+        set_error("%s: null pointer", who);
+        return ZB_EINVAL;
+    }
+    size_t lds = 0;
+    int limit = 0;
+    if (prep_lds(a, scratch, lds, limit)) {
+        set_error("%s: no device", who);
         return ZB_ELAUNCH;
     }
-    if (lds > (size_t)limit) {
-        set_error("zb_prep: a %d x %d pattern with these filters needs %zu bytes of LDS, the device gives a block %d", h, w, lds, limit);
+    const bool global = lds > (size_t)limit || (may_global && knob(KNOB_PREP_GLOBAL) != 0);
+    if (global && !may_global) {
+        set_error("%s: a %d x %d pattern with these filters needs %zu bytes of LDS, the device gives a block %d (zb_prep_work takes it "
+                  "through global memory)",
+                  who, h, w, lds, limit);
         return ZB_EINVAL;
+    }
+    const long slot = prep_slot(a, scratch);
+    int grid = 0;
+    if (global) {
+        // a block a slot: as many as the workspace holds, up to kPrepBlocks (fewer run slower)
+        grid = (int)std::min<long>(std::min<long>(n, kPrepBlocks), work ? work_bytes / (slot * 4) : 0);
+        if (grid < 1) {
+            set_error("%s: workspace %lld bytes, need at least %lld (one slot; %lld for the full grid)", who, (long long)work_bytes,
+                      (long long)slot * 4, (long long)std::min(n, kPrepBlocks) * slot * 4);
+            return ZB_ESPACE;
+        }
     }
     hipStream_t s = (hipStream_t)stream;
     const float* bg = o->background;
     if (pixel == 0)
-        ZB_LAUNCH("k_prep", k_prep<uint8_t>, dim3(n), dim3(256), lds, s, (const uint8_t*)raw, bg, a, out);
+        prep_launch<uint8_t>(raw, bg, a, n, global, lds, (float*)work, slot, grid, out, s);
     else if (pixel == 1)
-        ZB_LAUNCH("k_prep", k_prep<uint16_t>, dim3(n), dim3(256), lds, s, (const uint16_t*)raw, bg, a, out);
+        prep_launch<uint16_t>(raw, bg, a, n, global, lds, (float*)work, slot, grid, out, s);
     else
-        ZB_LAUNCH("k_prep", k_prep<float>, dim3(n), dim3(256), lds, s, (const float*)raw, bg, a, out);
-    return check_launch("zb_prep");
+        prep_launch<float>(raw, bg, a, n, global, lds, (float*)work, slot, grid, out, s);
+    return check_launch(who);
+}
+
+extern "C" int32_t zb_prep(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* o,
+                           float* out, void* stream) {
+    return prep_run("zb_prep", raw, pixel, n, h, w, mask, o, nullptr, 0, false, out, stream);
+}
+
+extern "C" int64_t zb_prep_workspace(int32_t n, int32_t h, int32_t w, const zb_prep_opts* o) {
+    PrepArg a;
+    const float mask[3] = {0.f, 0.f, 0.f};
+    const long scratch = prep_arg("zb_prep_workspace", 2, n, h, w, mask, o, a);
+    if (scratch < 0) return ZB_EINVAL;
+    size_t lds = 0;
+    int limit = 0;
+    if (prep_lds(a, scratch, lds, limit)) {
+        set_error("zb_prep_workspace: no device");
+        return ZB_ELAUNCH;
+    }
+    if (lds <= (size_t)limit && knob(KNOB_PREP_GLOBAL) == 0) return 0;
+    return (int64_t)std::min(n, kPrepBlocks) * prep_slot(a, scratch) * 4;
+}
+
+extern "C" int32_t zb_prep_work(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* o,
+                                void* work, int64_t work_bytes, float* out, void* stream) {
+    return prep_run("zb_prep_work", raw, pixel, n, h, w, mask, o, work, work_bytes, true, out, stream);
 }
 
 extern "C" int32_t zb_project(const float* master, int32_t side, const zb_frame* frame, const float* poses, int32_t n, float* out,

@@ -587,7 +587,18 @@ class Engine:
         o.clahe_grid, o.clahe_bins, o.clahe_clip = int(s["clahe_grid"]), int(s["clahe_bins"]), float(s["clahe_clip"])
         o.bandpass = int(bool(s["bandpass"]))
         o.bp_fine, o.bp_coarse = float(s["bp_fine"]), float(s["bp_coarse"])  # pixels of the pattern as stored (binning 1)
-        nat.check(lib.zb_prep(raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), out.data_ptr(), st), "prep")
+        # patterns past one block's LDS (with these filters) go through a workspace in global
+        # memory, kept for the next batch of the same shape
+        need = int(lib.zb_prep_workspace(n, h, w, nat.ctypes.byref(o)))
+        if need < 0:
+            raise nat.NativeError(f"prep: {nat.last_error()}")
+        ws, wsz = None, 0
+        if need > 0:
+            cur = getattr(self, "_prep_ws", None)
+            if cur is None or cur.numel() < need or cur.device != out.device:
+                cur = self._prep_ws = torch.empty((need,), dtype=torch.uint8, device=out.device)
+            ws, wsz = cur.data_ptr(), cur.numel()
+        nat.check(lib.zb_prep_work(raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), ws, wsz, out.data_ptr(), st), "prep")
 
     @staticmethod
     def _pack_items(chunks, items):
