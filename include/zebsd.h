@@ -363,8 +363,8 @@ int64_t zb_prep_workspace(int32_t n, int32_t h, int32_t w, const zb_prep_opts* o
 
 /* zb_prep for patterns of any size (the reference's 240 x 320 frames with its default filters):
  * where LDS holds the image, zb_prep itself; past it, the same arithmetic in the same order over
- * a slot of global memory a block (work: zb_prep_workspace bytes; a smaller one takes fewer
- * blocks, down to one slot), each block taking patterns in turn. The results are bit-identical
+ * a slot of global memory a block (work: zb_prep_workspace bytes, 8-byte aligned; a smaller one
+ * takes fewer blocks, down to one slot), each block taking patterns in turn. The results are bit-identical
  * to zb_prep's where both run (the knob "prep_global" forces the global form). Replaces the
  * same Prep.run + Prep.filter as zb_prep. */
 int32_t zb_prep_work(const void* raw, int32_t pixel, int32_t n, int32_t h, int32_t w, const float mask[3], const zb_prep_opts* opts,

@@ -77,6 +77,16 @@ def test_bad_arguments_fail_with_a_message_and_launch_nothing():
     ph[1].seed_stride = 8
     ph[1].records = None
     assert lib.zb_refine_phases(*args(2, 1024)) == -1 and "phase 1: null pointer" in nat.last_error()
+    # zb_prep_work: filter arguments as zb_prep's, and a workspace the box blur can read as float pairs
+    po = nat.PrepOpts(None, 1, 0.0)  # a dynamic background needs a positive sigma
+    mask = (ctypes.c_float * 3)(0, 0, 0)
+    assert lib.zb_prep_workspace(4, 60, 60, ctypes.byref(po)) == -1 and "zb_prep_workspace" in nat.last_error()
+    assert lib.zb_prep_work(fake, 2, 4, 60, 60, mask, ctypes.byref(po), fake, 1 << 20, fake, None) == -1
+    assert "zb_prep_work: bad arguments" in nat.last_error()
+    po = nat.PrepOpts(None, 1, 9.0)
+    assert lib.zb_prep_work(fake + 4, 2, 4, 60, 60, mask, ctypes.byref(po), fake + 4, 1 << 20, fake, None) == -1
+    assert "8-byte aligned" in nat.last_error()
+    assert lib.zb_prep_work(None, 2, 0, 60, 60, mask, ctypes.byref(po), None, 0, None, None) == 0  # nothing to do
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")

@@ -726,6 +726,10 @@ static int32_t prep_run(const char* who, const void* raw, int32_t pixel, int32_t
         set_error("%s: null pointer", who);
         return ZB_EINVAL;
     }
+    if (((uintptr_t)work & 7) != 0) {  // the box blur's scratch is read as float pairs
+        set_error("%s: the workspace must be 8-byte aligned", who);
+        return ZB_EINVAL;
+    }
     size_t lds = 0;
     int limit = 0;
     if (prep_lds(a, scratch, lds, limit)) {
