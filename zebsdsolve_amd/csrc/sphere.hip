@@ -82,10 +82,18 @@ __device__ __forceinline__ float2 sample_dir(const SphArg& a, const ItemGeom& g,
     return make_float2(val, w);
 }
 
-// The samples of an item at grid column cg on ring y (cos theta = ct) and on its mirror ring
-// n - 1 - y (cos theta = -ct, the same sine).
-__device__ __forceinline__ void sample_pair(const SphArg& a, const ItemGeom& g, const ColGeom& cg, float ct, float2& north, float2& south) {
-    const float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+// A ring pair's values as a block's threads read them from LDS: cos theta of the northern ring,
+// the sine both rings share, and their quadrature weight. Computed once per ring by the block
+// (the sine's square root takes some twenty instructions) instead of once per sample pair.
+__device__ __forceinline__ float4 ring_entry(const float* __restrict__ ring, int y) {
+    const float ct = ring[2 * y];
+    return make_float4(ct, sqrtf(fmaxf(1.f - ct * ct, 0.f)), ring[2 * y + 1], 0.f);
+}
+
+// The samples of an item at grid column cg on ring y (cos theta = ct, sin theta = st) and on its
+// mirror ring n - 1 - y (cos theta = -ct, the same sine).
+__device__ __forceinline__ void sample_pair(const SphArg& a, const ItemGeom& g, const ColGeom& cg, float ct, float st, float2& north,
+                                            float2& south) {
     const float h0 = st * cg.q0, h1 = st * cg.q1, h2 = st * cg.q2;
     const float z0 = a.c0[2] * ct, z1 = a.c1[2] * ct, z2 = a.c2[2] * ct;
     north = sample_dir(a, g, h2 + z2, h0 + z0, h1 + z1);
@@ -97,6 +105,7 @@ __device__ __forceinline__ void sample_pair(const SphArg& a, const ItemGeom& g, 
 __global__ void __launch_bounds__(256) k_sums(SphArg a, const float* __restrict__ ring, const float2* __restrict__ tw,
                                               const float* __restrict__ pats, const float4* __restrict__ items, float2* __restrict__ sums) {
     __shared__ float2 red[4];
+    __shared__ float4 rt[ZB_TW / 4];  // the ring pairs' entries (n / 2 <= ZB_TW / 4)
     const int b = blockIdx.x;
     const ItemGeom g = item_geom(a, pats, items[b]);
     const int nphi = 1 << a.log2nphi;
@@ -104,13 +113,16 @@ __global__ void __launch_bounds__(256) k_sums(SphArg a, const float* __restrict_
     const int cols = nphi < 256 ? nphi : 256;   // columns a pass covers
     const int rows = 256 / cols;                // ring pairs a pass covers
     const int tid = threadIdx.x;
+    for (int y = tid; y < nh; y += 256) rt[y] = ring_entry(ring, y);
+    __syncthreads();
     float sv = 0.f, sw = 0.f;
     for (int j = tid % cols; j < nphi; j += cols) {
         const ColGeom cg = col_geom(a, tw[j << (10 - a.log2nphi)]);
         for (int y = tid / cols; y < nh; y += rows) {
             float2 gn, gs;
-            sample_pair(a, g, cg, ring[2 * y], gn, gs);
-            const float wq = ring[2 * y + 1];
+            const float4 r = rt[y];
+            sample_pair(a, g, cg, r.x, r.y, gn, gs);
+            const float wq = r.z;
             sv += (gn.x * gn.y + gs.x * gs.y) * wq;
             sw += (gn.y + gs.y) * wq;
         }
@@ -134,12 +146,15 @@ __device__ __forceinline__ float window_mean(const SphArg& a, float2 tot) {
 __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __restrict__ ring, const float2* __restrict__ tw,
                                                const float* __restrict__ pats, const float4* __restrict__ items,
                                                const float2* __restrict__ sums, float4* __restrict__ spectra) {
-    extern __shared__ float2 buf[];
+    extern __shared__ float2 buf[];  // [G][nphi] lines, then the group's G ring entries
     const int nh = a.n >> 1;
     const int groups = nh / G;
     const int b = blockIdx.x / groups;
     const int y0 = (blockIdx.x - b * groups) * G;
     const int nphi = 1 << a.log2nphi;
+    float4* rt = reinterpret_cast<float4*>(buf + G * nphi);
+    for (int i = threadIdx.x; i < G; i += 256) rt[i] = ring_entry(ring, y0 + i);
+    __syncthreads();
     const ItemGeom g = item_geom(a, pats, items[b]);
     const float mean = window_mean(a, sums[b]);
     // each thread keeps grid columns and walks the group's rings
@@ -149,7 +164,8 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
         const ColGeom cg = col_geom(a, tw[j << (10 - a.log2nphi)]);
         for (int i = threadIdx.x / cols; i < G; i += rows) {
             float2 gn, gs;
-            sample_pair(a, g, cg, ring[2 * (y0 + i)], gn, gs);
+            const float4 r = rt[i];
+            sample_pair(a, g, cg, r.x, r.y, gn, gs);
             buf[i * nphi + j] = make_float2((gn.x - mean) * gn.y, (gs.x - mean) * gs.y);
         }
     }
@@ -476,7 +492,7 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
     if (G > 8192 / nphi) G = 8192 / nphi;
     if (G < 1) G = 1;
     if (G > nh) G = nh;
-    ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8, s, a, G, band->ring, tw, pats,
+    ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8 + (size_t)G * 16, s, a, G, band->ring, tw, pats,
                        (const float4*)items, (const float2*)sums, spectra);
     if (int rc = check_launch("zb_sh_coefficients/rings")) return rc;
     if ((flags & (ZB_FLAG_FP16 | ZB_FLAG_BF16)) && nh >= 16) {
