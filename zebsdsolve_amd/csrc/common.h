@@ -352,6 +352,58 @@ __device__ void fft_dif(float2* buf, int log2n, int nlines, int lstride, int pst
     }
 }
 
+// fft_dif for contiguous lines (pstride 1, lstride 2^LOG2N) of a size known at compile time:
+// the same butterflies in the same order (bit-identical results), with the stage sizes, strides
+// and twiddle steps constants, so that the index arithmetic folds away.
+template <int NT, int LOG2N, int S>
+__device__ __forceinline__ void fft_dif_stages(float2* buf, int nlines, const float2* __restrict__ tw, int tid) {
+    if constexpr (S >= 2) {
+        constexpr int lq = S - 2, q = 1 << lq, nq = 1 << (LOG2N - 2);
+        constexpr int t4 = ZB_TW >> S, t2 = ZB_TW >> (S - 1);
+        const int total = nlines * nq;
+        for (int e = tid; e < total; e += NT) {
+            const int line = e >> (LOG2N - 2);
+            const int r = e & (nq - 1);
+            const int j = r & (q - 1);
+            float2* b = buf + (line << LOG2N) + ((r >> lq) << S) + j;
+            float2 x0 = b[0];
+            float2 x1 = b[q];
+            float2 x2 = b[2 * q];
+            float2 x3 = b[3 * q];
+            float2 w4 = tw[j * t4];
+            float2 w2 = tw[j * t2];
+            float2 a0 = cadd(x0, x2);
+            float2 a2 = cmul(csub(x0, x2), w4);
+            float2 a1 = cadd(x1, x3);
+            float2 a3 = cmul(cmul_mi(csub(x1, x3)), w4);
+            b[0] = cadd(a0, a1);
+            b[q] = cmul(csub(a0, a1), w2);
+            b[2 * q] = cadd(a2, a3);
+            b[3 * q] = cmul(csub(a2, a3), w2);
+        }
+        __syncthreads();
+        fft_dif_stages<NT, LOG2N, S - 2>(buf, nlines, tw, tid);
+    } else if constexpr (S == 1) {
+        constexpr int half = 1 << (LOG2N - 1);
+        const int total = nlines * half;
+        for (int e = tid; e < total; e += NT) {
+            const int line = e >> (LOG2N - 1);
+            const int r = e & (half - 1);
+            float2* b = buf + (line << LOG2N) + 2 * r;
+            float2 x0 = b[0];
+            float2 x1 = b[1];
+            b[0] = cadd(x0, x1);
+            b[1] = csub(x0, x1);
+        }
+        __syncthreads();
+    }
+}
+
+template <int NT, int LOG2N>
+__device__ void fft_dif_fixed(float2* buf, int nlines, const float2* __restrict__ tw, int tid) {
+    fft_dif_stages<NT, LOG2N, LOG2N>(buf, nlines, tw, tid);
+}
+
 __device__ __forceinline__ float warp_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;

@@ -143,10 +143,14 @@ __device__ __forceinline__ float window_mean(const SphArg& a, float2 tot) {
 }
 
 // Ring pairs y0 .. y0 + G - 1 of one item: sample, FFT along phi, keep m < L (sphere.wgsl:157-207).
+// LOG2NPHI: the points a ring has, 2^LOG2NPHI, known at compile time (the FFT's index arithmetic
+// folded into constants); 0: any, from the arguments.
+template <int LOG2NPHI>
 __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __restrict__ ring, const float2* __restrict__ tw,
                                                const float* __restrict__ pats, const float4* __restrict__ items,
                                                const float2* __restrict__ sums, float4* __restrict__ spectra) {
     extern __shared__ float2 buf[];  // [G][nphi] lines, then the group's G ring entries
+    if constexpr (LOG2NPHI > 0) a.log2nphi = LOG2NPHI;
     const int nh = a.n >> 1;
     const int groups = nh / G;
     const int b = blockIdx.x / groups;
@@ -170,7 +174,10 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
         }
     }
     __syncthreads();
-    fft_dif<256>(buf, a.log2nphi, G, nphi, 1, tw, threadIdx.x);
+    if constexpr (LOG2NPHI > 0)
+        fft_dif_fixed<256, LOG2NPHI>(buf, G, tw, threadIdx.x);
+    else
+        fft_dif<256>(buf, a.log2nphi, G, nphi, 1, tw, threadIdx.x);
     // Z = N + i S for real N, S: N_m = (Z_m + conj Z_{-m}) / 2, S_m = (Z_m - conj Z_{-m}) / 2i
     const int L = a.L;
     for (int e = threadIdx.x; e < G * L; e += 256) {
@@ -492,8 +499,17 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
     if (G > 8192 / nphi) G = 8192 / nphi;
     if (G < 1) G = 1;
     if (G > nh) G = nh;
-    ZB_LAUNCH("k_rings", k_rings, dim3(n_items * (nh / G)), dim3(256), (size_t)G * nphi * 8 + (size_t)G * 16, s, a, G, band->ring, tw, pats,
-                       (const float4*)items, (const float2*)sums, spectra);
+    const dim3 rgrid(n_items * (nh / G));
+    const size_t rlds = (size_t)G * nphi * 8 + (size_t)G * 16;
+    const float4* its = (const float4*)items;
+    if (a.log2nphi == 7)
+        ZB_LAUNCH("k_rings", k_rings<7>, rgrid, dim3(256), rlds, s, a, G, band->ring, tw, pats, its, (const float2*)sums, spectra);
+    else if (a.log2nphi == 8)
+        ZB_LAUNCH("k_rings", k_rings<8>, rgrid, dim3(256), rlds, s, a, G, band->ring, tw, pats, its, (const float2*)sums, spectra);
+    else if (a.log2nphi == 9)
+        ZB_LAUNCH("k_rings", k_rings<9>, rgrid, dim3(256), rlds, s, a, G, band->ring, tw, pats, its, (const float2*)sums, spectra);
+    else
+        ZB_LAUNCH("k_rings", k_rings<0>, rgrid, dim3(256), rlds, s, a, G, band->ring, tw, pats, its, (const float2*)sums, spectra);
     if (int rc = check_launch("zb_sh_coefficients/rings")) return rc;
     if ((flags & (ZB_FLAG_FP16 | ZB_FLAG_BF16)) && nh >= 16) {
         // 16-bit products, 16 rings a product (below n = 32 the FP32 sums stand in)
