@@ -180,9 +180,10 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
         fft_dif<256>(buf, a.log2nphi, G, nphi, 1, tw, threadIdx.x);
     // Z = N + i S for real N, S: N_m = (Z_m + conj Z_{-m}) / 2, S_m = (Z_m - conj Z_{-m}) / 2i
     const int L = a.L;
+    const int lg = __builtin_ctz((unsigned)G);  // G is a power of two
     for (int e = threadIdx.x; e < G * L; e += 256) {
-        const int m = e / G;
-        const int i = e - m * G;
+        const int m = e >> lg;
+        const int i = e & (G - 1);
         const float2 z = buf[i * nphi + bitrev(m, a.log2nphi)];
         const float2 zr = buf[i * nphi + bitrev((nphi - m) & (nphi - 1), a.log2nphi)];
         const float2 rn = make_float2(0.5f * (z.x + zr.x), 0.5f * (z.y - zr.y));
@@ -488,7 +489,8 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
     float2* sums = (float2*)ws;
     float4* spectra = (float4*)(ws + ((int64_t)n_items * 8 + 255) / 256 * 256);
     const float2* tw = (const float2*)band->twiddle;
-    ZB_LAUNCH("k_sums", k_sums, dim3(n_items), dim3(256), 0, s, a, band->ring, tw, pats, (const float4*)items, sums);
+    const float4* its = (const float4*)items;
+    ZB_LAUNCH("k_sums", k_sums, dim3(n_items), dim3(256), 0, s, a, band->ring, tw, pats, its, sums);
     if (int rc = check_launch("zb_sh_coefficients/sums")) return rc;
     const int nh = n / 2;
     const int nphi = 2 * n;
@@ -501,7 +503,6 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
     if (G > nh) G = nh;
     const dim3 rgrid(n_items * (nh / G));
     const size_t rlds = (size_t)G * nphi * 8 + (size_t)G * 16;
-    const float4* its = (const float4*)items;
     if (a.log2nphi == 7)
         ZB_LAUNCH("k_rings", k_rings<7>, rgrid, dim3(256), rlds, s, a, G, band->ring, tw, pats, its, (const float2*)sums, spectra);
     else if (a.log2nphi == 8)
