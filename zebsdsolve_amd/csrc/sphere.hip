@@ -12,8 +12,9 @@
 //   k_legendre a_lm = sum_y wplm[y, pair] (sum or difference by the parity of l + m): per order
 //              m and tile of 16 items, two GEMMs on the matrix cores (one per parity)
 //   k_norm     unit harmonic norm and the item's stats
-// Workspace: per item the window sums (float2) and the ring spectra [L][n/2] float4 (m-major,
-// so that k_legendre reads them contiguously).
+// Workspace: per item the window sums (float2), and the ring spectra [L][items][n/2] float4
+// (order-major, so that a k_legendre block's items lie side by side, not a power-of-two stride
+// apart).
 #include <mutex>
 
 #include "common.h"
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
     // Z = N + i S for real N, S: N_m = (Z_m + conj Z_{-m}) / 2, S_m = (Z_m - conj Z_{-m}) / 2i
     const int L = a.L;
     const int lg = __builtin_ctz((unsigned)G);  // G is a power of two
+    const int n_items = gridDim.x / groups;
     for (int e = threadIdx.x; e < G * L; e += 256) {
         const int m = e >> lg;
         const int i = e & (G - 1);
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(256) k_rings(SphArg a, int G, const float* __r
         const float2 zr = buf[i * nphi + bitrev((nphi - m) & (nphi - 1), a.log2nphi)];
         const float2 rn = make_float2(0.5f * (z.x + zr.x), 0.5f * (z.y - zr.y));
         const float2 rs = make_float2(0.5f * (z.y + zr.y), -0.5f * (z.x - zr.x));
-        spectra[((size_t)b * L + m) * nh + y0 + i] = make_float4(rn.x + rs.x, rn.y + rs.y, rn.x - rs.x, rn.y - rs.y);
+        spectra[((size_t)m * n_items + b) * nh + y0 + i] = make_float4(rn.x + rs.x, rn.y + rs.y, rn.x - rs.x, rn.y - rs.y);
     }
 }
 
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(64) k_legendre(SphArg a, int n_items, const fl
     const int item = b0 + (r32 >> 1);
     const bool im = (r32 & 1) != 0;
     const bool iok = item < n_items;
-    const float4* sp = spectra + ((size_t)(iok ? item : 0) * L + m) * nh;
+    const float4* sp = spectra + ((size_t)m * n_items + (iok ? item : 0)) * nh;
     const int ncol = (L - m + 1) / 2;  // degrees l = m, m + 2, ...
     for (int ct = 0; ct * 32 < ncol; ++ct) {
         const int le = m + 2 * (ct * 32 + r32);  // this lane's column: even l + m
@@ -283,7 +285,7 @@ __global__ void __launch_bounds__(64) k_legendre16(SphArg a, int n_items, int sp
     const int item = b0 + (r32 >> 1);
     const bool im = (r32 & 1) != 0;
     const bool iok = item < n_items;
-    const float4* sp = spectra + ((size_t)(iok ? item : 0) * L + m) * nh;
+    const float4* sp = spectra + ((size_t)m * n_items + (iok ? item : 0)) * nh;
     const int ncol = (L - m + 1) / 2;
     constexpr float kScale = BF ? 1.f : 4096.f;
     for (int ct = 0; ct * 32 < ncol; ++ct) {
