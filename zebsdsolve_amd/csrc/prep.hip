@@ -25,28 +25,67 @@ __device__ __forceinline__ bool in_mask(float row, float col, const float* m) {
 
 struct MaskArg { float m[3]; };
 
+// VEC consecutive pixels as one load (VEC 4: 16 bytes of float32, 8 of u16, 4 of u8).
+template <typename T, int VEC>
+struct PixVec;
+template <typename T>
+struct PixVec<T, 1> {
+    static __device__ __forceinline__ void load(const T* p, float* x) { x[0] = (float)p[0]; }
+};
+template <>
+struct PixVec<float, 4> {
+    static __device__ __forceinline__ void load(const float* p, float* x) {
+        const float4 t = *reinterpret_cast<const float4*>(p);
+        x[0] = t.x, x[1] = t.y, x[2] = t.z, x[3] = t.w;
+    }
+};
+template <>
+struct PixVec<uint16_t, 4> {
+    static __device__ __forceinline__ void load(const uint16_t* p, float* x) {
+        const ushort4 t = *reinterpret_cast<const ushort4*>(p);
+        x[0] = (float)t.x, x[1] = (float)t.y, x[2] = (float)t.z, x[3] = (float)t.w;
+    }
+};
+template <>
+struct PixVec<uint8_t, 4> {
+    static __device__ __forceinline__ void load(const uint8_t* p, float* x) {
+        const uchar4 t = *reinterpret_cast<const uchar4*>(p);
+        x[0] = (float)t.x, x[1] = (float)t.y, x[2] = (float)t.z, x[3] = (float)t.w;
+    }
+};
+
 // One block of 256 per pattern: mean and norm under the mask (prep.wgsl:405-443). The pattern
-// is read once into registers (PER pixels a thread, PER * 256 >= h w), written once.
-template <typename T, int PER>
+// is read once into registers (PER loads of VEC consecutive pixels a thread, PER * VEC * 256 >=
+// h w; with VEC 4, h w a multiple of 4 and the rows 4-pixel aligned), written once.
+template <typename T, int PER, int VEC>
 __global__ void __launch_bounds__(256) k_normalize(const T* __restrict__ raw, int h, int w, MaskArg mask, float* __restrict__ out) {
+    static_assert(PER * VEC <= 64, "one mask bit a pixel in 64 bits");
     __shared__ float red[4];
     const int b = blockIdx.x;
     const int hw = h * w;
     const T* src = raw + (size_t)b * hw;
     float* dst = out + (size_t)b * hw;
     const float inv_w = 1.f / (float)w;
-    float v[PER];
+    float v[PER * VEC];
+    unsigned long long in = 0ull;  // bit k VEC + u: pixel u of load k is under the mask
     float s = 0.f, c = 0.f;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int i = threadIdx.x + 256 * k;
-        v[k] = 0.f;
-        if (i < hw) {
+        const int i0 = VEC * (threadIdx.x + 256 * k);
+        float x[VEC];
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) x[u] = 0.f;
+        if (i0 < hw) PixVec<T, VEC>::load(src + i0, x);
+#pragma unroll
+        for (int u = 0; u < VEC; ++u) {
+            const int i = i0 + u;
+            v[k * VEC + u] = 0.f;
             // row and column without an integer division (exact for h w <= 65536)
             const int row = (int)(((float)i + 0.5f) * inv_w);
-            if (in_mask((float)row, (float)(i - row * w), mask.m)) {
-                v[k] = (float)src[i];
-                s += v[k];
+            if (i < hw && in_mask((float)row, (float)(i - row * w), mask.m)) {
+                v[k * VEC + u] = x[u];
+                in |= 1ull << (k * VEC + u);
+                s += x[u];
                 c += 1.f;
             }
         }
@@ -56,23 +95,24 @@ __global__ void __launch_bounds__(256) k_normalize(const T* __restrict__ raw, in
     const float mean = s / fmaxf(c, 1.f);
     float ss = 0.f;
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = threadIdx.x + 256 * k;
-        const int row = (int)(((float)i + 0.5f) * inv_w);
-        const bool in = i < hw && in_mask((float)row, (float)(i - row * w), mask.m);
-        v[k] = in ? v[k] - mean : 0.f;
+    for (int k = 0; k < PER * VEC; ++k) {
+        v[k] = ((in >> k) & 1ull) ? v[k] - mean : 0.f;
         ss += v[k] * v[k];
     }
     ss = block_sum<256>(ss, red);
     const float inv = 1.f / sqrtf(fmaxf(ss, 1e-30f));
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int i = threadIdx.x + 256 * k;
-        if (i < hw) dst[i] = v[k] * inv;
+        const int i0 = VEC * (threadIdx.x + 256 * k);
+        if (i0 < hw) {
+            if constexpr (VEC == 4)
+                *reinterpret_cast<float4*>(dst + i0) = make_float4(v[4 * k] * inv, v[4 * k + 1] * inv, v[4 * k + 2] * inv, v[4 * k + 3] * inv);
+            else
+                dst[i0] = v[k] * inv;
+        }
     }
 }
 
-// The same for frames of more than 16384 pixels, in passes over memory.
 template <typename T>
 __global__ void __launch_bounds__(256) k_normalize_big(const T* __restrict__ raw, int h, int w, MaskArg mask, float* __restrict__ out) {
     __shared__ float red[4];
@@ -605,12 +645,18 @@ extern "C" int32_t zb_normalize(const void* raw, int32_t pixel, int32_t n, int32
     m.m[2] = mask[2];
     hipStream_t s = (hipStream_t)stream;
     const long hw = (long)h * w;
+    // four pixels a load where every pattern's row of h w starts 4-pixel aligned in both buffers
+    const bool vec = hw % 4 == 0 && ((uintptr_t)raw % (4 * ((pixel == 0) ? 1 : (pixel == 1) ? 2 : 4))) == 0 && ((uintptr_t)out % 16) == 0;
 #define ZB_NORM(T)                                                                                                            \
     do {                                                                                                                      \
-        if (hw <= 4096)                                                                                                       \
-            ZB_LAUNCH("k_normalize", (k_normalize<T, 16>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);          \
+        if (vec && hw <= 4096)                                                                                                \
+            ZB_LAUNCH("k_normalize", (k_normalize<T, 4, 4>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);        \
+        else if (vec && hw <= 16384)                                                                                          \
+            ZB_LAUNCH("k_normalize", (k_normalize<T, 16, 4>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);       \
+        else if (hw <= 4096)                                                                                                  \
+            ZB_LAUNCH("k_normalize", (k_normalize<T, 16, 1>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);       \
         else if (hw <= 16384)                                                                                                 \
-            ZB_LAUNCH("k_normalize", (k_normalize<T, 64>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);          \
+            ZB_LAUNCH("k_normalize", (k_normalize<T, 64, 1>), dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);       \
         else                                                                                                                  \
             ZB_LAUNCH("k_normalize", k_normalize_big<T>, dim3(n), dim3(256), 0, s, (const T*)raw, h, w, m, out);            \
     } while (0)

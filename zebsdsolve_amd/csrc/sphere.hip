@@ -339,30 +339,58 @@ __global__ void __launch_bounds__(64) k_legendre16(SphArg a, int n_items, int sp
 }
 
 // Unit norm sqrt(sum_m k |a|^2), k = 1 for m = 0 else 2, and the stats (sphere.wgsl:228-262).
+// PER > 0: a thread holds its PER coefficients (PER * 256 >= pairs) in registers between the norm
+// and the scaling; 0: any size, the coefficients read again.
+template <int PER>
 __global__ void __launch_bounds__(256) k_norm(SphArg a, const float2* __restrict__ sums, float2* __restrict__ alm, float* __restrict__ stats) {
     __shared__ float red[4];
     const int b = blockIdx.x;
     float2* c = alm + (size_t)b * a.pairs;
+    float2 v[PER > 0 ? PER : 1];
     float e = 0.f;
-    for (int p = threadIdx.x; p < a.pairs; p += 256) {
-        const float2 v = c[p];
-        e += (p < a.L ? 1.f : 2.f) * (v.x * v.x + v.y * v.y);
+    if constexpr (PER > 0) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int p = threadIdx.x + 256 * k;
+            v[k] = p < a.pairs ? c[p] : make_float2(0.f, 0.f);
+            e += (p < a.L ? 1.f : 2.f) * (v[k].x * v[k].x + v[k].y * v[k].y);
+        }
+    } else {
+        for (int p = threadIdx.x; p < a.pairs; p += 256) {
+            const float2 x = c[p];
+            e += (p < a.L ? 1.f : 2.f) * (x.x * x.x + x.y * x.y);
+        }
     }
     e = warp_sum(e);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
     const float tot = red[0] + red[1] + red[2] + red[3];
     const float nrm = sqrtf(fmaxf(tot, 1e-30f));
-    if (a.flags & ZB_FLAG_NORM) {
-        for (int p = threadIdx.x; p < a.pairs; p += 256) c[p] = make_float2(c[p].x / nrm, c[p].y / nrm);
+    const bool scale = (a.flags & ZB_FLAG_NORM) != 0;
+    const float inv = 1.f / nrm;
+    float c0 = 0.f;  // the (0, 0) coefficient as stored
+    if constexpr (PER > 0) {
+        if (scale) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int p = threadIdx.x + 256 * k;
+                v[k] = make_float2(v[k].x * inv, v[k].y * inv);
+                if (p < a.pairs) c[p] = v[k];
+            }
+        }
+        c0 = v[0].x;
+    } else {
+        if (scale)
+            for (int p = threadIdx.x; p < a.pairs; p += 256) c[p] = make_float2(c[p].x * inv, c[p].y * inv);
+        __syncthreads();
+        c0 = c[0].x;
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
         const float2 t = sums[b];
         stats[4 * b] = t.y * ZB_TAU / (float)(1 << a.log2nphi);
         stats[4 * b + 1] = window_mean(a, t);
         stats[4 * b + 2] = nrm;
-        stats[4 * b + 3] = c[0].x;
+        stats[4 * b + 3] = c0;
     }
 }
 
@@ -528,6 +556,11 @@ static int32_t sh_run(const zb_band* band, const zb_frame* frame, const float* p
                   (float2*)alm);
     }
     if (int rc = check_launch("zb_sh_coefficients/legendre")) return rc;
-    ZB_LAUNCH("k_norm", k_norm, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
+    if (a.pairs <= 256 * 10)
+        ZB_LAUNCH("k_norm", k_norm<10>, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
+    else if (a.pairs <= 256 * 20)
+        ZB_LAUNCH("k_norm", k_norm<20>, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
+    else
+        ZB_LAUNCH("k_norm", k_norm<0>, dim3(n_items), dim3(256), 0, s, a, (const float2*)sums, (float2*)alm, stats);
     return check_launch("zb_sh_coefficients/norm");
 }
