@@ -402,10 +402,12 @@ int32_t zb_master_blur(const float* src, float* dst, int32_t side, float sigma_r
  * Replaces Prep.gradient (src/prep.zig:253-260; mastersph.wgsl:124-138). */
 int32_t zb_master_gradient(const float* sheets, int32_t side, float* grad, void* stream);
 
-/* The gradient field in half precision, each value rounded to nearest even: grad_h
- * [2][side][side][4] f16 (8 bytes a texel), for zb_refine's Gauss-Newton passes. The reference
+/* The gradient field in half precision, each value rounded to nearest even, each texel stored
+ * with the texel below it: grad_h [2][side][side][2][4] f16 (16 bytes a texel: [0] the texel,
+ * [1] the next row's at the same column; the last row of a sheet repeats itself), for
+ * zb_refine's Gauss-Newton passes: a bilinear lookup is two adjacent 16-byte loads. The reference
  * keeps only the float32 field (mastersph.wgsl:124-138); this copy halves the refinement's
- * vector-memory instructions. */
+ * vector-memory instructions and the cache lines a lookup touches. */
 int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* grad_h, void* stream);
 
 /* Kernel timing (events around every launch while on). The reference times its dispatches the

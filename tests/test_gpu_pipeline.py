@@ -90,11 +90,15 @@ def test_master_blur_and_gradient(case):
 
 def test_master_pack_rounds_the_field_to_half(case):
     """zb_master_pack: every value of the gradient field rounded to the nearest half (ties to
-    even), bit for bit as numpy's float16 conversion."""
+    even), bit for bit as numpy's float16 conversion, each texel followed by the one below it
+    (the last row of a sheet by itself)."""
     e = case["e"]
-    gh = e.grad_half(0).cpu().numpy().view(np.uint16)
-    ref = case["grad"].reshape(-1).astype(np.float16).view(np.uint16)
-    assert gh.shape == ref.shape and np.array_equal(gh, ref)
+    g = case["grad"].reshape(2, -1, 4)
+    side = int(round(np.sqrt(g.shape[1])))
+    gh = e.grad_half(0).cpu().numpy().view(np.uint16).reshape(2, side, side, 2, 4)
+    ref = g.reshape(2, side, side, 4).astype(np.float16).view(np.uint16)
+    below = np.concatenate([ref[:, 1:], ref[:, -1:]], axis=1)
+    assert np.array_equal(gh[:, :, :, 0], ref) and np.array_equal(gh[:, :, :, 1], below)
 
 
 def _gpu_coefficients(case, flags=3, window=False):

@@ -214,17 +214,18 @@ __device__ __forceinline__ float2 unpack_half2(unsigned u) {
     return make_float2((float)h.x, (float)h.y);
 }
 
-// The four texels of a lookup in the half-precision field: each row's pair in one 16-byte load
-// (texels o, o + 1, then o + side, o + side + 1), through a buffer resource over the field.
+// The four texels of a lookup in the half-precision field, where each texel is stored with the
+// one below it (16 bytes, zb_master_pack): texels o and o + side in one 16-byte load (left),
+// o + 1 and o + 1 + side in the next (right), through a buffer resource over the field.
 struct TapsH {
-    zb_u32x4 top, bot;
+    zb_u32x4 left, right;
     float tj, ti;
 };
 
 __device__ __forceinline__ void sheet_taps_h(__amdgpu_buffer_rsrc_t field, int side, float3 v, TapsH& t) {
     const int o = sheet_texel(side, v, t.tj, t.ti);
-    t.top = __builtin_amdgcn_raw_buffer_load_b128(field, o * 8, 0, 0);
-    t.bot = __builtin_amdgcn_raw_buffer_load_b128(field, (o + side) * 8, 0, 0);
+    t.left = __builtin_amdgcn_raw_buffer_load_b128(field, o * 16, 0, 0);
+    t.right = __builtin_amdgcn_raw_buffer_load_b128(field, o * 16 + 16, 0, 0);
 }
 
 // The four texels of a lookup in float32 sheets: each row's pair in one 8-byte load.
@@ -258,14 +259,14 @@ __device__ __forceinline__ float4 taps_blend(const Taps<float4>& t) {
 }
 
 __device__ __forceinline__ float4 taps_blend(const TapsH& t) {
-    Taps<float4> f;
-    float2 lo = unpack_half2(t.top.x), hi = unpack_half2(t.top.y);
+    Taps<float4> f;  // a, b: texels o, o + 1; c, d: the row below
+    float2 lo = unpack_half2(t.left.x), hi = unpack_half2(t.left.y);
     f.a = make_float4(lo.x, lo.y, hi.x, hi.y);
-    lo = unpack_half2(t.top.z), hi = unpack_half2(t.top.w);
-    f.b = make_float4(lo.x, lo.y, hi.x, hi.y);
-    lo = unpack_half2(t.bot.x), hi = unpack_half2(t.bot.y);
+    lo = unpack_half2(t.left.z), hi = unpack_half2(t.left.w);
     f.c = make_float4(lo.x, lo.y, hi.x, hi.y);
-    lo = unpack_half2(t.bot.z), hi = unpack_half2(t.bot.w);
+    lo = unpack_half2(t.right.x), hi = unpack_half2(t.right.y);
+    f.b = make_float4(lo.x, lo.y, hi.x, hi.y);
+    lo = unpack_half2(t.right.z), hi = unpack_half2(t.right.w);
     f.d = make_float4(lo.x, lo.y, hi.x, hi.y);
     f.tj = t.tj;
     f.ti = t.ti;

@@ -920,13 +920,17 @@ extern "C" int32_t zb_master_symmetrize(const float* src, float* dst, int32_t si
     return check_launch("zb_master_symmetrize");
 }
 
-// The gradient field in half precision (each float rounded to nearest even): 8 bytes a texel, so
-// that a texel and its right-hand neighbour come in one 16-byte load.
-__global__ void __launch_bounds__(256) k_pack_half(const float4* __restrict__ grad, int total, uint2* __restrict__ out) {
+// The gradient field in half precision (each float rounded to nearest even), 8 bytes a texel,
+// each texel stored with the texel below it (16 bytes; the last row of a sheet with itself): a
+// lookup's four texels are two adjacent 16-byte loads, (o, o + side) and (o + 1, o + 1 + side),
+// which mostly share one cache line, where a row-major field puts the two rows in two.
+__global__ void __launch_bounds__(256) k_pack_half(const float4* __restrict__ grad, int side, int total, uint4* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
+    const int row = (i % (side * side)) / side;
     const float4 g = grad[i];
-    out[i] = make_uint2(pack_half2(g.x, g.y), pack_half2(g.z, g.w));
+    const float4 u = grad[row < side - 1 ? i + side : i];
+    out[i] = make_uint4(pack_half2(g.x, g.y), pack_half2(g.z, g.w), pack_half2(u.x, u.y), pack_half2(u.z, u.w));
 }
 
 extern "C" int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* grad_h, void* stream) {
@@ -935,7 +939,7 @@ extern "C" int32_t zb_master_pack(const float* grad, int32_t side, uint16_t* gra
         return ZB_EINVAL;
     }
     const int total = 2 * side * side;
-    ZB_LAUNCH("k_pack_half", k_pack_half, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, (const float4*)grad, total,
-              (uint2*)grad_h);
+    ZB_LAUNCH("k_pack_half", k_pack_half, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, (const float4*)grad, side, total,
+              (uint4*)grad_h);
     return check_launch("zb_master_pack");
 }

@@ -199,8 +199,9 @@ __device__ __forceinline__ float3 solve3(const float* h, float3 g) {
 
 // One item's refinement, NMI and record (at records[out0 + b]), with the masked-pixel list (LIST)
 // already built and the histogram cleared.
-// The Gauss-Newton passes read the half-precision gradient field (mgh: one 16-byte load per
-// texel row pair, half the vector-memory instructions of float32 texels); the recorded NCC and
+// The Gauss-Newton passes read the half-precision gradient field (mgh: two adjacent 16-byte loads
+// a lookup, each a texel and the one below it; half the vector-memory instructions of float32
+// texels); the recorded NCC and
 // the NMI are those of the float32 master (ms, the values the field holds in .w) at the best
 // orientation, from the NMI's first pass (one 8-byte load per texel row pair).
 // MAXP > 0: each lane keeps the master values of its (at most MAXP) pixels at the best
@@ -227,7 +228,7 @@ __device__ __forceinline__ void refine_item(const RefArg& a, const float* __rest
     float3 gbest = make_float3(0.f, 0.f, 0.f);
     float hbest[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float last_step = 0.f, promised = 0.f, step_size = 0.f, accepted = 0.f;
-    const __amdgpu_buffer_rsrc_t fh = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint2*>(mgh), (short)0, 2 * a.side * a.side * 8, 0x00020000);
+    const __amdgpu_buffer_rsrc_t fh = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint2*>(mgh), (short)0, 2 * a.side * a.side * 16, 0x00020000);
     const __amdgpu_buffer_rsrc_t fm = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(ms), (short)0, 2 * a.side * a.side * 4, 0x00020000);
 
     for (int itn = 0; itn <= a.iters; ++itn) {

@@ -166,7 +166,7 @@ class Phase:
     def __init__(self, m):
         self.master = m
         self.baked = {}     # bake key -> (sheets_dev [2, s, s], grad_dev [2, s, s, 4], sheets_host)
-        self.baked_h = {}   # bake key -> grad_dev in half precision [2, s, s, 4] (zb_master_pack)
+        self.baked_h = {}   # bake key -> grad_dev in half precision [2, s, s, 2, 4]: each texel and the one below (zb_master_pack)
         self.bands = {}     # (L, bake key) -> DevicePhase
 
 
@@ -380,7 +380,7 @@ class Engine:
         p = self.phases[pi]
         key, (_, grad, _) = self.baked(pi)
         if key not in p.baked_h:
-            gh = torch.empty((2 * p.master.side * p.master.side * 4,), dtype=torch.float16, device=grad.device)
+            gh = torch.empty((2 * p.master.side * p.master.side * 8,), dtype=torch.float16, device=grad.device)
             nat.check(nat.load().zb_master_pack(grad.data_ptr(), p.master.side, gh.data_ptr(), stream_ptr()), "master pack")
             p.baked_h[key] = gh
         return p.baked_h[key]
