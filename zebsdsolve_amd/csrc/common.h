@@ -258,19 +258,20 @@ __device__ __forceinline__ float4 taps_blend(const Taps<float4>& t) {
     return make_float4(lerpf(top.x, bot.x, ti), lerpf(top.y, bot.y, ti), lerpf(top.z, bot.z, ti), lerpf(top.w, bot.w, ti));
 }
 
+// a + t (b - a) as a - t a + t b: two FMAs on values converted exactly from half precision (the
+// compiler may read the halves in place with mixed-precision FMAs).
+__device__ __forceinline__ float lerp_m(float a, float b, float t) { return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a)); }
+
 __device__ __forceinline__ float4 taps_blend(const TapsH& t) {
-    Taps<float4> f;  // a, b: texels o, o + 1; c, d: the row below
-    float2 lo = unpack_half2(t.left.x), hi = unpack_half2(t.left.y);
-    f.a = make_float4(lo.x, lo.y, hi.x, hi.y);
-    lo = unpack_half2(t.left.z), hi = unpack_half2(t.left.w);
-    f.c = make_float4(lo.x, lo.y, hi.x, hi.y);
-    lo = unpack_half2(t.right.x), hi = unpack_half2(t.right.y);
-    f.b = make_float4(lo.x, lo.y, hi.x, hi.y);
-    lo = unpack_half2(t.right.z), hi = unpack_half2(t.right.w);
-    f.d = make_float4(lo.x, lo.y, hi.x, hi.y);
-    f.tj = t.tj;
-    f.ti = t.ti;
-    return taps_blend(f);
+    // left: texel o then the one below; right: texel o + 1 then the one below
+    const float2 a0 = unpack_half2(t.left.x), a1 = unpack_half2(t.left.y);
+    const float2 c0 = unpack_half2(t.left.z), c1 = unpack_half2(t.left.w);
+    const float2 b0 = unpack_half2(t.right.x), b1 = unpack_half2(t.right.y);
+    const float2 d0 = unpack_half2(t.right.z), d1 = unpack_half2(t.right.w);
+    const float tj = t.tj, ti = t.ti;
+    const float4 top = make_float4(lerp_m(a0.x, b0.x, tj), lerp_m(a0.y, b0.y, tj), lerp_m(a1.x, b1.x, tj), lerp_m(a1.y, b1.y, tj));
+    const float4 bot = make_float4(lerp_m(c0.x, d0.x, tj), lerp_m(c0.y, d0.y, tj), lerp_m(c1.x, d1.x, tj), lerp_m(c1.y, d1.y, tj));
+    return make_float4(lerpf(top.x, bot.x, ti), lerpf(top.y, bot.y, ti), lerpf(top.z, bot.z, ti), lerpf(top.w, bot.w, ti));
 }
 
 template <>
