@@ -47,8 +47,9 @@ def test_clahe_on_the_scan_is_the_references():
     """The reference's CLAHE check on the scan (tests/test_api.py:698-706): pattern 7 after the
     default backgrounds, then CLAHE at three grid / level / clip settings, against the oracle's
     restatement of prep.wgsl applied to the pattern without it (a stretch first, so the
-    normalization before does not matter), within 2e-4 (pixels are ~1/60); then with the mask,
-    which comes last."""
+    normalization before does not matter), within 2e-4 (pixels are ~1/60); also at 16 x 16 tiles
+    of 256 levels, whose histograms (256 KB) pass one block's LDS, so that the filters run through
+    global memory (zb_prep_work); then with the mask, which comes last."""
     from oracle import pipeline as op
     import zebsdsolve_amd as ze
     s = ze.Solver()
@@ -56,7 +57,7 @@ def test_clahe_on_the_scan_is_the_references():
     s.configure(mask=False)
     before = s.pattern(7)
     worst = 0.0
-    for grid, bins, clip in ((4, 64, 40.0), (7, 32, 2.0), (8, 128, 40.0)):
+    for grid, bins, clip in ((4, 64, 40.0), (7, 32, 2.0), (8, 128, 40.0), (16, 256, 40.0)):
         s.configure(clahe=True, clahe_grid=grid, clahe_bins=bins, clahe_clip=clip)
         worst = max(worst, float(np.abs(s.pattern(7) - _unit(op.clahe(op.stretch(before), grid, bins, clip))).max()))
     print(f"CLAHE: largest difference from the oracle {worst:.1e}", flush=True)
