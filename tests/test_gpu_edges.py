@@ -127,18 +127,31 @@ def test_zero_count_calls_return_success():
     assert "bad arguments" in nat.last_error()
 
 
-def test_index_cells_over_two_streams_matches_one():
+@pytest.mark.parametrize("global_filters", [False, True])
+def test_index_cells_over_two_streams_matches_one(global_filters):
     """A request large enough to be cut over two streams (Engine.index_lanes) gives every cell
-    the record one stream gives it."""
+    the record one stream gives it; also with the dynamic background through the filters'
+    global-memory workspace (knob prep_global), which each stream holds for itself."""
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
+    print(f"two streams, filters through global memory {global_filters}", flush=True)
     s = _solver()
     s.synthesize(toy_rows=48, toy_cols=96, toy_grains=400, toy_counts=400, toy_seed=13)
     n = 48 * 96
-    s._e.index_lanes = 1
-    one = s.index_cells(np.arange(n))
-    s._e.index_lanes = 2
-    two = s.index_cells(np.arange(n))
-    s._e.index_lanes = 1
+    prev = lib.zb_knob_set(b"prep_global", int(global_filters))
+    try:
+        if global_filters:
+            s.configure(dynamic_background="subtract")
+        s._e.index_lanes = 1
+        one = s.index_cells(np.arange(n))
+        s._e.index_lanes = 2
+        two = s.index_cells(np.arange(n))
+        s._e.index_lanes = 1
+    finally:
+        lib.zb_knob_set(b"prep_global", prev)
     assert len(s._e._lanes) >= 2
+    if global_filters:
+        assert all("prep_ws" in lane.bufs for lane in s._e._lanes[:2])
     for k in ("quaternions", "ncc", "nmi", "step"):
         assert np.array_equal(one[k], two[k]), (k, np.abs(one[k] - two[k]).max())
 

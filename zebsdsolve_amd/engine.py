@@ -568,9 +568,10 @@ class Engine:
             self._bg = (key, avg)
         return self._bg[1]
 
-    def prep_into(self, raw, n, h, w, out, st):
+    def prep_into(self, raw, n, h, w, out, st, lane=None):
         """Raw patterns [n, h, w] (device) to the normalized ones in `out`: the pattern filters the
-        settings ask for (zb_prep), else the normalization alone (zb_normalize)."""
+        settings ask for (zb_prep_work), else the normalization alone (zb_normalize). On a lane's
+        stream the filters' workspace is the lane's own."""
         lib = nat.load()
         s = self.settings
         mask = (nat.ctypes.c_float * 3)(*self.mask_tuple())
@@ -588,15 +589,18 @@ class Engine:
         o.bandpass = int(bool(s["bandpass"]))
         o.bp_fine, o.bp_coarse = float(s["bp_fine"]), float(s["bp_coarse"])  # pixels of the pattern as stored (binning 1)
         # patterns past one block's LDS (with these filters) go through a workspace in global
-        # memory, kept for the next batch of the same shape
+        # memory, kept for the next batch of the same shape (one per stream: a lane's is its own)
         need = int(lib.zb_prep_workspace(n, h, w, nat.ctypes.byref(o)))
         if need < 0:
             raise nat.NativeError(f"prep: {nat.last_error()}")
         ws, wsz = None, 0
         if need > 0:
-            cur = getattr(self, "_prep_ws", None)
-            if cur is None or cur.numel() < need or cur.device != out.device:
-                cur = self._prep_ws = torch.empty((need,), dtype=torch.uint8, device=out.device)
+            if lane is not None:
+                cur = lane.buf("prep_ws", (need,), torch.uint8)
+            else:
+                cur = getattr(self, "_prep_ws", None)
+                if cur is None or cur.numel() < need or cur.device != out.device:
+                    cur = self._prep_ws = torch.empty((need,), dtype=torch.uint8, device=out.device)
             ws, wsz = cur.data_ptr(), cur.numel()
         nat.check(lib.zb_prep_work(raw.data_ptr(), kind, n, h, w, mask, nat.ctypes.byref(o), ws, wsz, out.data_ptr(), st), "prep")
 
@@ -690,7 +694,7 @@ class Engine:
             items.record_stream(lane.stream)
         off = 0
         for c in chunks:
-            self.prep_into(c, int(c.shape[0]), h, w, pats[off:], st)
+            self.prep_into(c, int(c.shape[0]), h, w, pats[off:], st, lane)
             off += int(c.shape[0])
         sh_flags = self.sh_flags()
         for b0 in range(0, n, big):
@@ -916,6 +920,8 @@ class GraphBatch:
         with torch.cuda.graph(self.g):
             self.records, self.stamp = engine.run_batch(self.raw, self.items, L)
         torch.cuda.synchronize(dev)
+        # the filters' workspace the replays write into (the engine may later replace its own)
+        self._prep_ws = getattr(engine, "_prep_ws", None)
 
     def __call__(self, raw, items):
         """records [phases, n, 8] for raw [n, h, w] and items [n, 4] (device tensors)."""
