@@ -702,9 +702,13 @@ class Engine:
             it = items[b0:b0 + nbt]
             nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nbt, sh_flags, workspace.data_ptr(), workspace.numel(),
                                              alm.data_ptr(), stats.data_ptr(), st), "coefficients")
+            # the batch cut into equal correlation launches (whole 128-item GEMM tiles, at most
+            # `chunk`), so that no launch is a small remainder that leaves most of the GPU idle
+            parts = -(-nbt // chunk)
+            even = min(chunk, -(-(-(-nbt // parts)) // 128) * 128)
             for pi, ph in enumerate(phs):
-                for c0 in range(0, nbt, chunk):
-                    nt = min(chunk, nbt - c0)
+                for c0 in range(0, nbt, even):
+                    nt = min(even, nbt - c0)
                     a_c, p_c = alm[c0:c0 + nt], peaks[pi, c0:c0 + nt]
                     if mfma[pi]:
                         nat.check(lib.zb_vcorr_correlate(vts[pi], band.tw.data_ptr(), a_c.data_ptr(), nt, rdens[pi].data_ptr(), nt,
