@@ -401,3 +401,18 @@ def test_solver_surface_returns_the_reference_shapes():
     assert not s._cancel
     with pytest.raises(ze.Error):
         s.reduce(q, phase=3)
+
+
+def test_correlation_launches_are_equal_whole_tiles():
+    """A batch's correlation launches: the fewest that fit the workspace, of equal whole 128-item
+    tiles (the last one the rest), never past the workspace's items."""
+    from zebsdsolve_amd.engine import correlation_part
+    for n in (1, 129, 1000, 7300, 7301, 7500, 8000, 14700, 16384):
+        for chunk in (100, 1950, 7300, 16384):
+            part = correlation_part(n, chunk)
+            sizes = [min(part, n - c0) for c0 in range(0, n, part)]
+            assert sum(sizes) == n and max(sizes) <= chunk
+            assert len(sizes) == -(-n // chunk), (n, chunk, sizes)
+            if chunk >= 128 and len(sizes) > 1:
+                assert part % 128 == 0
+    assert correlation_part(7500, 7300) == 3840  # not 7300 + 200

@@ -142,6 +142,14 @@ def mfma_correlation(L, n, nc=None, slots=None):
     return (nc is None or nc <= 64) and (slots is None or slots <= 64)
 
 
+def correlation_part(n, chunk):
+    """Items a correlation launch takes when n items are cut into launches of at most `chunk`
+    (the workspace's): the fewest launches, of equal whole 128-item GEMM tiles where the cap
+    allows, so that no launch is a small remainder that leaves most of the GPU idle."""
+    parts = -(-n // chunk)
+    return min(chunk, -(-(-(-n // parts)) // 128) * 128)
+
+
 class Lane:
     """A HIP stream (default priority) with scratch buffers of its own, so that consecutive
     batches run concurrently: one batch's GEMM on the matrix cores beside another's refinement
@@ -702,10 +710,7 @@ class Engine:
             it = items[b0:b0 + nbt]
             nat.check(lib.zb_sh_coefficients(band.c, fr, pats.data_ptr(), it.data_ptr(), nbt, sh_flags, workspace.data_ptr(), workspace.numel(),
                                              alm.data_ptr(), stats.data_ptr(), st), "coefficients")
-            # the batch cut into equal correlation launches (whole 128-item GEMM tiles, at most
-            # `chunk`), so that no launch is a small remainder that leaves most of the GPU idle
-            parts = -(-nbt // chunk)
-            even = min(chunk, -(-(-(-nbt // parts)) // 128) * 128)
+            even = correlation_part(nbt, chunk)
             for pi, ph in enumerate(phs):
                 for c0 in range(0, nbt, even):
                     nt = min(even, nbt - c0)
