@@ -254,8 +254,9 @@ typedef struct zb_refine_phase {
  * device-side queues (one per XCD, over contiguous runs of the phase-major, per-phase ordered
  * items), each block building the masked-pixel list once. Every record is bit-identical to
  * zb_refine's for the same phase (opts.order is ignored: each phase brings its own). work:
- * ZB_REFINE_QUEUE_BYTES of device memory, cleared by a kernel on `stream` at the start of the
- * call (calls that overlap on one work buffer race).
+ * ZB_REFINE_QUEUE_BYTES of device memory for the persistent form (the knob "refine_queue"),
+ * cleared by a kernel on `stream` at the start of the call (calls that overlap on one work
+ * buffer race); the block-per-work-item form leaves it untouched.
  * Replaces the per-phase loop of Detector.refine dispatches (src/engine.zig:2085-2119). */
 int32_t zb_refine_phases(const zb_refine_phase* phases, int32_t n_phases, const zb_frame* frame, const float* pats,
                          const float* items, int32_t n_items, const zb_refine_opts* opts, int32_t out0, void* work,

@@ -439,8 +439,9 @@ __global__ void __launch_bounds__(64) k_refine(RefArg a, const float* __restrict
 // block reaches that exit. Each block builds the masked-pixel list once for all its items.
 // The phases' array pointers are separate restrict kernel arguments (so that the compiler keeps
 // k_refine's register budget: without the no-alias guarantee it holds more values live); their
-// sides, seed strides and orders sit in a table in the work buffer after the queue counters,
-// written with them by k_queue_setup on the same stream before the launch.
+// sides, seed strides and orders come as one more argument to the grid form, and for the queue
+// form sit in a table in the work buffer after the queue counters, written with them by
+// k_queue_setup on the same stream before the launch.
 constexpr int kMaxRefinePhases = 4;
 constexpr int kQueues = 8;
 constexpr int kQueueStride = 16;
@@ -464,12 +465,9 @@ __device__ __forceinline__ T pick(int p, T x0, T x1, T x2, T x3) {
 
 // Work item w of the sequence: phase p, item b (in that phase's order); then the item refined.
 template <int MAXP, bool LIST>
-__device__ __forceinline__ void refine_work(const RefArg& a, const RefPhase* table, unsigned w, int n_items, ZB_PHASE_ARGS(0), ZB_PHASE_ARGS(1),
-                                            ZB_PHASE_ARGS(2), ZB_PHASE_ARGS(3), const float* __restrict__ pats,
-                                            const float4* __restrict__ items, unsigned* hist, const unsigned short* list, int n_px, int lane) {
-    const int p = __builtin_amdgcn_readfirstlane((int)(w / (unsigned)n_items));
-    const int i = __builtin_amdgcn_readfirstlane((int)(w - (unsigned)p * (unsigned)n_items));
-    const RefPhase& e = table[p];
+__device__ __forceinline__ void refine_work(const RefArg& a, RefPhase e, int p, int i, ZB_PHASE_ARGS(0), ZB_PHASE_ARGS(1), ZB_PHASE_ARGS(2),
+                                            ZB_PHASE_ARGS(3), const float* __restrict__ pats, const float4* __restrict__ items, unsigned* hist,
+                                            const unsigned short* list, int n_px, int lane) {
     const int* order = e.order;
     const int b = order ? __builtin_amdgcn_readfirstlane(order[i]) : i;
     RefArg ap = a;
@@ -478,10 +476,16 @@ __device__ __forceinline__ void refine_work(const RefArg& a, const RefPhase* tab
                             __builtin_amdgcn_readfirstlane(e.seed_stride), pick(p, rec0, rec1, rec2, rec3), a.out0, hist, list, n_px, b, lane);
 }
 
+// Work item w of the sequence: phase p and item i (in that phase's order).
+__device__ __forceinline__ void work_item(unsigned w, int n_items, int& p, int& i) {
+    p = __builtin_amdgcn_readfirstlane((int)(w / (unsigned)n_items));
+    i = __builtin_amdgcn_readfirstlane((int)(w - (unsigned)p * (unsigned)n_items));
+}
+
 template <int MAXP, bool LIST>
 __global__ void __launch_bounds__(64) k_refine_grid(RefArg a, ZB_PHASE_ARGS(0), ZB_PHASE_ARGS(1), ZB_PHASE_ARGS(2), ZB_PHASE_ARGS(3),
                                                     const float* __restrict__ pats, const float4* __restrict__ items, int n_items,
-                                                    const unsigned* __restrict__ work) {
+                                                    RefPhases ph) {
     extern __shared__ unsigned lds[];
     unsigned* hist = lds;
     unsigned short* list = LIST ? (unsigned short*)(lds + a.nmi_bins * a.nmi_bins) : nullptr;
@@ -490,9 +494,12 @@ __global__ void __launch_bounds__(64) k_refine_grid(RefArg a, ZB_PHASE_ARGS(0), 
     int n_px = a.h * a.w;
     if constexpr (LIST) n_px = build_list(a, list, lane);
     else __syncthreads();
-    refine_work<MAXP, LIST>(a, (const RefPhase*)((const char*)work + kTableOffset), (unsigned)xcd_block(blockIdx.x, gridDim.x), n_items,
-                            ms0, mgh0, sd0, rec0, ms1, mgh1, sd1, rec1, ms2, mgh2, sd2, rec2, ms3, mgh3, sd3, rec3, pats, items, hist, list, n_px,
-                            lane);
+    int p, i;
+    work_item((unsigned)xcd_block(blockIdx.x, gridDim.x), n_items, p, i);
+    // the phase's entry picked from the argument (an index into it would place it in memory)
+    const RefPhase e = pick(p, ph.p[0], ph.p[1], ph.p[2], ph.p[3]);
+    refine_work<MAXP, LIST>(a, e, p, i, ms0, mgh0, sd0, rec0, ms1, mgh1, sd1, rec1, ms2, mgh2, sd2, rec2, ms3, mgh3, sd3, rec3, pats, items, hist,
+                            list, n_px, lane);
 }
 
 template <int MAXP, bool LIST>
@@ -518,8 +525,10 @@ __global__ void __launch_bounds__(64) k_refine_queue(RefArg a, ZB_PHASE_ARGS(0),
             if (k >= len) break;
             for (int j = lane; j < a.nmi_bins * a.nmi_bins; j += 64) hist[j] = 0u;
             __syncthreads();
-            refine_work<MAXP, LIST>(a, table, lo + k, n_items, ms0, mgh0, sd0, rec0, ms1, mgh1, sd1, rec1, ms2, mgh2, sd2, rec2, ms3, mgh3, sd3,
-                                    rec3, pats, items, hist, list, n_px, lane);
+            int p, i;
+            work_item(lo + k, n_items, p, i);
+            refine_work<MAXP, LIST>(a, table[p], p, i, ms0, mgh0, sd0, rec0, ms1, mgh1, sd1, rec1, ms2, mgh2, sd2, rec2, ms3, mgh3, sd3, rec3, pats,
+                                    items, hist, list, n_px, lane);
             __syncthreads();  // every lane done with the histogram before the next item clears it
         }
     }
@@ -856,8 +865,8 @@ extern "C" int32_t zb_refine_phases(const zb_refine_phase* phases, int32_t n_pha
     // 0: the list and each lane's master values kept between the NMI's passes; 1: the list, the
     // values looked up again (fewer registers: with the queue knob at 2); 2: the whole frame
     const int kind = (a.use_list && n_px <= 64 * 48 && knob(KNOB_REFINE_QUEUE) != 2) ? 0 : a.use_list ? 1 : 2;
-    ZB_LAUNCH("k_queue_setup", k_queue_setup, dim3(1), dim3(64), 0, st, ph, n_phases, queues);
-    if (int rc = check_launch("zb_refine_phases/setup")) return rc;
+    // phases past n_phases repeat the last one's entry (never read)
+    for (int p = n_phases; p < kMaxRefinePhases; ++p) ph.p[p] = ph.p[n_phases - 1];
     // the phases' arrays as kernel arguments (phases past n_phases repeat the last: never read)
     const float* ms[kMaxRefinePhases];
     const uint2* mgh[kMaxRefinePhases];
@@ -872,6 +881,9 @@ extern "C" int32_t zb_refine_phases(const zb_refine_phase* phases, int32_t n_pha
     }
 #define ZB_PHASES ms[0], mgh[0], sd[0], rec[0], ms[1], mgh[1], sd[1], rec[1], ms[2], mgh[2], sd[2], rec[2], ms[3], mgh[3], sd[3], rec[3]
     if (knob(KNOB_REFINE_QUEUE)) {
+        // the queue counters cleared and the phase table written into the work buffer
+        ZB_LAUNCH("k_queue_setup", k_queue_setup, dim3(1), dim3(64), 0, st, ph, n_phases, queues);
+        if (int rc = check_launch("zb_refine_phases/setup")) return rc;
         // as many blocks as stay resident at once (more would only wait for the queues to empty)
         const int resident = queue_blocks(kind, lds);
         if (resident < 1) {
@@ -890,12 +902,12 @@ extern "C" int32_t zb_refine_phases(const zb_refine_phase* phases, int32_t n_pha
                       n_items, queues);
     } else {
         if (kind == 0)
-            ZB_LAUNCH("k_refine", (k_refine_grid<48, true>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items, queues);
+            ZB_LAUNCH("k_refine", (k_refine_grid<48, true>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items, ph);
         else if (kind == 1)
-            ZB_LAUNCH("k_refine", (k_refine_grid<0, true>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items, queues);
+            ZB_LAUNCH("k_refine", (k_refine_grid<0, true>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items, ph);
         else
             ZB_LAUNCH("k_refine", (k_refine_grid<0, false>), dim3(total), dim3(64), lds, st, a, ZB_PHASES, pats, (const float4*)items, n_items,
-                      queues);
+                      ph);
     }
 #undef ZB_PHASES
     return check_launch("zb_refine_phases");
