@@ -79,7 +79,22 @@ struct VArg {
     int jg, jp;
     int lp_gs;  // 16-bit products: the first group of 8 slots that takes the hi + lo product
     int vk;     // V's elements: 0 FP32 pairs (cols floats a row), 1 FP16 / 2 BF16 pairs (nc a row)
+    // the items' best (value, index) keys, which the correlation's GEMM clears (its blocks of
+    // m' = 0 and column group 0, one per item tile) for the peak search after it; null: none
+    unsigned long long* best;
 };
+
+// The keys of this wave's item tile cleared by the blocks that hold m' = 0 and column group 0.
+template <int MB>
+__device__ __forceinline__ void clear_best(const VArg& a, int mp, int cg, int tile, int half, int r32) {
+    if (a.best && mp == 0 && cg == 0 && half == 0) {
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            const int item = (tile * MB + mb) * 32 + r32;
+            if (item < a.n_items) a.best[item] = 0ull;
+        }
+    }
+}
 
 // V's complex elements: FP32 (re, im) (8 bytes) or FP16 / BF16 pairs packed in 32 bits
 template <int VT>
@@ -293,6 +308,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_vgemm(VArg a, const float* __re
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int r32 = lane & 31;
+    clear_best<MB>(a, mp, cg, tile, half, r32);
     const int s0 = (mp + a.l_step - 1) / a.l_step;
     const int pbase = mp * a.L - (mp * (mp - 1)) / 2 - mp;  // pair(mp, l) = pbase + l
     // B: rows k = (re, im) of one degree slot, columns = items; zero outside [s0, slots) and
@@ -488,6 +504,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_vgemm16(VArg a, const floatx4* 
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int r32 = lane & 31;
+    clear_best<MB>(a, mp, cg, tile, half, r32);
     const int s0 = (mp + a.l_step - 1) / a.l_step;
     const int pbase = mp * a.L - (mp * (mp - 1)) / 2 - mp;
     // A: row = item, the 8 k of this lane half = component `half` of slots 8 g .. 8 g + 7, as
@@ -865,11 +882,6 @@ __global__ void __launch_bounds__(256, 2) k_apeak128(VArg a, const void* __restr
     }
 }
 
-__global__ void k_vclear(unsigned long long* p, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0ull;
-}
-
 __device__ __forceinline__ float vertex_v(float lo, float mid, float hi) {
     const float den = lo - 2.f * mid + hi;
     if (den >= -1e-12f) return 0.f;
@@ -984,6 +996,7 @@ static int v_arg(const zb_vtable* t, int n_items, VArg& a, const char* who) {
     }
     a.lp_gs = t->lp_kind ? t->lp_split / 8 : 0;
     a.vk = t->lp_kind;
+    a.best = nullptr;
     a.L = t->L;
     a.n = t->n;
     a.log2n = ilog2v(t->n);
@@ -1208,7 +1221,7 @@ extern "C" int32_t zb_vcorr_correlate(const zb_vtable* t, const float* twiddle, 
     hipStream_t s = (hipStream_t)stream;
     float* v = (float*)work;
     unsigned long long* best = (unsigned long long*)((char*)work + al256(v_bytes(a)));
-    ZB_LAUNCH("k_vclear", k_vclear, dim3((n_items + 255) / 256), dim3(256), 0, s, best, n_items);
+    a.best = best;  // cleared by the GEMM, before the peak search's atomics
     if (int rc = t->lp_kind ? launch_vgemm16(a, t->lp_kind, t->qlp, alm, v, s) : launch_vgemm(a, t->q, alm, v, s)) return rc;
     if (int rc = launch_apeak<0>(a, v, rden, rden_div, best, nullptr, s)) return rc;
     if (a.vk == 0)

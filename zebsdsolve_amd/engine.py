@@ -914,7 +914,7 @@ class GraphBatch:
     own buffers, the captured launches replay on the current stream, the records land in the
     graph's own buffer (valid until the next replay). Built by one eager run (which makes every
     table and volume the batch needs) and one capture; the library's entry points allocate,
-    clear (k_vclear is a kernel, no memset node) and synchronize nothing, so all of the batch is
+    clear (with kernels, never a memset node) and synchronize nothing, so all of the batch is
     in the graph. The records carry the stamp of the capture: a replay's records are checked
     against it (Engine.index_cells' guard does not apply here)."""
 
