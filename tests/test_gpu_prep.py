@@ -274,3 +274,32 @@ def test_240x320_toy_scan_through_the_default_filters():
     r = s.index()
     d = np.degrees(s.disorientation(r["quaternions"].reshape(-1, 4), truth["quaternions"].reshape(-1, 4)))
     assert np.median(d) < 0.15 and d.max() < 1.0, (np.median(d), d.max())
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+def test_normalize_of_every_pixel_type_and_alignment_matches_the_oracle(dtype):
+    """zb_normalize for u8, u16 and f32 pixels: four pixels a load where the frame's pixel count
+    is a multiple of 4 and the buffers are aligned (60 x 60, 120 x 120), one pixel a load
+    otherwise (45 x 70, 121 x 130, and a raw buffer one pixel off its alignment), all against the
+    float64 oracle."""
+    import torch
+    from oracle import pipeline as op
+    from zebsdsolve_amd import _native as nat, engine
+    lib = nat.load()
+    kind = {np.float32: 2, np.uint16: 1, np.uint8: 0}[dtype]
+    for h, w, skew in ((60, 60, 0), (120, 120, 0), (45, 70, 0), (121, 130, 0), (60, 60, 1)):
+        print(f"normalize {np.dtype(dtype).name} {h}x{w} skew {skew}", flush=True)
+        n = 3
+        raw = _patterns(n, h, w, seed=h * w + kind).astype(dtype)
+        fr = op.frame(np.eye(3), h, w)
+        mask = (nat.ctypes.c_float * 3)(*[float(x) for x in fr["mask"]])
+        flat = np.concatenate([np.zeros(skew, dtype), raw.reshape(-1)])
+        t_raw = torch.from_numpy(flat).cuda()
+        out = torch.empty((n, h, w), dtype=torch.float32, device="cuda")
+        ptr = t_raw.data_ptr() + skew * np.dtype(dtype).itemsize
+        nat.check(lib.zb_normalize(ptr, kind, n, h, w, mask, out.data_ptr(), engine.stream_ptr()), "normalize")
+        got = out.cpu().numpy()
+        for b in range(n):
+            ref = op.normalize(raw[b].astype(np.float64), fr["mask"])
+            err = np.abs(got[b] - ref).max() / np.abs(ref).max()
+            assert err < 2e-5, (h, w, skew, b, err)
