@@ -135,6 +135,42 @@ def test_coefficients(case):
         assert abs(stats[b, 2] - st[2]) < 1e-4 * st[2]
 
 
+@pytest.mark.parametrize("Lx", [128, 160])
+def test_coefficients_at_large_bandwidths(case, Lx):
+    """The coefficients at bandwidths past the unrolled forms of the sphere stage: L = 128 (2 x 256
+    rings of 512 points; the unit norm over a thread's coefficients read twice, past the forms
+    that keep them in registers) and L = 160 (2 x 512 rings of 1 024 points, the ring FFT at a
+    line length given at run time), against the oracle."""
+    import torch
+    from zebsdsolve_amd import _native as nat
+    op, e = case["op"], case["e"]
+    print(f"coefficients at L = {Lx}", flush=True)
+    lib = nat.load()
+    band = e.band(Lx)
+    dev = torch.device("cuda")
+    k = 2
+    raw = torch.from_numpy(case["pats"][:k]).cuda()
+    pats = torch.empty_like(raw)
+    mask = (nat.ctypes.c_float * 3)(*e.mask_tuple())
+    nat.check(lib.zb_normalize(raw.data_ptr(), 2, k, 60, 60, mask, pats.data_ptr(), engine_stream()), "normalize")
+    t = e.positions(np.arange(k))
+    items = torch.from_numpy(np.concatenate([t, np.arange(k, dtype=np.float32)[:, None]], 1)).to(dev)
+    alm = torch.empty((k, band.host.pairs, 2), dtype=torch.float32, device=dev)
+    stats = torch.empty((k, 4), dtype=torch.float32, device=dev)
+    ws = torch.empty((lib.zb_sh_workspace(band.c, k),), dtype=torch.uint8, device=dev)
+    nat.check(lib.zb_sh_coefficients(band.c, e.frame(), pats.data_ptr(), items.data_ptr(), k, 3, ws.data_ptr(), ws.numel(), alm.data_ptr(),
+                                     stats.data_ptr(), engine_stream()), "coefficients")
+    a = alm.cpu().numpy()
+    got = a[..., 0] + 1j * a[..., 1]
+    pn = pats.cpu().numpy()
+    for b in range(k):
+        ref, st = op.coefficients(pn[b].astype(np.float64), case["fr"], case["t"], Lx)
+        err = np.linalg.norm(got[b] - ref) / np.linalg.norm(ref)
+        print(f"L = {Lx}, pattern {b}: relative error {err:.2e}", flush=True)
+        assert err < 5e-5, (Lx, b, err)
+        assert abs(stats[b, 2] - st[2]) < 1e-4 * st[2]
+
+
 def test_denominators_and_correlation(case):
     from zebsdsolve_amd.tables import volume_natural
     op, e = case["op"], case["e"]
