@@ -201,3 +201,24 @@ def test_round16_is_nearest_even():
     assert hv[0] == 1.0 and hv[1] == 65504.0
     with pytest.raises(ValueError):
         tables.round16(np.array([70000.0]), 1)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libzebsd_hip.so not built")
+def test_every_documented_knob_has_its_documented_range_and_default():
+    """The knobs include/zebsd.h lists ("name" lo..hi (default)) are the library's: each reads its
+    default, takes both ends of its range and refuses the value past them; nothing else is
+    settable under another name."""
+    from zebsdsolve_amd import _native as nat
+    lib = nat.load()
+    text = open(HEADER).read()
+    knobs = re.findall(r'^\s*\*\s+"(\w+)"\s+(-?\d+)\.\.(-?\d+)\s+\((-?\d+)\)', text, flags=re.M)
+    assert {k[0] for k in knobs} >= {"refine_queue", "rings_group", "prep_global"}
+    for name, lo, hi, default in knobs:
+        key, lo, hi, default = name.encode(), int(lo), int(hi), int(default)
+        assert lib.zb_knob_get(key) == default, name
+        try:
+            assert lib.zb_knob_set(key, hi) == default, name
+            assert lib.zb_knob_set(key, lo) == hi, name
+            assert lib.zb_knob_set(key, hi + 1) == -1 and lib.zb_knob_get(key) == lo, name
+        finally:
+            lib.zb_knob_set(key, default)
